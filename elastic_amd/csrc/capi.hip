@@ -1,0 +1,575 @@
+// C-ABI of the library (include/elastic_hip.h): contexts, host set-up,
+// system upload, products, solves and the drop-in for the reference's solv.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+
+#include "elastic_internal.h"
+
+
+namespace elastic {
+
+int set_error(Context* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+template <typename T>
+static int dalloc(Context* c, T** p, size_t n) {
+  if (*p) {
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(c, ELASTIC_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return ELASTIC_OK;
+}
+
+template <typename T>
+static int upload(Context* c, T** p, const T* host, size_t n) {
+  int rc = dalloc(c, p, n);
+  if (rc) return rc;
+  if (n) ELASTIC_HIP_CHECK(c, hipMemcpyAsync(*p, host, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return ELASTIC_OK;
+}
+
+static void free_tiles(DevTiles& t) {
+  void* ptrs[] = {t.row0, t.nrows, t.nnz, t.ncols, t.ext0, t.nz0, t.rp0, t.seg0, t.word,
+                  t.rowptr, t.segptr, t.ext, t.inc_ptr, t.inc_idx};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  t = DevTiles();
+}
+
+static void free_system(Context* c) {
+  free_tiles(c->tiles);
+  void* ptrs[] = {c->A64.ad, c->A64.al, c->A64.m, c->A64.bm, c->A32.ad, c->A32.al,
+                  c->A32.m,  c->A32.bm, c->epart64, c->epart32};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  c->A64 = DevMatrix<double>();
+  c->A32 = DevMatrix<float>();
+  c->epart64 = nullptr;
+  c->epart32 = nullptr;
+  for (auto& v : c->v64)
+    if (v) hipFree(v), v = nullptr;
+  for (auto& v : c->v32)
+    if (v) hipFree(v), v = nullptr;
+  c->values_set = c->values32 = c->bdiag_set = false;
+}
+
+}  // namespace elastic
+
+using namespace elastic;
+
+// vector slots
+enum { VX = 0, VR, VP, VQ, VB, VT, VU, VW };
+
+extern "C" {
+
+int elastic_create(elastic_ctx** out, int32_t device) {
+  if (!out) return ELASTIC_BAD_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ELASTIC_HIP_ERROR;
+  if (device < 0 || device >= ndev) return ELASTIC_BAD_ARG;
+  if (hipSetDevice(device) != hipSuccess) return ELASTIC_HIP_ERROR;
+  auto* x = new elastic_ctx();
+  Context* c = &x->c;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
+      hipMalloc((void**)&c->scal, sizeof(SolveScalars)) != hipSuccess ||
+      hipHostMalloc((void**)&c->scal_host, sizeof(SolveScalars), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void**)&c->red64, sizeof(double) * (8 * kRedBlocks + 64)) != hipSuccess) {
+    delete x;
+    return ELASTIC_HIP_ERROR;
+  }
+  const char* ce = getenv("ELASTIC_CHECK_EVERY");
+  if (ce) c->check_every = std::max(1, atoi(ce));
+  *out = x;
+  return ELASTIC_OK;
+}
+
+int elastic_destroy(elastic_ctx* x) {
+  if (!x) return ELASTIC_OK;
+  Context* c = &x->c;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_system(c);
+  if (c->scal) hipFree(c->scal);
+  if (c->scal_host) hipHostFree(c->scal_host);
+  if (c->red64) hipFree(c->red64);
+  for (hipEvent_t e : {c->ev0, c->ev1, c->ev2, c->ev3})
+    if (e) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete x;
+  return ELASTIC_OK;
+}
+
+const char* elastic_last_error(const elastic_ctx* x) { return x ? x->c.err.c_str() : "null context"; }
+
+// ------------------------------------------------------------ host set-up
+int elastic_reord(int32_t nnode, int32_t numel, int32_t nen, const int32_t* ix, int32_t* perm) {
+  if (nnode <= 0 || numel < 0 || nen <= 0 || !ix || !perm) return ELASTIC_BAD_ARG;
+  try {
+    rcm_order(nnode, numel, nen, ix, perm);
+  } catch (const std::bad_alloc&) {
+    return ELASTIC_OOM;
+  }
+  return ELASTIC_OK;
+}
+
+int elastic_numeq(int32_t nnode, int32_t ndf, const int32_t* perm, int32_t* id, int32_t* neq) {
+  if (nnode <= 0 || ndf <= 0 || !perm || !id || !neq) return ELASTIC_BAD_ARG;
+  *neq = number_equations(id, perm, nnode, ndf);
+  return ELASTIC_OK;
+}
+
+int elastic_csrstruct(int32_t nnode, int32_t numel, int32_t nen, int32_t ndf, int32_t neq,
+                      const int32_t* ix, const int32_t* id, int32_t* ia, int32_t* ja,
+                      int64_t* nad) {
+  if (nnode <= 0 || neq < 0 || !ix || !id || !ia || !nad) return ELASTIC_BAD_ARG;
+  try {
+    std::vector<int32_t> vi, vj;
+    csrc_pattern(id, ix, nnode, numel, nen, ndf, neq, vi, vj);
+    std::copy(vi.begin(), vi.end(), ia);
+    *nad = (int64_t)vj.size();
+    if (ja) std::copy(vj.begin(), vj.end(), ja);
+  } catch (const std::bad_alloc&) {
+    return ELASTIC_OOM;
+  }
+  return ELASTIC_OK;
+}
+
+// ----------------------------------------------------------- device system
+int elastic_set_tile_nnz(elastic_ctx* x, int32_t tile_nnz) {
+  if (!x || tile_nnz < 64 || tile_nnz > kTileColsMax - kTileRowsMax) return ELASTIC_BAD_ARG;
+  x->c.tile_nnz = tile_nnz;
+  return ELASTIC_OK;
+}
+
+int elastic_set_structure(elastic_ctx* x, int32_t neq, int64_t nad, const int32_t* ia,
+                          const int32_t* ja) {
+  if (!x || neq <= 0 || nad < 0 || !ia || (nad > 0 && !ja)) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  hipSetDevice(c->device);
+  if (ia[0] != 1 || (int64_t)ia[neq] - 1 != nad)
+    return set_error(c, ELASTIC_BAD_ARG, "ia must start at 1 and end at nad+1");
+  for (int i = 0; i < neq; ++i) {
+    if (ia[i + 1] < ia[i]) return set_error(c, ELASTIC_BAD_ARG, "ia not monotone");
+    for (int64_t k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
+      if (ja[k] < 1 || ja[k] > i || (k > ia[i] - 1 && ja[k] <= ja[k - 1]))
+        return set_error(c, ELASTIC_BAD_ARG, "ja must be strictly lower and ascending per row");
+  }
+  hipStreamSynchronize(c->stream);
+  free_system(c);
+  try {
+    build_tiles(neq, ia, ja, c->tile_nnz, c->layout);
+  } catch (const std::exception& e) {
+    return set_error(c, ELASTIC_BAD_ARG, e.what());
+  }
+  x->ia.assign(ia, ia + neq + 1);
+  x->ja.assign(ja, ja + nad);
+  c->neq = neq;
+  c->nad = nad;
+  TileLayout& L = c->layout;
+  DevTiles& t = c->tiles;
+  int rc = 0;
+  rc = rc ? rc : upload(c, &t.row0, L.row0.data(), L.row0.size());
+  rc = rc ? rc : upload(c, &t.nrows, L.nrows.data(), L.nrows.size());
+  rc = rc ? rc : upload(c, &t.nnz, L.nnz.data(), L.nnz.size());
+  rc = rc ? rc : upload(c, &t.ncols, L.ncols.data(), L.ncols.size());
+  rc = rc ? rc : upload(c, &t.ext0, L.ext0.data(), L.ext0.size());
+  rc = rc ? rc : upload(c, &t.nz0, L.nz0.data(), L.nz0.size());
+  rc = rc ? rc : upload(c, &t.rp0, L.rp0.data(), L.rp0.size());
+  rc = rc ? rc : upload(c, &t.seg0, L.seg0.data(), L.seg0.size());
+  rc = rc ? rc : upload(c, &t.word, L.word.data(), L.word.size());
+  rc = rc ? rc : upload(c, &t.rowptr, L.rowptr.data(), L.rowptr.size());
+  rc = rc ? rc : upload(c, &t.segptr, L.segptr.data(), L.segptr.size());
+  rc = rc ? rc : upload(c, &t.ext, L.ext.data(), L.ext.size());
+  rc = rc ? rc : upload(c, &t.inc_ptr, L.inc_ptr.data(), L.inc_ptr.size());
+  rc = rc ? rc : upload(c, &t.inc_idx, L.inc_idx.data(), L.inc_idx.size());
+  t.ntiles = L.ntiles;
+  t.nnz_max = L.nnz_max;
+  t.ncols_max = L.ncols_max;
+  t.next_total = L.next_total;
+  rc = rc ? rc : dalloc(c, &c->A64.ad, neq);
+  rc = rc ? rc : dalloc(c, &c->A64.al, (size_t)nad);
+  rc = rc ? rc : dalloc(c, &c->A64.m, neq);
+  rc = rc ? rc : dalloc(c, &c->A32.ad, neq);
+  rc = rc ? rc : dalloc(c, &c->A32.al, (size_t)nad);
+  rc = rc ? rc : dalloc(c, &c->A32.m, neq);
+  rc = rc ? rc : dalloc(c, &c->epart64, (size_t)L.next_total);
+  rc = rc ? rc : dalloc(c, &c->epart32, (size_t)L.next_total);
+  for (auto& v : c->v64) rc = rc ? rc : dalloc(c, &v, neq);
+  for (auto& v : c->v32) rc = rc ? rc : dalloc(c, &v, neq);
+  if (rc) {
+    free_system(c);
+    return rc;
+  }
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  // the host layout is no longer needed except its sizes
+  L.word.clear();
+  L.word.shrink_to_fit();
+  return ELASTIC_OK;
+}
+
+int elastic_set_values(elastic_ctx* x, const double* ad, const double* al) {
+  if (!x || !ad || (x->c.nad > 0 && !al)) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  if (!c->tiles.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
+  hipSetDevice(c->device);
+  c->zero_diag_row = -1;
+  for (int i = 0; i < c->neq; ++i)
+    if (ad[i] == 0.0) {
+      c->zero_diag_row = i + 1;
+      break;
+    }
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->A64.ad, ad, sizeof(double) * c->neq, hipMemcpyHostToDevice, c->stream));
+  if (c->nad)
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->A64.al, al, sizeof(double) * c->nad, hipMemcpyHostToDevice, c->stream));
+  launch_recip<double>(c->neq, c->A64.ad, c->A64.m, c->stream);
+  launch_cast<double, float>(c->neq, c->A64.ad, c->A32.ad, c->stream);
+  launch_cast<double, float>(c->nad, c->A64.al, c->A32.al, c->stream);
+  launch_cast<double, float>(c->neq, c->A64.m, c->A32.m, c->stream);
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  c->values_set = true;
+  c->bdiag_set = false;
+  return ELASTIC_OK;
+}
+
+int elastic_get_values(elastic_ctx* x, double* ad, double* al) {
+  if (!x || !ad) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(ad, c->A64.ad, sizeof(double) * c->neq, hipMemcpyDeviceToHost, c->stream));
+  if (al && c->nad)
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(al, c->A64.al, sizeof(double) * c->nad, hipMemcpyDeviceToHost, c->stream));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return ELASTIC_OK;
+}
+
+static int spmv_resident(Context* c, int precision, const double* x64, double* y64) {
+  if (precision == ELASTIC_MIXED) {
+    launch_cast<double, float>(c->neq, x64, c->v32[VX], c->stream);
+    launch_spmv_tiles<float>(c->tiles, c->A32.ad, c->A32.al, c->v32[VX], c->v32[VQ], c->epart32,
+                             c->stream);
+    launch_merge<float>(c->tiles, c->neq, c->epart32, c->v32[VQ], c->stream);
+    launch_cast<float, double>(c->neq, c->v32[VQ], y64, c->stream);
+  } else {
+    launch_spmv_tiles<double>(c->tiles, c->A64.ad, c->A64.al, x64, y64, c->epart64, c->stream);
+    launch_merge<double>(c->tiles, c->neq, c->epart64, y64, c->stream);
+  }
+  return ELASTIC_OK;
+}
+
+int elastic_spmv(elastic_ctx* x, int32_t precision, const double* xv, double* yv) {
+  if (!x || !xv || !yv) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
+  hipSetDevice(c->device);
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->v64[VT], xv, sizeof(double) * c->neq, hipMemcpyHostToDevice, c->stream));
+  spmv_resident(c, precision, c->v64[VT], c->v64[VU]);
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(yv, c->v64[VU], sizeof(double) * c->neq, hipMemcpyDeviceToHost, c->stream));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  ELASTIC_HIP_CHECK(c, hipGetLastError());
+  return ELASTIC_OK;
+}
+
+int elastic_time_spmv(elastic_ctx* x, int32_t precision, int32_t reps, double* total_ms,
+                      double* kernel_ms) {
+  if (!x || reps <= 0) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
+  hipSetDevice(c->device);
+  const bool f32 = precision == ELASTIC_MIXED;
+  auto tiles = [&]() {
+    if (f32)
+      launch_spmv_tiles<float>(c->tiles, c->A32.ad, c->A32.al, c->v32[VX], c->v32[VQ], c->epart32,
+                               c->stream);
+    else
+      launch_spmv_tiles<double>(c->tiles, c->A64.ad, c->A64.al, c->v64[VT], c->v64[VU],
+                                c->epart64, c->stream);
+  };
+  auto merge = [&]() {
+    if (f32) launch_merge<float>(c->tiles, c->neq, c->epart32, c->v32[VQ], c->stream);
+    else launch_merge<double>(c->tiles, c->neq, c->epart64, c->v64[VU], c->stream);
+  };
+  for (int i = 0; i < 3; ++i) tiles(), merge();  // warm-up
+  ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
+  for (int i = 0; i < reps; ++i) tiles();
+  ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
+  for (int i = 0; i < reps; ++i) tiles(), merge();
+  ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev2, c->stream));
+  ELASTIC_HIP_CHECK(c, hipEventSynchronize(c->ev2));
+  float a = 0, b = 0;
+  hipEventElapsedTime(&a, c->ev0, c->ev1);
+  hipEventElapsedTime(&b, c->ev1, c->ev2);
+  if (kernel_ms) *kernel_ms = a / reps;
+  if (total_ms) *total_ms = b / reps;
+  ELASTIC_HIP_CHECK(c, hipGetLastError());
+  return ELASTIC_OK;
+}
+
+int elastic_layout_info(elastic_ctx* x, int64_t* info) {
+  if (!x || !info) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  info[0] = c->tiles.ntiles;
+  info[1] = c->tiles.nnz_max;
+  info[2] = c->tiles.next_total;
+  info[3] = c->tiles.ncols_max;
+  return ELASTIC_OK;
+}
+
+int elastic_upload_rhs(elastic_ctx* x, const double* b) {
+  if (!x || !b) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  if (!c->tiles.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
+  hipSetDevice(c->device);
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->v64[VB], b, sizeof(double) * c->neq, hipMemcpyHostToDevice, c->stream));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return ELASTIC_OK;
+}
+
+int elastic_download_x(elastic_ctx* x, double* xv) {
+  if (!x || !xv) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(xv, c->v64[VX], sizeof(double) * c->neq, hipMemcpyDeviceToHost, c->stream));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return ELASTIC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ solves
+namespace elastic {
+int solve_krylov_other(Context* c, int solver, int precond, int precision, double tol, int maxit,
+                       int ngram, elastic_stats* st);
+int ensure_bdiag(elastic_ctx* x);
+
+// statistics of Iters.f:435-457 on the resident x (FP64 operator)
+static int post_stats(Context* c, elastic_stats* st) {
+  double* out = c->red64 + 8 * kRedBlocks;
+  launch_spmv_tiles<double>(c->tiles, c->A64.ad, c->A64.al, c->v64[VX], c->v64[VQ], c->epart64,
+                            c->stream);
+  launch_post(c->tiles, c->neq, c->epart64, c->v64[VX], c->v64[VQ], c->v64[VB], c->A64.m, c->red64,
+              out, c->stream);
+  double h[4];
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  st->xkx = h[0];
+  st->norm_x = std::sqrt(h[1]);
+  st->norm_m_r = std::sqrt(std::fabs(h[2]));
+  st->norm_r = std::sqrt(h[3]);
+  return ELASTIC_OK;
+}
+
+static int solve_pcg(Context* c, int precond, int precision, double tol, int maxit,
+                     elastic_stats* st) {
+  hipStream_t s = c->stream;
+  double* dslots = c->red64 + 8 * kRedBlocks + 8;
+  const bool bd = precond == ELASTIC_PRECOND_BDIAG3;
+  int total_it = 0, outer = 0, spmvs = 0;
+  ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev0, s));
+  if (precision != ELASTIC_MIXED) {
+    PcgBuffers<double> B{c->v64[VX], c->v64[VR], c->v64[VP], c->v64[VQ], c->v64[VB],
+                         c->A64.m,   bd ? c->A64.bm : nullptr, c->A64.ad, c->A64.al, c->epart64};
+    int rc = pcg_device<double>(c, B, tol, maxit, dslots);
+    if (rc) return rc;
+    total_it = c->scal_host->iter;
+    spmvs = total_it;
+    st->conv = c->scal_host->conv;
+    outer = 1;
+    if (!c->scal_host->done) st->status = ELASTIC_MAXIT;
+  } else {
+    // FP64 iterative refinement around FP32 PCG (inner relative tolerance)
+    double inner_tol = 1e-5;
+    if (const char* e = getenv("ELASTIC_INNER_TOL")) inner_tol = atof(e);
+    int max_outer = 60;
+    double* r64 = c->v64[VR];
+    double* q64 = c->v64[VQ];
+    // x64 = 0 ; r64 = b ; b32 = b ; b.M b
+    ELASTIC_HIP_CHECK(c, hipMemsetAsync(c->v64[VX], 0, sizeof(double) * c->neq, s));
+    ELASTIC_HIP_CHECK(c, hipMemsetAsync(q64, 0, sizeof(double) * c->neq, s));
+    ELASTIC_HIP_CHECK(c, hipMemsetAsync(c->epart64, 0, sizeof(double) * std::max<int64_t>(1, c->tiles.next_total), s));
+    launch_ir_residual(c->tiles, c->neq, c->epart64, q64, c->v64[VB], c->A64.m, r64,
+                       c->v32[VB], c->red64, s);
+    double* out = c->red64 + 8 * kRedBlocks;
+    launch_sum_parts(c->red64, 1, out, s);
+    double bmb = 0;
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(&bmb, out, sizeof(double), hipMemcpyDeviceToHost, s));
+    ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
+    const double conv = tol * std::sqrt(std::fabs(bmb));
+    st->conv = conv;
+    double rmr = bmb;
+    bool ok = (bmb == 0.0);
+    while (!ok && outer < max_outer && total_it < maxit) {
+      PcgBuffers<float> B{c->v32[VX], c->v32[VR], c->v32[VP], c->v32[VQ], c->v32[VB],
+                          c->A32.m,   bd ? c->A32.bm : nullptr, c->A32.ad, c->A32.al, c->epart32};
+      int rc = pcg_device<float>(c, B, inner_tol, maxit - total_it, dslots);
+      if (rc) return rc;
+      total_it += c->scal_host->iter;
+      spmvs += c->scal_host->iter;
+      ++outer;
+      launch_ir_correct(c->neq, c->v32[VX], c->v64[VX], s);
+      launch_spmv_tiles<double>(c->tiles, c->A64.ad, c->A64.al, c->v64[VX], q64, c->epart64, s);
+      ++spmvs;
+      launch_ir_residual(c->tiles, c->neq, c->epart64, q64, c->v64[VB], c->A64.m, r64,
+                         c->v32[VB], c->red64, s);
+      launch_sum_parts(c->red64, 1, out, s);
+      ELASTIC_HIP_CHECK(c, hipMemcpyAsync(&rmr, out, sizeof(double), hipMemcpyDeviceToHost, s));
+      ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
+      ok = std::sqrt(std::fabs(rmr)) < conv;
+      if (c->scal_host->iter == 0 && !ok) break;  // no progress possible
+    }
+    if (!ok) st->status = ELASTIC_MAXIT;
+  }
+  ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev1, s));
+  ELASTIC_HIP_CHECK(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  st->t_device_ms = ms;
+  st->iters = total_it;
+  st->outer = outer;
+  st->spmv_count = spmvs;
+  int rc = post_stats(c, st);
+  if (rc) return rc;
+  st->rel_res = st->conv > 0 ? st->norm_m_r / (st->conv / tol) : 0.0;
+  ELASTIC_HIP_CHECK(c, hipGetLastError());
+  return ELASTIC_OK;
+}
+
+int solve_resident(elastic_ctx* x, int solver, int precond, int precision, double tol, int maxit,
+                   int ngram, elastic_stats* st) {
+  Context* c = &x->c;
+  if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
+  if (c->zero_diag_row > 0)
+    return set_error(c, ELASTIC_ZERO_DIAG,
+                     "zero diagonal coefficient at equation " + std::to_string(c->zero_diag_row));
+  if (tol <= 0.0) tol = 2.220446049250313e-16;  // tol 0 -> machine epsilon (Rdata.f:2056)
+  if (maxit <= 0) return set_error(c, ELASTIC_BAD_ARG, "maxit must be positive");
+  if (precond == ELASTIC_PRECOND_BDIAG3) {
+    int rc = ensure_bdiag(x);
+    if (rc) return rc;
+  } else if (precond != ELASTIC_PRECOND_DIAG) {
+    return set_error(c, ELASTIC_BAD_ARG, "preconditioner must be 2 (diag) or 6 (3x3 blocks)");
+  }
+  memset(st, 0, sizeof(*st));
+  if (solver == ELASTIC_SOLVER_PCG) return solve_pcg(c, precond, precision, tol, maxit, st);
+  return solve_krylov_other(c, solver, precond, precision, tol, maxit, ngram, st);
+}
+
+}  // namespace elastic
+
+extern "C" {
+
+int elastic_solve_resident(elastic_ctx* x, int32_t solver, int32_t precond, int32_t precision,
+                           double tol, int32_t maxit, int32_t ngram, elastic_stats* st) {
+  if (!x) return ELASTIC_BAD_ARG;
+  elastic_stats local;
+  if (!st) st = &local;
+  hipSetDevice(x->c.device);
+  int rc = solve_resident(x, solver, precond, precision, tol, maxit, ngram, st);
+  if (rc) return rc;
+  return st->status;
+}
+
+int elastic_solve(elastic_ctx* x, int32_t solver, int32_t precond, int32_t precision,
+                  const double* b, double* xv, double tol, int32_t maxit, int32_t ngram,
+                  elastic_stats* st) {
+  if (!x || !b || !xv) return ELASTIC_BAD_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  elastic_stats local;
+  if (!st) st = &local;
+  Context* c = &x->c;
+  if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
+  // zero diagonal check (Iters.f:337-342) on the device copy
+  int rc = elastic_upload_rhs(x, b);
+  if (rc) return rc;
+  rc = solve_resident(x, solver, precond, precision, tol, maxit, ngram, st);
+  if (rc) return rc;
+  rc = elastic_download_x(x, xv);
+  if (rc) return rc;
+  st->t_total_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return st->status;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- drop-in for Solv.f:13
+namespace {
+struct DropIn {
+  elastic_ctx* ctx = nullptr;
+  int32_t neq = -1, nad = -1;
+  const int32_t *ip = nullptr, *ja = nullptr;
+};
+DropIn g_dropin;
+}  // namespace
+
+extern "C" int32_t elastic_solv_(int32_t* neq, int32_t* nad, int32_t* ip, int32_t* ja, double* ad,
+                                 double* au, double* al, double* m, double* b, double* x,
+                                 double* tol, int32_t* maxit, int32_t* ngram, int32_t* unsym,
+                                 int32_t* solver, int32_t* neqf1i, int32_t* neqf2i, int32_t* neq3i,
+                                 int32_t* neq4i, int32_t* neq_doti, int64_t* i_fmapi,
+                                 int64_t* i_xfi, int64_t* i_rcvsi, int64_t* i_dspli) {
+  (void)au;  // symmetric storage: au is al (Datastruct.f:89-90)
+  (void)neqf1i, (void)neqf2i, (void)neq3i, (void)neq4i, (void)neq_doti;
+  (void)i_fmapi, (void)i_xfi, (void)i_rcvsi, (void)i_dspli;
+  if (!neq || !nad || !ip || !ja || !ad || !al || !m || !b || !x || !tol || !maxit || !solver)
+    return ELASTIC_BAD_ARG;
+  if (unsym && *unsym) {
+    fprintf(stderr, "Solver %d nao disponivel para matriz nao-simetrica !\n", *solver);
+    return ELASTIC_BAD_ARG;
+  }
+  DropIn& g = g_dropin;
+  int rc;
+  if (!g.ctx) {
+    rc = elastic_create(&g.ctx, 0);
+    if (rc) return rc;
+  }
+  if (g.neq != *neq || g.nad != *nad || g.ip != ip || g.ja != ja) {
+    rc = elastic_set_structure(g.ctx, *neq, *nad, ip, ja);
+    if (rc) {
+      fprintf(stderr, "elastic_solv: %s\n", elastic_last_error(g.ctx));
+      return rc;
+    }
+    g.neq = *neq, g.nad = *nad, g.ip = ip, g.ja = ja;
+  }
+  rc = elastic_set_values(g.ctx, ad, al);
+  if (rc) return rc;
+  for (int i = 0; i < *neq; ++i) m[i] = 1.0 / ad[i];  // pre_diag (Solv.f:65-69)
+  int precision = ELASTIC_FP64;
+  if (const char* e = getenv("ELASTIC_PRECISION")) precision = atoi(e) == 32 ? ELASTIC_MIXED : ELASTIC_FP64;
+  elastic_stats st;
+  rc = elastic_solve(g.ctx, *solver, ELASTIC_PRECOND_DIAG, precision, b, x, *tol, *maxit,
+                     ngram ? *ngram : 50, &st);
+  if (getenv("ELASTIC_VERBOSE")) {
+    printf(" (PCG) solver [gfx950]:\n");
+    printf("     Solver tol           = %20.6E\n", *tol);
+    printf("     tol * ||b||m         = %20.6E\n", st.conv);
+    printf("     Number of equations  = %20d\n", *neq);
+    printf("     nad                  = %20d\n", *nad);
+    printf("     Number of iterations = %20d\n", st.iters);
+    printf("     x * Kx               = %20.10E\n", st.xkx);
+    printf("     || x ||              = %20.10E\n", st.norm_x);
+    printf("     || b - Ax ||         = %20.10E\n", st.norm_r);
+    printf("     || b - Ax ||m        = %20.10E\n", st.norm_m_r);
+    printf("     time (s)             = %20.4f\n", st.t_total_ms * 1e-3);
+  }
+  if (rc == ELASTIC_MAXIT)
+    printf(" *** WARNING: No convergence reached after %9d iterations !\n", *maxit);
+  else if (rc)
+    fprintf(stderr, "elastic_solv: %s\n", elastic_last_error(g.ctx));
+  return rc;
+}

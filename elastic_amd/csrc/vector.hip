@@ -1,0 +1,149 @@
+// Vector kernels around the Krylov loops: precision casts, the Jacobi
+// inverse, the post-solve statistics of Iters.f:435-457 and the FP64 steps of
+// the mixed-precision iterative refinement.  All reductions use the fixed
+// two-stage order of pcg.hip.
+#include "elastic_internal.h"
+
+namespace elastic {
+
+constexpr int kT = 256;
+
+__device__ inline double blk_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return r;
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(kT) void k_cast(int64_t n, const S* __restrict__ a, D* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    b[i] = (D)a[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kT) void k_recip(int n, const T* __restrict__ ad, T* __restrict__ m) {
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) m[i] = T(1) / ad[i];
+}
+
+// y += merged partials (when inc_ptr != null); then 4 partial sums:
+// x.y, x.x, (b-y).m(b-y), (b-y).(b-y)
+__global__ __launch_bounds__(kT) void k_post(int n, const int32_t* __restrict__ inc_ptr,
+                                              const int32_t* __restrict__ inc_idx,
+                                              const double* __restrict__ epart,
+                                              const double* __restrict__ x, double* y,
+                                              const double* __restrict__ b,
+                                              const double* __restrict__ m, double* part) {
+  __shared__ double sh[8];
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
+    double s = y[i];
+    if (inc_ptr)
+      for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
+    y[i] = s;
+    const double xi = x[i], ri = b[i] - s;
+    a0 += xi * s;
+    a1 += xi * xi;
+    a2 += ri * (ri * m[i]);
+    a3 += ri * ri;
+  }
+  a0 = blk_sum(a0, sh);
+  a1 = blk_sum(a1, sh);
+  a2 = blk_sum(a2, sh);
+  a3 = blk_sum(a3, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a0;
+    part[kRedBlocks + blockIdx.x] = a1;
+    part[2 * kRedBlocks + blockIdx.x] = a2;
+    part[3 * kRedBlocks + blockIdx.x] = a3;
+  }
+}
+
+// out[j] = sum of part[j*kRedBlocks ...] for j < nsum (one block)
+__global__ __launch_bounds__(kT) void k_sum_parts(const double* __restrict__ part, int nsum,
+                                                   double* out) {
+  __shared__ double sh[8];
+  for (int j = 0; j < nsum; ++j) {
+    double v = 0.0;
+    for (int q = threadIdx.x; q < kRedBlocks; q += kT) v += part[j * kRedBlocks + q];
+    v = blk_sum(v, sh);
+    if (threadIdx.x == 0) out[j] = v;
+  }
+}
+
+// Iterative refinement: x64 += e32 (widened)
+__global__ __launch_bounds__(kT) void k_ir_correct(int n, const float* __restrict__ e,
+                                                    double* __restrict__ x) {
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) x[i] += (double)e[i];
+}
+
+// r = b - (y + merged partials); r32 = r; partial r.m r (FP64)
+__global__ __launch_bounds__(kT) void k_ir_residual(int n, const int32_t* __restrict__ inc_ptr,
+                                                     const int32_t* __restrict__ inc_idx,
+                                                     const double* __restrict__ epart,
+                                                     const double* __restrict__ y,
+                                                     const double* __restrict__ b,
+                                                     const double* __restrict__ m,
+                                                     double* __restrict__ r,
+                                                     float* __restrict__ r32, double* part) {
+  __shared__ double sh[8];
+  double acc = 0.0;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
+    double s = y[i];
+    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
+    const double ri = b[i] - s;
+    r[i] = ri;
+    r32[i] = (float)ri;
+    acc += ri * (ri * m[i]);
+  }
+  acc = blk_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// ------------------------------------------------------------- launchers
+static int grid_for(int64_t n) {
+  int64_t g = (n + kT - 1) / kT;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
+}
+
+template <typename S, typename D>
+void launch_cast(int64_t n, const S* a, D* b, hipStream_t s) {
+  hipLaunchKernelGGL((k_cast<S, D>), dim3(grid_for(n)), dim3(kT), 0, s, n, a, b);
+}
+template void launch_cast<double, float>(int64_t, const double*, float*, hipStream_t);
+template void launch_cast<float, double>(int64_t, const float*, double*, hipStream_t);
+template void launch_cast<double, double>(int64_t, const double*, double*, hipStream_t);
+
+template <typename T>
+void launch_recip(int n, const T* ad, T* m, hipStream_t s) {
+  hipLaunchKernelGGL(k_recip<T>, dim3(grid_for(n)), dim3(kT), 0, s, n, ad, m);
+}
+template void launch_recip<double>(int, const double*, double*, hipStream_t);
+template void launch_recip<float>(int, const float*, float*, hipStream_t);
+
+void launch_post(const DevTiles& t, int n, const double* epart, const double* x, double* y,
+                 const double* b, const double* m, double* part, double* out4, hipStream_t s) {
+  hipLaunchKernelGGL(k_post, dim3(kRedBlocks), dim3(kT), 0, s, n, t.inc_ptr, t.inc_idx, epart, x, y,
+                     b, m, part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, s, part, 4, out4);
+}
+
+void launch_sum_parts(const double* part, int nsum, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, s, part, nsum, out);
+}
+
+void launch_ir_correct(int n, const float* e, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(kT), 0, s, n, e, x);
+}
+
+void launch_ir_residual(const DevTiles& t, int n, const double* epart, const double* y,
+                        const double* b, const double* m, double* r, float* r32, double* part,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_ir_residual, dim3(kRedBlocks), dim3(kT), 0, s, n, t.inc_ptr, t.inc_idx,
+                     epart, y, b, m, r, r32, part);
+}
+
+}  // namespace elastic

@@ -1,0 +1,151 @@
+/*
+ * elastic_hip.h -- C-ABI of the MI355X (gfx950) implementation of the Elastic
+ * solve path: CSRC assembly and PCG / BiCGSTAB / GMRES on one GPU.
+ *
+ * Conventions (those of the reference's Fortran, so that a Fortran caller can
+ * bind every entry point with ISO_C_BINDING and pass its own arrays):
+ *   - plain host pointers and sizes; index arrays int32 with 1-based values;
+ *   - CSRC symmetric storage: ia(neq+1) with ia(1)=1, ja(nad) strictly lower
+ *     columns sorted per row, al(nad) values, ad(neq) diagonal
+ *     (Datastruct.f:71-93, Csr.f:13-305);
+ *   - the caller owns host arrays; the library owns the device copies inside a
+ *     context; one host thread per context;
+ *   - every entry point returns a status code instead of stopping the program.
+ * Unless stated otherwise an entry point is synchronous (it returns when its
+ * results are in the caller's arrays).
+ */
+#ifndef ELASTIC_HIP_H
+#define ELASTIC_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ELASTIC_OK 0
+#define ELASTIC_ZERO_DIAG 1      /* Iters.f:337-342 */
+#define ELASTIC_MAXIT 2          /* Iters.f:429-431 */
+#define ELASTIC_NEG_JACOBIAN 3   /* Elmlib.f:2004, 2685; Elmlib02.f:152 */
+#define ELASTIC_OOM 4            /* Malloc.f:348 */
+#define ELASTIC_HIP_ERROR 5
+#define ELASTIC_BAD_ARG 6
+#define ELASTIC_NOT_READY 7      /* values or structure not set */
+#define ELASTIC_BREAKDOWN 8      /* zero denominator inside a Krylov method */
+
+/* solver codes (Main.f:193, Solv.f:467-498) */
+#define ELASTIC_SOLVER_PCG 1
+#define ELASTIC_SOLVER_GMRES 2
+#define ELASTIC_SOLVER_BICGSTAB 4
+/* preconditioner codes (Main.f:190, precond.fi) */
+#define ELASTIC_PRECOND_NONE 1
+#define ELASTIC_PRECOND_DIAG 2
+#define ELASTIC_PRECOND_BDIAG3 6
+/* arithmetic */
+#define ELASTIC_FP64 64          /* FP64 values, vectors and sums */
+#define ELASTIC_MIXED 32         /* FP32 inner solves inside FP64 iterative refinement */
+
+typedef struct elastic_ctx elastic_ctx;
+
+/* Results of one solve, the quantities pcg prints (Iters.f:435-507). */
+typedef struct {
+  int32_t iters;        /* Krylov iterations (all inner iterations for MIXED) */
+  int32_t outer;        /* refinement steps (MIXED) or restart cycles (GMRES) */
+  int32_t status;
+  int32_t spmv_count;   /* products with A (any precision) */
+  double xkx;           /* x . K x */
+  double norm_x;        /* ||x|| */
+  double norm_r;        /* ||b - A x|| (FP64, explicit) */
+  double norm_m_r;      /* ||b - A x||_M with M = diag(1/ad) */
+  double conv;          /* tol * ||b||_M, the stopping threshold (Iters.f:355-360) */
+  double rel_res;       /* norm_m_r / ||b||_M */
+  double t_device_ms;   /* device time of the iterations (HIP events, inputs resident) */
+  double t_total_ms;    /* wall time of the call incl. b upload and x download */
+} elastic_stats;
+
+/* ------------------------------------------------------------ context */
+/* Creates a context on HIP device `device`. */
+int elastic_create(elastic_ctx** ctx, int32_t device);
+int elastic_destroy(elastic_ctx* ctx);
+/* Human-readable text of the last error on this context (never NULL). */
+const char* elastic_last_error(const elastic_ctx* ctx);
+
+/* --------------------------------------------------- host-side set-up */
+/* These replace the reference's host steps and return exactly its arrays. */
+/* Graph.f:20-257 + Reord.f:1-434 (reord with afl=.true.): perm(k) = old node
+ * at position k.  ix is ix(nen+1,numel) (element nodes, then material). */
+int elastic_reord(int32_t nnode, int32_t numel, int32_t nen, const int32_t* ix, int32_t* perm);
+/* Numeq.f:139-191: id(ndf,nnode) 0 = free / 1 = restrained on input, equation
+ * numbers on output; *neq receives the equation count. */
+int elastic_numeq(int32_t nnode, int32_t ndf, const int32_t* perm, int32_t* id, int32_t* neq);
+/* Csr.f:13-305 via Datastruct.f:71-93: first call with ja == NULL returns *nad
+ * and fills ia(neq+1); second call fills ja(nad). */
+int elastic_csrstruct(int32_t nnode, int32_t numel, int32_t nen, int32_t ndf, int32_t neq,
+                      const int32_t* ix, const int32_t* id, int32_t* ia, int32_t* ja,
+                      int64_t* nad);
+
+/* ------------------------------------------------- system on the GPU */
+/* Uploads the CSRC pattern once and builds the device layout (row tiles,
+ * packed local indices, merge lists).  Replaces the structure half of
+ * datastruct (Datastruct.f:71-93) for the device. */
+int elastic_set_structure(elastic_ctx* ctx, int32_t neq, int64_t nad, const int32_t* ia,
+                          const int32_t* ja);
+/* Uploads matrix values (as the reference assembled them on the host). */
+int elastic_set_values(elastic_ctx* ctx, const double* ad, const double* al);
+/* Copies the device values back (after elastic_assemble, so that the Fortran
+ * caller's CSRC stays authoritative). */
+int elastic_get_values(elastic_ctx* ctx, double* ad, double* al);
+/* y = (D + L + L^T) x, Matvec.f:528-621.  precision ELASTIC_FP64 or
+ * ELASTIC_MIXED (FP32 values and sums, result widened). */
+int elastic_spmv(elastic_ctx* ctx, int32_t precision, const double* x, double* y);
+
+/* ----------------------------------------------------------- solving */
+/* Solve A x = b.  Replaces solv -> pcg (Solv.f:13-139, Iters.f:252-524) for
+ * solver 1, gmres (Iters.f:528-808) for 2 and pbicgstab (Iters.f:1079-1349)
+ * for 4.  x0 = 0 (fnew = .true.); the stopping rule is the reference's:
+ * PCG ||r||_M < tol ||b||_M, GMRES ||M r|| < tol ||M b||, BiCGSTAB
+ * ||r|| < tol ||b||.  MIXED runs FP32 inner solves and refines x in FP64
+ * until the FP64 residual meets the same rule.  st may be NULL. */
+int elastic_solve(elastic_ctx* ctx, int32_t solver, int32_t precond, int32_t precision,
+                  const double* b, double* x, double tol, int32_t maxit, int32_t ngram,
+                  elastic_stats* st);
+
+/* Drop-in for the reference's solver call, Solv.f:13-15:
+ *   call solv(neq,nad,ip,ja,ad,au,al,m,b,x,tol,maxit,ngram,unsym,solver,
+ *             neqf1i,neqf2i,neq3i,neq4i,neq_doti,i_fmapi,i_xfi,i_rcvsi,i_dspli)
+ * Fortran passes every argument by reference (integer = int32, real*8 =
+ * double, logical = int32, pool offsets = int64).  Behaviour: m is
+ * overwritten with 1/ad (Solv.f:65-69); x receives the solution; the MPI
+ * arguments are accepted and ignored (one process).  Uses one internal context
+ * per device-0 process, rebuilt when neq/nad or the ia/ja pointers change.
+ * Returns the status (a Fortran caller may ignore it or declare a function). */
+int32_t elastic_solv_(int32_t* neq, int32_t* nad, int32_t* ip, int32_t* ja, double* ad,
+                      double* au, double* al, double* m, double* b, double* x, double* tol,
+                      int32_t* maxit, int32_t* ngram, int32_t* unsym, int32_t* solver,
+                      int32_t* neqf1i, int32_t* neqf2i, int32_t* neq3i, int32_t* neq4i,
+                      int32_t* neq_doti, int64_t* i_fmapi, int64_t* i_xfi, int64_t* i_rcvsi,
+                      int64_t* i_dspli);
+
+/* ------------------------------------------------ resident benchmarking */
+/* Uploads b to the device copy used by elastic_solve_resident. */
+int elastic_upload_rhs(elastic_ctx* ctx, const double* b);
+/* Solves with b already resident (elastic_upload_rhs) and leaves x on the
+ * device; st->t_device_ms is the device time of the iterations alone. */
+int elastic_solve_resident(elastic_ctx* ctx, int32_t solver, int32_t precond, int32_t precision,
+                           double tol, int32_t maxit, int32_t ngram, elastic_stats* st);
+int elastic_download_x(elastic_ctx* ctx, double* x);
+/* Average device time (ms) of `reps` symmetric SpMV launches on resident data
+ * (precision as in elastic_spmv), measured with HIP events on the library's
+ * stream; also returns the tile-kernel time alone in *kernel_ms. */
+int elastic_time_spmv(elastic_ctx* ctx, int32_t precision, int32_t reps, double* total_ms,
+                      double* kernel_ms);
+/* Layout facts: ntiles, max tile nnz, external partials, merge entries. */
+int elastic_layout_info(elastic_ctx* ctx, int64_t* info4);
+/* Tile size (nonzeros per tile, 256..3840); takes effect at the next
+ * elastic_set_structure. */
+int elastic_set_tile_nnz(elastic_ctx* ctx, int32_t tile_nnz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
