@@ -1,0 +1,180 @@
+"""Benchmark of the Elastic solve path on one MI355X.
+
+Workload: hex8 unit cube 100^3 (1,030,301 nodes, 3,060,300 free DOF), interior
+nodes jittered by U[-0.15h, 0.15h], E = 1, nu = 0.3, z=0 clamped, traction
+(0,0,-1e-3) on z=1; RCM-ordered CSRC; Jacobi PCG to a relative residual
+||r||_M / ||b||_M < 1e-10.  The system is assembled on the GPU.
+
+One step = one solve of that system with b and the matrix resident in HBM:
+FP32 CSRC SpMV/preconditioner inside FP64 iterative refinement (the measured
+method), timed between device synchronisations.  The plain FP64 GPU PCG is
+timed beside it, and the oracle's OpenMP FP64 PCG (the reference's pcg_omp
+restated) is timed on the host cores on a bounded number of iterations of the
+same system.
+
+Prints one JSON line (see the keys below).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+TOL = 1e-10
+
+
+def spmv_bytes(neq: int, nad: int, s: int) -> int:
+    """Algorithmic bytes of one symmetric CSRC product: ia, ja, al, ad, x, y."""
+    return 4 * (neq + 1) + 4 * nad + s * nad + s * neq + s * neq + s * neq
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=100, help="cells per cube edge")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: multi-GPU sharding of this path is not built "
+              "(see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from elastic_amd import mesh as M
+    from elastic_amd.solver import FP64, MIXED
+    from elastic_amd.system import System
+
+    t0 = time.time()
+    mesh = M.hex8_cube(args.n, jitter=0.15, seed=M.BASE_SEED + 2)
+    sysm = System(mesh)
+    ctx = sysm.ctx
+    neq, nad = sysm.neq, sysm.nad
+    t_setup = time.time() - t0
+    print("setup: neq=%d nad=%d in %.1f s" % (neq, nad, t_setup), file=sys.stderr, flush=True)
+    ctx.upload_rhs(sysm.b)
+
+    def sync():
+        torch.cuda.synchronize()
+
+    # reference answer on the GPU: FP64 PCG tightened to 1e-12
+    st_ref = ctx.solve_resident(precision=FP64, tol=1e-12, maxit=20000)
+    x_ref = ctx.download_x()
+
+    # plain FP64 GPU PCG (the drop-in), timed
+    lat64 = []
+    for i in range(args.warmup + args.steps):
+        sync()
+        a = time.perf_counter()
+        st64 = ctx.solve_resident(precision=FP64, tol=TOL, maxit=20000)
+        sync()
+        if i >= args.warmup:
+            lat64.append(time.perf_counter() - a)
+    x64 = ctx.download_x()
+
+    # measured method: FP32 inner PCG + FP64 refinement
+    for _ in range(args.warmup):
+        ctx.solve_resident(precision=MIXED, tol=TOL, maxit=20000)
+    lat = []
+    sync()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        stm = ctx.solve_resident(precision=MIXED, tol=TOL, maxit=20000)
+        sync()
+        lat.append(time.perf_counter() - a)
+    sync()
+    t_total = time.perf_counter() - t_start
+    xm = ctx.download_x()
+    err_m = float(np.linalg.norm(xm - x_ref) / np.linalg.norm(x_ref))
+    err_64 = float(np.linalg.norm(x64 - x_ref) / np.linalg.norm(x_ref))
+
+    # SpMV kernel roofline (FP32 tile kernel, the dominant kernel of the step)
+    tot32, ker32 = ctx.time_spmv(MIXED, reps=50)
+    tot64, ker64 = ctx.time_spmv(FP64, reps=50)
+    b32, b64 = spmv_bytes(neq, nad, 4), spmv_bytes(neq, nad, 8)
+    gbs32 = b32 / (ker32 * 1e-3) / 1e9
+    gbs64 = b64 / (ker64 * 1e-3) / 1e9
+    t_asm = sysm.time_assembly(reps=3)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sysm, st64["iters"], args.cpu_seconds)
+
+    value = neq * args.steps / t_total
+    out = {
+        "metric": "solved DOF/s + p50/p99 solve latency at 1e-8 rel err; CSRC SpMV GB/s vs HBM",
+        "value": value,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-inner/fp64-refinement",
+        "data": "synthetic (generated mesh, GPU-assembled system)",
+        "config": {"workload": "hex8 100^3 jittered 0.15h, E=1 nu=0.3, Jacobi PCG to 1e-10 "
+                               "(%d DOF, nad %d)" % (neq, nad),
+                   "method": "FP32 CSRC SpMV + Jacobi inner PCG, FP64 iterative refinement"},
+        "latency_ms_p50": 1e3 * float(np.percentile(lat, 50)),
+        "latency_ms_p99": 1e3 * float(np.percentile(lat, 99)),
+        "rel_err_vs_fp64": err_m,
+        "iters_mixed": stm["iters"], "refinements": stm["outer"],
+        "fp64": {"dof_per_s": neq / float(np.mean(lat64)), "iters": st64["iters"],
+                 "latency_ms_p50": 1e3 * float(np.percentile(lat64, 50)),
+                 "rel_err_vs_tight": err_64},
+        "speedup_mixed_vs_fp64": float(np.mean(lat64)) / float(np.mean(lat)),
+        "spmv": {"fp32_kernel_ms": ker32, "fp32_with_merge_ms": tot32, "fp32_GBs": gbs32,
+                 "fp64_kernel_ms": ker64, "fp64_with_merge_ms": tot64, "fp64_GBs": gbs64,
+                 "bytes_fp32": b32, "bytes_fp64": b64},
+        "assembly_ms": t_asm,
+        "setup_s": t_setup,
+        "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_spmv_tiles<float>"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(sysm, iters_fp64: int, seconds: float):
+    """The oracle's OpenMP PCG (Iters_omp.f restated) on the same system, on the
+    host cores, for a bounded number of iterations; solved DOF/s extrapolated
+    with the FP64 iteration count."""
+    from oracle import binding as O
+
+    nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    ad, al = sysm.values()
+    m = 1.0 / ad
+    b = sysm.b
+    # calibrate with a few iterations, then run a sample of about `seconds`
+    a = time.perf_counter()
+    O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=1e-30, maxit=3, m=m, nth=nth)
+    t3 = (time.perf_counter() - a) / 3.0
+    k = max(5, min(iters_fp64, int(seconds / max(t3, 1e-6))))
+    a = time.perf_counter()
+    _, st = O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=1e-30, maxit=k, m=m, nth=nth)
+    t = time.perf_counter() - a
+    t_it = t / k
+    return {"value": sysm.neq / (t_it * iters_fp64), "unit": "DOF/s", "cores": nth,
+            "kind": "port",
+            "sample": "%d of the %d FP64 PCG iterations (OpenMP pcg_omp restatement, %d threads, "
+                      "%.1f s), extrapolated to a full solve" % (k, iters_fp64, nth, t),
+            "s_per_iter": t_it}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,807 @@
+// Element stiffness and load assembly on gfx950 (Pform.f:1-254, Assbly.f:66-146,
+// Elmlib02.f elmt02..07, Elmlib.f shape functions and Jacobians).
+//
+// Stiffness (isw = 2): one 64-lane wavefront per element, one launch per
+// colour.  Every lane evaluates the Jacobian of each Gauss point itself (the
+// same operations in the same order as the reference, FP contraction off), so
+// the element matrix s(nst,nst) is bit-identical to the reference's; each lane
+// accumulates its share of the nst*nst entries over the Gauss points, the
+// matrix goes to LDS, p = s u is formed in the reference's order
+// (Elmlib.f:336-369), and the entries are added to ad/al/b through the
+// element's slot map with plain read-modify-writes (elements of one colour
+// share no node).
+// Loads (isw = 4): one lane per element of a colour, face/edge tractions and
+// self weight as Elmlib02.f computes them.
+#include <cmath>
+
+#include "elastic_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace elastic {
+
+// element kinds
+enum { K_TRIA3 = 0, K_QUAD4 = 1, K_TET4 = 2, K_HEX8 = 3 };
+
+template <int KIND>
+struct ElemTraits;
+template <>
+struct ElemTraits<K_TRIA3> { static constexpr int NEN = 3, NDF = 2, NDM = 2; };
+template <>
+struct ElemTraits<K_QUAD4> { static constexpr int NEN = 4, NDF = 2, NDM = 2; };
+template <>
+struct ElemTraits<K_TET4> { static constexpr int NEN = 4, NDF = 3, NDM = 3; };
+template <>
+struct ElemTraits<K_HEX8> { static constexpr int NEN = 8, NDF = 3, NDM = 3; };
+
+__constant__ double kPG2[2] = {-.577350269189626, .577350269189626};
+constexpr double kDIV6 = 0.166666666666667;
+constexpr double kDIV144 = 0.364444444444444e-02;
+constexpr double kJZERO = 1.0e-14;
+
+struct AsmArgs {
+  const int32_t* ix;     // (nen+1) per element
+  const double* x;       // ndm per node
+  const int32_t* id;     // equation numbers, ndf per node
+  const int32_t* ie;     // element type per material
+  const double* e;       // 15 per material
+  const int32_t* eload;  // 7 per element
+  const int32_t* order;  // elements (0-based) grouped by colour
+  const int32_t* slots;  // npair per element
+  const int32_t* ltype;  // per load
+  const int32_t* lnparc;
+  const double* lf;      // 3 per load
+  const double* u;       // ndf per node (may be null)
+  double *ad, *al, *b;
+  int32_t* err;          // first element (1-based) with a non-positive Jacobian
+  int nen, ndf, ndm, neq, nloads;
+  double g0, g1, g2;     // gravity (Main.f:205-207)
+};
+
+// ---------------------------------------------------------- shape functions
+__device__ inline void sfquad4_d(double r, double s, double* h, double* hx, double* hy) {
+  double opr = 1.0 + r, ops = 1.0 + s, omr = 1.0 - r, oms = 1.0 - s;
+  h[0] = opr * ops * 0.25;
+  h[1] = omr * ops * 0.25;
+  h[2] = omr * oms * 0.25;
+  h[3] = opr * oms * 0.25;
+  hx[0] = ops * 0.25;
+  hx[1] = -ops * 0.25;
+  hx[2] = -oms * 0.25;
+  hx[3] = oms * 0.25;
+  hy[0] = opr * 0.25;
+  hy[1] = omr * 0.25;
+  hy[2] = -omr * 0.25;
+  hy[3] = -opr * 0.25;
+}
+
+__device__ inline void sfhexa8_d(double r, double s, double t, double* h, double* hx, double* hy,
+                                 double* hz) {
+  double pR = 1.0 + r, pS = 1.0 + s, pT = 1.0 + t, mR = 1.0 - r, mS = 1.0 - s, mT = 1.0 - t;
+  h[0] = pR * pS * pT * 0.125;
+  h[1] = mR * pS * pT * 0.125;
+  h[2] = mR * mS * pT * 0.125;
+  h[3] = pR * mS * pT * 0.125;
+  h[4] = pR * pS * mT * 0.125;
+  h[5] = mR * pS * mT * 0.125;
+  h[6] = mR * mS * mT * 0.125;
+  h[7] = pR * mS * mT * 0.125;
+  hx[0] = pS * pT * 0.125;
+  hx[1] = -pS * pT * 0.125;
+  hx[2] = -mS * pT * 0.125;
+  hx[3] = mS * pT * 0.125;
+  hx[4] = pS * mT * 0.125;
+  hx[5] = -pS * mT * 0.125;
+  hx[6] = -mS * mT * 0.125;
+  hx[7] = mS * mT * 0.125;
+  hy[0] = pR * pT * 0.125;
+  hy[1] = mR * pT * 0.125;
+  hy[2] = -mR * pT * 0.125;
+  hy[3] = -pR * pT * 0.125;
+  hy[4] = pR * mT * 0.125;
+  hy[5] = mR * mT * 0.125;
+  hy[6] = -mR * mT * 0.125;
+  hy[7] = -pR * mT * 0.125;
+  hz[0] = pR * pS * 0.125;
+  hz[1] = mR * pS * 0.125;
+  hz[2] = mR * mS * 0.125;
+  hz[3] = pR * mS * 0.125;
+  hz[4] = -pR * pS * 0.125;
+  hz[5] = -mR * pS * 0.125;
+  hz[6] = -mR * mS * 0.125;
+  hz[7] = -pR * mS * 0.125;
+}
+
+// jacob2d_m (Elmlib.f:1769-1852); x has leading dimension ldx
+__device__ inline bool jacob2d_d(double* hx, double* hy, const double* x, int nen, int ldx,
+                                 double* det) {
+  double j00 = 0.0, j01 = 0.0, j10 = 0.0, j11 = 0.0;
+  for (int k = 0; k < nen; ++k) {
+    j00 = j00 + hx[k] * x[k * ldx + 0];
+    j10 = j10 + hy[k] * x[k * ldx + 0];
+  }
+  for (int k = 0; k < nen; ++k) {
+    j01 = j01 + hx[k] * x[k * ldx + 1];
+    j11 = j11 + hy[k] * x[k * ldx + 1];
+  }
+  *det = j00 * j11 - j10 * j01;
+  if (*det <= kJZERO) return false;
+  const double d = *det;
+  const double i11 = j11 / d, i12 = -j01 / d, i21 = -j10 / d, i22 = j00 / d;
+  for (int k = 0; k < nen; ++k) {
+    double a = hx[k], b = hy[k];
+    hx[k] = i11 * a + i12 * b;
+    hy[k] = i21 * a + i22 * b;
+  }
+  return true;
+}
+
+// jacob3d_m (Elmlib.f:1942-2037)
+__device__ inline bool jacob3d_d(double* hx, double* hy, double* hz, const double* x, int nen,
+                                 double* det) {
+  double xj[3][3];
+  for (int j = 0; j < 3; ++j) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = 0; k < nen; ++k) {
+      a = a + hx[k] * x[k * 3 + j];
+      b = b + hy[k] * x[k * 3 + j];
+      c = c + hz[k] * x[k * 3 + j];
+    }
+    xj[0][j] = a;
+    xj[1][j] = b;
+    xj[2][j] = c;
+  }
+#define XJ(p, q) xj[(p)-1][(q)-1]
+  *det = XJ(1, 1) * XJ(2, 2) * XJ(3, 3) + XJ(1, 2) * XJ(2, 3) * XJ(3, 1) +
+         XJ(1, 3) * XJ(2, 1) * XJ(3, 2) - XJ(3, 1) * XJ(2, 2) * XJ(1, 3) -
+         XJ(2, 1) * XJ(1, 2) * XJ(3, 3) - XJ(1, 1) * XJ(3, 2) * XJ(2, 3);
+  if (*det <= kJZERO) return false;
+  const double d = *det;
+  const double i11 = (XJ(2, 2) * XJ(3, 3) - XJ(2, 3) * XJ(3, 2)) / d;
+  const double i21 = -(XJ(2, 1) * XJ(3, 3) - XJ(2, 3) * XJ(3, 1)) / d;
+  const double i31 = (XJ(2, 1) * XJ(3, 2) - XJ(2, 2) * XJ(3, 1)) / d;
+  const double i12 = -(XJ(1, 2) * XJ(3, 3) - XJ(1, 3) * XJ(3, 2)) / d;
+  const double i22 = (XJ(1, 1) * XJ(3, 3) - XJ(1, 3) * XJ(3, 1)) / d;
+  const double i32 = -(XJ(1, 1) * XJ(3, 2) - XJ(1, 2) * XJ(3, 1)) / d;
+  const double i13 = (XJ(1, 2) * XJ(2, 3) - XJ(1, 3) * XJ(2, 2)) / d;
+  const double i23 = -(XJ(1, 1) * XJ(2, 3) - XJ(1, 3) * XJ(2, 1)) / d;
+  const double i33 = (XJ(1, 1) * XJ(2, 2) - XJ(1, 2) * XJ(2, 1)) / d;
+#undef XJ
+  for (int k = 0; k < nen; ++k) {
+    double a = hx[k], b = hy[k], c = hz[k];
+    hx[k] = i11 * a + i12 * b + i13 * c;
+    hy[k] = i21 * a + i22 * b + i23 * c;
+    hz[k] = i31 * a + i32 * b + i33 * c;
+  }
+  return true;
+}
+
+// jtetra4 (Elmlib.f:2636-2713)
+__device__ inline bool jtetra4_d(const double* x, double* hx, double* hy, double* hz, double* det,
+                                 bool afl) {
+#define X(i, n) x[((n)-1) * 3 + (i)-1]
+  const double j11 = X(1, 1) - X(1, 4), j12 = X(2, 1) - X(2, 4), j13 = X(3, 1) - X(3, 4);
+  const double j21 = X(1, 2) - X(1, 4), j22 = X(2, 2) - X(2, 4), j23 = X(3, 2) - X(3, 4);
+  const double j31 = X(1, 3) - X(1, 4), j32 = X(2, 3) - X(2, 4), j33 = X(3, 3) - X(3, 4);
+#undef X
+  *det = j11 * j22 * j33 + j12 * j23 * j31 + j13 * j21 * j32 - j31 * j22 * j13 -
+         j12 * j21 * j33 - j11 * j32 * j23;
+  if (*det <= kJZERO) return false;
+  if (afl) {
+    const double di = 1.0 / *det;
+    hx[0] = (j22 * j33 - j23 * j32) * di;
+    hx[1] = (j13 * j32 - j12 * j33) * di;
+    hx[2] = (j12 * j23 - j13 * j22) * di;
+    hy[0] = (j23 * j31 - j21 * j33) * di;
+    hy[1] = (j11 * j33 - j13 * j31) * di;
+    hy[2] = (j13 * j21 - j11 * j23) * di;
+    hz[0] = (j21 * j32 - j22 * j31) * di;
+    hz[1] = (j12 * j31 - j11 * j32) * di;
+    hz[2] = (j11 * j22 - j12 * j21) * di;
+    hx[3] = -hx[0] - hx[1] - hx[2];
+    hy[3] = -hy[0] - hy[1] - hy[2];
+    hz[3] = -hz[0] - hz[1] - hz[2];
+  }
+  return true;
+}
+
+// ------------------------------------------------------- stiffness entries
+// s(ndf*a+r, ndf*bn+c) contribution of one Gauss point (3-D, a,b,c of the
+// constitutive matrix; Elmlib02.f:1715-1777 hex8, 1353-1383 tet4)
+__device__ inline double term3(bool hex, int r, int c, double xi, double yi, double zi, double xj,
+                               double yj, double zj, double b, double cc, double wt1) {
+  if (r == 0 && c == 0)
+    return hex ? (xi * xj + cc * (yi * yj + zi * zj)) * wt1
+               : (xi * xj + cc * yi * yj + cc * zi * zj) * wt1;
+  if (r == 1 && c == 1)
+    return hex ? (yi * yj + cc * (xi * xj + zi * zj)) * wt1
+               : (yi * yj + cc * xi * xj + cc * zi * zj) * wt1;
+  if (r == 2 && c == 2)
+    return hex ? (zi * zj + cc * (yi * yj + xi * xj)) * wt1
+               : (zi * zj + cc * yi * yj + cc * xi * xj) * wt1;
+  if (r == 0 && c == 1) return (b * xi * yj + cc * yi * xj) * wt1;
+  if (r == 0 && c == 2) return (b * xi * zj + cc * zi * xj) * wt1;
+  if (r == 1 && c == 0) return (b * yi * xj + cc * xi * yj) * wt1;
+  if (r == 1 && c == 2) return (b * yi * zj + cc * zi * yj) * wt1;
+  if (r == 2 && c == 0) return (b * zi * xj + cc * xi * zj) * wt1;
+  return (b * zi * yj + cc * yi * zj) * wt1;  // (2,1)
+}
+
+// 2-D (Elmlib02.f:207-224 tria3, 780-800 quad4)
+__device__ inline double term2(int r, int c, double xi, double yi, double xj, double yj, double d11,
+                               double d12, double d22, double d33, double wt1) {
+  if (r == 0 && c == 0) return (xi * d11 * xj + yi * d33 * yj) * wt1;
+  if (r == 0 && c == 1) return (xi * d12 * yj + yi * d33 * xj) * wt1;
+  if (r == 1 && c == 0) return (yi * d12 * xj + xi * d33 * yj) * wt1;
+  return (yi * d22 * yj + xi * d33 * xj) * wt1;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64) void k_stiffness(AsmArgs a, int c0, int lhs, int rhs) {
+  using TR = ElemTraits<KIND>;
+  constexpr int NEN = TR::NEN, NDF = TR::NDF, NDM = TR::NDM, NST = NEN * NDF;
+  constexpr int NPAIR = NST * (NST + 1) / 2, NENT = NST * NST;
+  constexpr int PER = (NENT + 63) / 64;
+  __shared__ double xl[NEN * NDM], ul[NST], s[NENT], pl[NST];
+  __shared__ int32_t ldv[NST];
+  __shared__ int bad;
+  const int el = a.order[c0 + blockIdx.x];
+  const int tid = threadIdx.x;
+  const int32_t* ixe = a.ix + (size_t)el * (NEN + 1);
+  if (tid == 0) bad = 0;
+  if (tid < NEN * NDM) {
+    int n = tid / NDM, d = tid % NDM;
+    xl[tid] = a.x[(size_t)(ixe[n] - 1) * NDM + d];
+  }
+  if (tid < NST) {
+    int n = tid / NDF, d = tid % NDF;
+    ldv[tid] = a.id[(size_t)(ixe[n] - 1) * NDF + d];
+    ul[tid] = a.u ? a.u[(size_t)(ixe[n] - 1) * NDF + d] : 0.0;
+  }
+  const int ma = ixe[NEN];
+  const int iel = a.ie[ma - 1];
+  const double* em = a.e + (size_t)(ma - 1) * 15;
+  const double ym = em[0], ps = em[1];
+  __syncthreads();
+  // constitutive constants (Elmlib02.f:1693-1702 and the 2-D variants)
+  double ca = 0, cb = 0, cc = 0, d11 = 0, d12 = 0, d22 = 0, d33 = 0, thic = 1.0;
+  const bool stress = (iel == 3 || iel == 5);
+  if (!stress) {
+    const double a1 = 1.0 - ps, a2 = 1.0 - 2.0 * ps, a3 = 1.0 + ps;
+    ca = (ym * a1) / (a3 * a2);
+    cb = ps / a1;
+    cc = 0.5 * (a2 / a1);
+  } else {
+    thic = em[2];
+    if (thic == 0) thic = 1.0;
+    const double a1 = 1.0 - ps, a2 = 1.0 - ps * ps;
+    ca = ym / a2;
+    cb = ps;
+    cc = 0.5 * a1;
+  }
+  d11 = ca;
+  d12 = ca * cb;
+  d22 = ca;
+  d33 = ca * cc;
+  double acc[PER];
+  for (int q = 0; q < PER; ++q) acc[q] = 0.0;
+  double hx[NEN], hy[NEN], hz[NEN], h[NEN];
+  if (KIND == K_HEX8) {
+    for (int lz = 0; lz < 2; ++lz)
+      for (int ly = 0; ly < 2; ++ly)
+        for (int lx = 0; lx < 2; ++lx) {
+          double det;
+          sfhexa8_d(kPG2[lx], kPG2[ly], kPG2[lz], h, hx, hy, hz);
+          if (!jacob3d_d(hx, hy, hz, xl, NEN, &det)) bad = 1;
+          const double w = 1.0 * 1.0 * 1.0 * det;
+          const double wt1 = w * ca;
+#pragma unroll
+          for (int q = 0; q < PER; ++q) {
+            const int en = tid + 64 * q;
+            if (en < NENT) {
+              const int i = en % NST, j = en / NST;
+              const int na = i / NDF, r = i % NDF, nb = j / NDF, c = j % NDF;
+              acc[q] = acc[q] + term3(true, r, c, hx[na], hy[na], hz[na], hx[nb], hy[nb], hz[nb],
+                                      cb, cc, wt1);
+            }
+          }
+        }
+  } else if (KIND == K_TET4) {
+    double det;
+    if (!jtetra4_d(xl, hx, hy, hz, &det, true)) bad = 1;
+    const double wt1 = ca * det * kDIV6;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int en = tid + 64 * q;
+      if (en < NENT) {
+        const int i = en % NST, j = en / NST;
+        const int na = i / NDF, r = i % NDF, nb = j / NDF, c = j % NDF;
+        acc[q] = term3(false, r, c, hx[na], hy[na], hz[na], hx[nb], hy[nb], hz[nb], cb, cc, wt1);
+      }
+    }
+  } else if (KIND == K_QUAD4) {
+    for (int ly = 0; ly < 2; ++ly)
+      for (int lx = 0; lx < 2; ++lx) {
+        double det;
+        sfquad4_d(kPG2[lx], kPG2[ly], h, hx, hy);
+        if (!jacob2d_d(hx, hy, xl, NEN, NDM, &det)) bad = 1;
+        double w = 1.0 * 1.0 * det;
+        if (stress) w = w * thic;
+        const double wt1 = w;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int en = tid + 64 * q;
+          if (en < NENT) {
+            const int i = en % NST, j = en / NST;
+            const int na = i / NDF, r = i % NDF, nb = j / NDF, c = j % NDF;
+            acc[q] = acc[q] + term2(r, c, hx[na], hy[na], hx[nb], hy[nb], d11, d12, d22, d33, wt1);
+          }
+        }
+      }
+  } else {  // K_TRIA3 (Elmlib02.f:143-229: determinant test against 0)
+    const double xj11 = xl[0] - xl[4], xj12 = xl[1] - xl[5];
+    const double xj21 = xl[2] - xl[4], xj22 = xl[3] - xl[5];
+    const double det = xj11 * xj22 - xj12 * xj21;
+    if (det <= 0.0) bad = 1;
+    const double ji11 = xj22 / det, ji12 = -xj12 / det, ji21 = -xj21 / det, ji22 = xj11 / det;
+    hx[0] = ji11;
+    hx[1] = ji12;
+    hx[2] = -ji11 - ji12;
+    hy[0] = ji21;
+    hy[1] = ji22;
+    hy[2] = -ji21 - ji22;
+    const double wt1 = 0.5 * det;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int en = tid + 64 * q;
+      if (en < NENT) {
+        const int i = en % NST, j = en / NST;
+        const int na = i / NDF, r = i % NDF, nb = j / NDF, c = j % NDF;
+        acc[q] = term2(r, c, hx[na], hy[na], hx[nb], hy[nb], d11, d12, d22, d33, wt1);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int en = tid + 64 * q;
+    if (en < NENT) s[en] = acc[q];  // s(i,j) at i + NST*j (column-major)
+  }
+  __syncthreads();
+  if (bad) {
+    if (tid == 0) a.err[0] = el + 1;  // any offending element is reported
+    return;
+  }
+  // p = s u (Elmlib.f:336-369), b -= p
+  if (rhs && tid < NST) {
+    double p = 0.0;
+    for (int j = 0; j < NST; ++j) p = p + s[j * NST + tid] * ul[j];
+    pl[tid] = p;
+    const int k = ldv[tid];
+    if (k > 0 && k <= a.neq) a.b[k - 1] = a.b[k - 1] - p;
+  }
+  if (!lhs) return;
+  const int32_t* sl = a.slots + (size_t)el * NPAIR;
+  for (int pp = tid; pp < NPAIR; pp += 64) {
+    const int32_t t = sl[pp];
+    if (t == -1) continue;
+    // pair pp <-> (i >= j): i = floor((sqrt(8pp+1)-1)/2)
+    int i = (int)((sqrtf(8.0f * pp + 1.0f) - 1.0f) * 0.5f);
+    while ((i + 1) * (i + 2) / 2 <= pp) ++i;
+    while (i * (i + 1) / 2 > pp) --i;
+    const int j = pp - i * (i + 1) / 2;
+    if (t <= -2) {
+      const int k = -t - 2;
+      a.ad[k] = a.ad[k] + s[i * NST + i];
+    } else {
+      const bool rev = (t >> 30) & 1;
+      const int64_t slot = t & ((1 << 30) - 1);
+      const double v = rev ? s[i * NST + j] : s[j * NST + i];  // s(j,i) : s(i,j)
+      a.al[slot] = a.al[slot] + v;
+    }
+  }
+}
+
+// ---------------------------------------------------------- loads (isw = 4)
+__device__ inline void tload_vec(const AsmArgs& a, int nc, double* ff) {
+  ff[0] = ff[1] = ff[2] = 0.0;
+  if (nc <= 0 || nc > a.nloads) return;
+  if (a.ltype[nc - 1] != 40) return;
+  for (int q = 0; q < a.lnparc[nc - 1] && q < 3; ++q) ff[q] = a.lf[3 * (nc - 1) + q];
+}
+
+__device__ inline void rotmat_d(const double* x, double r[3][3]) {
+  double v1x = x[3] - x[0], v1y = x[4] - x[1], v1z = x[5] - x[2];
+  double v2x = x[6] - x[0], v2y = x[7] - x[1], v2z = x[8] - x[2];
+  double v3x = v1y * v2z - v1z * v2y, v3y = v1z * v2x - v1x * v2z, v3z = v1x * v2y - v1y * v2x;
+  v2x = v1z * v3y - v1y * v3z;
+  v2y = v1x * v3z - v1z * v3x;
+  v2z = v1y * v3x - v1x * v3y;
+  double n1 = sqrt(v1x * v1x + v1y * v1y + v1z * v1z);
+  double n2 = sqrt(v2x * v2x + v2y * v2y + v2z * v2z);
+  double n3 = sqrt(v3x * v3x + v3y * v3y + v3z * v3z);
+  r[0][0] = v1x / n1, r[0][1] = v1y / n1, r[0][2] = v1z / n1;
+  r[1][0] = v2x / n2, r[1][1] = v2y / n2, r[1][2] = v2z / n2;
+  r[2][0] = v3x / n3, r[2][1] = v3y / n3, r[2][2] = v3z / n3;
+}
+__device__ inline void rot_d(double* x, double r[3][3]) {
+  double x1 = r[0][0] * x[0] + r[0][1] * x[1] + r[0][2] * x[2];
+  double x2 = r[1][0] * x[0] + r[1][1] * x[1] + r[1][2] * x[2];
+  double x3 = r[2][0] * x[0] + r[2][1] * x[1] + r[2][2] * x[2];
+  x[0] = x1, x[1] = x2, x[2] = x3;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64) void k_loads(AsmArgs a, int c0, int cnt) {
+  using TR = ElemTraits<KIND>;
+  constexpr int NEN = TR::NEN, NDF = TR::NDF, NDM = TR::NDM, NST = NEN * NDF;
+  const int q0 = blockIdx.x * 64 + threadIdx.x;
+  if (q0 >= cnt) return;
+  const int el = a.order[c0 + q0];
+  const int32_t* ixe = a.ix + (size_t)el * (NEN + 1);
+  const int32_t* iq = a.eload + (size_t)el * 7;
+  const int ma = ixe[NEN];
+  const double* em = a.e + (size_t)(ma - 1) * 15;
+  double x[NEN * NDM], p[NST];
+  for (int n = 0; n < NEN; ++n)
+    for (int d = 0; d < NDM; ++d) x[n * NDM + d] = a.x[(size_t)(ixe[n] - 1) * NDM + d];
+  for (int i = 0; i < NST; ++i) p[i] = 0.0;
+  bool any = false, bad = false;
+  if (KIND == K_HEX8 || KIND == K_TET4) {
+    double density = em[2];
+    if (density != 0.0) {
+      any = true;
+      density = em[2] * 1.0e-06;
+      const double gl[3] = {a.g0 * density, a.g1 * density, a.g2 * density};
+      double h[8], hx[8], hy[8], hz[8], det;
+      if (KIND == K_HEX8) {
+        for (int lz = 0; lz < 2; ++lz)
+          for (int ly = 0; ly < 2; ++ly)
+            for (int lx = 0; lx < 2; ++lx) {
+              sfhexa8_d(kPG2[lx], kPG2[ly], kPG2[lz], h, hx, hy, hz);
+              if (!jacob3d_d(hx, hy, hz, x, 8, &det)) bad = true;
+              const double w = 1.0 * 1.0 * 1.0 * det;
+              for (int i = 0; i < 8; ++i) {
+                const double wt1 = h[i] * w;
+                p[3 * i] = p[3 * i] - wt1 * gl[0];
+                p[3 * i + 1] = p[3 * i + 1] - wt1 * gl[1];
+                p[3 * i + 2] = p[3 * i + 2] - wt1 * gl[2];
+              }
+            }
+      } else {
+        if (!jtetra4_d(x, hx, hy, hz, &det, false)) bad = true;
+        const double wt1 = det * kDIV144;
+        for (int i = 0; i < 4; ++i) {
+          p[3 * i] = -wt1 * gl[0];
+          p[3 * i + 1] = -wt1 * gl[1];
+          p[3 * i + 2] = -wt1 * gl[2];
+        }
+      }
+    }
+  }
+  if (KIND == K_HEX8) {
+    const int face[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 4, 5, 1},
+                            {3, 2, 6, 7}, {0, 3, 7, 4}, {1, 5, 6, 2}};
+    for (int j = 0; j < 6; ++j) {
+      if (iq[j] <= 0) continue;
+      any = true;
+      double xf[12], r[3][3], ff[3];
+      for (int i = 0; i < 4; ++i)
+        for (int d = 0; d < 3; ++d) xf[3 * i + d] = x[3 * face[j][i] + d];
+      rotmat_d(xf, r);
+      for (int i = 0; i < 4; ++i) rot_d(xf + 3 * i, r);
+      tload_vec(a, iq[j], ff);
+      for (int ly = 0; ly < 2; ++ly)
+        for (int lx = 0; lx < 2; ++lx) {
+          double h[4], hx[4], hy[4], det;
+          sfquad4_d(kPG2[lx], kPG2[ly], h, hx, hy);
+          if (!jacob2d_d(hx, hy, xf, 4, 3, &det)) bad = true;
+          const double w = 1.0 * 1.0 * det;
+          for (int i = 0; i < 4; ++i) {
+            const int no = face[j][i];
+            const double wt1 = h[i] * w;
+            p[3 * no] = p[3 * no] - ff[0] * wt1;
+            p[3 * no + 1] = p[3 * no + 1] - ff[1] * wt1;
+            p[3 * no + 2] = p[3 * no + 2] - ff[2] * wt1;
+          }
+        }
+    }
+  } else if (KIND == K_TET4) {
+    const int face[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+    for (int j = 0; j < 4; ++j) {
+      if (iq[j] <= 0) continue;
+      any = true;
+      double xf[9], r[3][3], ff[3];
+      for (int i = 0; i < 3; ++i)
+        for (int d = 0; d < 3; ++d) xf[3 * i + d] = x[3 * face[j][i] + d];
+      rotmat_d(xf, r);
+      for (int i = 0; i < 3; ++i) rot_d(xf + 3 * i, r);
+      tload_vec(a, iq[j], ff);
+      const double j11 = xf[0] - xf[6], j12 = xf[1] - xf[7];
+      const double j21 = xf[3] - xf[6], j22 = xf[4] - xf[7];
+      const double det = j11 * j22 - j12 * j21;
+      if (det <= kJZERO) bad = true;
+      const double wt1 = det * kDIV6;
+      for (int i = 0; i < 3; ++i) {
+        const int no = face[j][i];
+        p[3 * no] = p[3 * no] - ff[0] * wt1;
+        p[3 * no + 1] = p[3 * no + 1] - ff[1] * wt1;
+        p[3 * no + 2] = p[3 * no + 2] - ff[2] * wt1;
+      }
+    }
+  } else {
+    const int nsides = NEN;  // tria3: 3 sides, quad4: 4 sides
+    for (int j = 0; j < nsides; ++j) {
+      if (iq[j] <= 0) continue;
+      any = true;
+      const int n0 = j, n1 = (j + 1) % NEN;
+      const double dx = x[2 * n1] - x[2 * n0], dy = x[2 * n1 + 1] - x[2 * n0 + 1];
+      double det = dx * dx + dy * dy;
+      if (det <= 0.0) bad = true;
+      det = sqrt(det);
+      double ff[3];
+      tload_vec(a, iq[j], ff);
+      const double wt1 = 0.5 * det;
+      for (int i = 0; i < 2; ++i) {
+        const int no = i == 0 ? n0 : n1;
+        p[2 * no] = p[2 * no] - ff[0] * wt1;
+        p[2 * no + 1] = p[2 * no + 1] - ff[1] * wt1;
+      }
+    }
+  }
+  if (bad) {
+    a.err[0] = el + 1;
+    return;
+  }
+  if (!any) return;
+  for (int i = 0; i < NST; ++i) {
+    const int k = a.id[(size_t)(ixe[i / NDF] - 1) * NDF + (i % NDF)];
+    if (k > 0 && k <= a.neq) a.b[k - 1] = a.b[k - 1] - p[i];
+  }
+}
+
+// ------------------------------------------------------------- host driver
+struct MeshDev {
+  int nnode = 0, numel = 0, nen = 0, ndf = 0, ndm = 0, kind = -1, nloads = 0, ncolors = 0;
+  std::vector<int> color_start;  // ncolors + 1 offsets into order
+  int32_t *ix = nullptr, *id = nullptr, *ie = nullptr, *eload = nullptr, *order = nullptr,
+          *slots = nullptr, *ltype = nullptr, *lnparc = nullptr, *err = nullptr;
+  double *x = nullptr, *e = nullptr, *lf = nullptr, *u = nullptr, *b = nullptr;
+};
+
+int color_elements(const int32_t* ix, int nnode, int numel, int nen, int nenv,
+                   std::vector<int32_t>& icolor);
+void element_slots(const int32_t* ix, const int32_t* id, int numel, int nen, int ndf, int neq,
+                   const int32_t* ia1, const int32_t* ja1, std::vector<int32_t>& slots);
+
+void free_mesh(MeshDev* m) {
+  if (!m) return;
+  void* ptrs[] = {m->ix, m->id, m->ie, m->eload, m->order, m->slots, m->ltype,
+                  m->lnparc, m->err, m->x, m->e, m->lf, m->u, m->b};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  delete m;
+}
+
+template <typename T>
+static hipError_t up(T** d, const T* h, size_t n, hipStream_t s) {
+  hipError_t e = hipMalloc((void**)d, sizeof(T) * std::max<size_t>(n, 1));
+  if (e != hipSuccess) return e;
+  if (n && h) return hipMemcpyAsync(*d, h, sizeof(T) * n, hipMemcpyHostToDevice, s);
+  return hipSuccess;
+}
+
+static AsmArgs make_args(Context* c, MeshDev* m) {
+  AsmArgs a;
+  a.ix = m->ix, a.x = m->x, a.id = m->id, a.ie = m->ie, a.e = m->e, a.eload = m->eload;
+  a.order = m->order, a.slots = m->slots, a.ltype = m->ltype, a.lnparc = m->lnparc, a.lf = m->lf;
+  a.u = nullptr, a.ad = c->A64.ad, a.al = c->A64.al, a.b = m->b, a.err = m->err;
+  a.nen = m->nen, a.ndf = m->ndf, a.ndm = m->ndm, a.neq = c->neq, a.nloads = m->nloads;
+  a.g0 = 0.0, a.g1 = 0.0, a.g2 = -9.81;
+  return a;
+}
+
+template <int KIND>
+static void launch_stiffness(const AsmArgs& a, const MeshDev* m, int lhs, int rhs, hipStream_t s) {
+  for (int c = 0; c < m->ncolors; ++c) {
+    int c0 = m->color_start[c], cnt = m->color_start[c + 1] - c0;
+    if (cnt > 0) hipLaunchKernelGGL(k_stiffness<KIND>, dim3(cnt), dim3(64), 0, s, a, c0, lhs, rhs);
+  }
+}
+
+template <int KIND>
+static void launch_loads(const AsmArgs& a, const MeshDev* m, hipStream_t s) {
+  for (int c = 0; c < m->ncolors; ++c) {
+    int c0 = m->color_start[c], cnt = m->color_start[c + 1] - c0;
+    if (cnt > 0) hipLaunchKernelGGL(k_loads<KIND>, dim3((cnt + 63) / 64), dim3(64), 0, s, a, c0, cnt);
+  }
+}
+
+}  // namespace elastic
+
+using namespace elastic;
+
+extern "C" int elastic_set_mesh(elastic_ctx* xc, int32_t nnode, int32_t numel, int32_t nen,
+                                int32_t ndf, int32_t ndm, int32_t numat, const int32_t* ix,
+                                const double* x, const int32_t* id, const int32_t* ie,
+                                const double* e, const int32_t* eload, int32_t nloads,
+                                const int32_t* ltype, const int32_t* lnparc,
+                                const double* lvals) {
+  if (!xc || nnode <= 0 || numel <= 0 || !ix || !x || !id || !ie || !e || numat <= 0)
+    return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  if (!c->tiles.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
+  hipSetDevice(c->device);
+  // element kind from the materials used (one kind per mesh)
+  int kind = -1;
+  for (int el = 0; el < numel; ++el) {
+    int ma = ix[(size_t)el * (nen + 1) + nen];
+    if (ma < 1 || ma > numat) return set_error(c, ELASTIC_BAD_ARG, "material number out of range");
+    int iel = ie[ma - 1], k;
+    switch (iel) {
+      case 2: case 3: k = K_TRIA3; break;
+      case 4: case 5: k = K_QUAD4; break;
+      case 6: k = K_TET4; break;
+      case 7: k = K_HEX8; break;
+      default: return set_error(c, ELASTIC_BAD_ARG, "element type " + std::to_string(iel) + " not on the GPU path");
+    }
+    if (kind == -1) kind = k;
+    else if (kind != k) return set_error(c, ELASTIC_BAD_ARG, "one element kind per mesh");
+  }
+  const int want_nen[] = {3, 4, 4, 8}, want_ndf[] = {2, 2, 3, 3}, want_ndm[] = {2, 2, 3, 3};
+  if (nen != want_nen[kind] || ndf != want_ndf[kind] || ndm != want_ndm[kind])
+    return set_error(c, ELASTIC_BAD_ARG, "nen/ndf/ndm do not match the element type");
+  std::vector<int32_t> icolor;
+  int ncol = color_elements(ix, nnode, numel, nen, nen, icolor);
+  if (ncol < 0) return set_error(c, ELASTIC_BAD_ARG, "more than 200 element colours");
+  std::vector<int32_t> order;
+  order.reserve(numel);
+  std::vector<int> cstart(ncol + 1, 0);
+  for (int cc = 1; cc <= ncol; ++cc) {
+    cstart[cc - 1] = (int)order.size();
+    for (int el = 1; el <= numel; ++el)
+      if (icolor[el] == cc) order.push_back(el - 1);
+  }
+  cstart[ncol] = (int)order.size();
+  std::vector<int32_t> slots;
+  try {
+    element_slots(ix, id, numel, nen, ndf, c->neq, xc->ia.data(), xc->ja.data(), slots);
+  } catch (const std::exception& ex) {
+    return set_error(c, ELASTIC_BAD_ARG, ex.what());
+  }
+  if (c->mesh) free_mesh((MeshDev*)c->mesh);
+  MeshDev* m = new MeshDev();
+  c->mesh = m;
+  m->nnode = nnode, m->numel = numel, m->nen = nen, m->ndf = ndf, m->ndm = ndm, m->kind = kind;
+  m->nloads = nloads, m->ncolors = ncol, m->color_start = cstart;
+  std::vector<int32_t> lt(std::max(nloads, 1), 0), ln(std::max(nloads, 1), 0);
+  std::vector<double> lv(3 * std::max(nloads, 1), 0.0);
+  for (int i = 0; i < nloads; ++i) {
+    lt[i] = ltype[i];
+    ln[i] = lnparc[i];
+    for (int q = 0; q < 3; ++q) lv[3 * i + q] = lvals[3 * i + q];
+  }
+  std::vector<int32_t> el7((size_t)numel * 7, 0);
+  if (eload) std::copy(eload, eload + (size_t)numel * 7, el7.begin());
+  hipStream_t s = c->stream;
+  hipError_t r = hipSuccess;
+  auto chk = [&](hipError_t q) { if (r == hipSuccess) r = q; };
+  chk(up(&m->ix, ix, (size_t)numel * (nen + 1), s));
+  chk(up(&m->x, x, (size_t)nnode * ndm, s));
+  chk(up(&m->id, id, (size_t)nnode * ndf, s));
+  chk(up(&m->ie, ie, (size_t)numat, s));
+  chk(up(&m->e, e, (size_t)numat * 15, s));
+  chk(up(&m->eload, el7.data(), el7.size(), s));
+  chk(up(&m->order, order.data(), order.size(), s));
+  chk(up(&m->slots, slots.data(), slots.size(), s));
+  chk(up(&m->ltype, lt.data(), lt.size(), s));
+  chk(up(&m->lnparc, ln.data(), ln.size(), s));
+  chk(up(&m->lf, lv.data(), lv.size(), s));
+  chk(up<double>(&m->u, nullptr, (size_t)nnode * ndf, s));
+  chk(up<double>(&m->b, nullptr, (size_t)c->neq, s));
+  chk(up<int32_t>(&m->err, nullptr, 1, s));
+  chk(hipStreamSynchronize(s));
+  if (r != hipSuccess) {
+    free_mesh(m);
+    c->mesh = nullptr;
+    return set_error(c, r == hipErrorOutOfMemory ? ELASTIC_OOM : ELASTIC_HIP_ERROR,
+                     std::string("set_mesh: ") + hipGetErrorString(r));
+  }
+  return ELASTIC_OK;
+}
+
+// Pform.f:1-254: isw 2 (lhs: K into ad/al; rhs: b -= K_e u_e) or isw 4 (loads
+// into b).  u is u(ndf,nnode) (may be NULL: zero); b is b(neq), in/out on the
+// host (may be NULL when rhs = 0).
+extern "C" int elastic_assemble(elastic_ctx* xc, int32_t isw, int32_t lhs, int32_t rhs,
+                                const double* u, double* b) {
+  if (!xc) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  MeshDev* m = (MeshDev*)c->mesh;
+  if (!m) return set_error(c, ELASTIC_NOT_READY, "mesh not set");
+  if (rhs && !b) return set_error(c, ELASTIC_BAD_ARG, "rhs needs b");
+  if (isw != 2 && isw != 4) return set_error(c, ELASTIC_BAD_ARG, "isw must be 2 or 4");
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  AsmArgs a = make_args(c, m);
+  if (u) {
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(m->u, u, sizeof(double) * m->nnode * m->ndf,
+                                        hipMemcpyHostToDevice, s));
+    a.u = m->u;
+  }
+  if (rhs)
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(m->b, b, sizeof(double) * c->neq, hipMemcpyHostToDevice, s));
+  ELASTIC_HIP_CHECK(c, hipMemsetAsync(m->err, 0, sizeof(int32_t), s));
+  if (isw == 2) {
+    if (lhs) {
+      ELASTIC_HIP_CHECK(c, hipMemsetAsync(c->A64.ad, 0, sizeof(double) * c->neq, s));
+      ELASTIC_HIP_CHECK(c, hipMemsetAsync(c->A64.al, 0, sizeof(double) * std::max<int64_t>(c->nad, 1), s));
+    }
+    switch (m->kind) {
+      case K_TRIA3: launch_stiffness<K_TRIA3>(a, m, lhs, rhs, s); break;
+      case K_QUAD4: launch_stiffness<K_QUAD4>(a, m, lhs, rhs, s); break;
+      case K_TET4: launch_stiffness<K_TET4>(a, m, lhs, rhs, s); break;
+      default: launch_stiffness<K_HEX8>(a, m, lhs, rhs, s); break;
+    }
+  } else if (rhs) {
+    switch (m->kind) {
+      case K_TRIA3: launch_loads<K_TRIA3>(a, m, s); break;
+      case K_QUAD4: launch_loads<K_QUAD4>(a, m, s); break;
+      case K_TET4: launch_loads<K_TET4>(a, m, s); break;
+      default: launch_loads<K_HEX8>(a, m, s); break;
+    }
+  }
+  int32_t err = 0;
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(&err, m->err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (rhs)
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(b, m->b, sizeof(double) * c->neq, hipMemcpyDeviceToHost, s));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
+  ELASTIC_HIP_CHECK(c, hipGetLastError());
+  if (err) return set_error(c, ELASTIC_NEG_JACOBIAN, "non-positive Jacobian in element " + std::to_string(err));
+  if (isw == 2 && lhs) {
+    // derived arrays of the new values (Jacobi inverse, FP32 copies)
+    std::vector<double> ad(c->neq);
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(ad.data(), c->A64.ad, sizeof(double) * c->neq, hipMemcpyDeviceToHost, s));
+    ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
+    c->zero_diag_row = -1;
+    for (int i = 0; i < c->neq; ++i)
+      if (ad[i] == 0.0) {
+        c->zero_diag_row = i + 1;
+        break;
+      }
+    launch_recip<double>(c->neq, c->A64.ad, c->A64.m, s);
+    launch_cast<double, float>(c->neq, c->A64.ad, c->A32.ad, s);
+    launch_cast<double, float>(c->nad, c->A64.al, c->A32.al, s);
+    launch_cast<double, float>(c->neq, c->A64.m, c->A32.m, s);
+    ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
+    c->values_set = true;
+    c->bdiag_set = false;
+  }
+  return ELASTIC_OK;
+}
+
+// Average device time (ms) of one stiffness assembly (isw 2, lhs) over reps.
+extern "C" int elastic_time_assembly(elastic_ctx* xc, int32_t reps, double* ms) {
+  if (!xc || reps <= 0 || !ms) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  MeshDev* m = (MeshDev*)c->mesh;
+  if (!m) return set_error(c, ELASTIC_NOT_READY, "mesh not set");
+  hipStream_t s = c->stream;
+  AsmArgs a = make_args(c, m);
+  ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev0, s));
+  for (int i = 0; i < reps; ++i) {
+    ELASTIC_HIP_CHECK(c, hipMemsetAsync(c->A64.ad, 0, sizeof(double) * c->neq, s));
+    ELASTIC_HIP_CHECK(c, hipMemsetAsync(c->A64.al, 0, sizeof(double) * std::max<int64_t>(c->nad, 1), s));
+    switch (m->kind) {
+      case K_TRIA3: launch_stiffness<K_TRIA3>(a, m, 1, 0, s); break;
+      case K_QUAD4: launch_stiffness<K_QUAD4>(a, m, 1, 0, s); break;
+      case K_TET4: launch_stiffness<K_TET4>(a, m, 1, 0, s); break;
+      default: launch_stiffness<K_HEX8>(a, m, 1, 0, s); break;
+    }
+  }
+  ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev1, s));
+  ELASTIC_HIP_CHECK(c, hipEventSynchronize(c->ev1));
+  float t = 0;
+  hipEventElapsedTime(&t, c->ev0, c->ev1);
+  *ms = t / reps;
+  return ELASTIC_OK;
+}

@@ -106,6 +106,8 @@ int elastic_destroy(elastic_ctx* x) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   free_system(c);
+  if (c->mesh) free_mesh((MeshDev*)c->mesh);
+  c->mesh = nullptr;
   if (c->scal) hipFree(c->scal);
   if (c->scal_host) hipHostFree(c->scal_host);
   if (c->red64) hipFree(c->red64);

@@ -110,7 +110,7 @@ struct Context {
   SolveScalars* scal_host = nullptr;  // pinned
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   // assembly state (optional)
-  struct Mesh* mesh = nullptr;
+  void* mesh = nullptr;  // MeshDev (assembly.hip)
   // autotuning/measurement
   double last_spmv_ms = 0.0;
   int check_every = 16;
@@ -118,6 +118,8 @@ struct Context {
 };
 
 int set_error(Context* c, int code, const std::string& msg);
+struct MeshDev;
+void free_mesh(MeshDev* m);
 #define ELASTIC_HIP_CHECK(ctx, expr)                                                   \
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \

@@ -99,6 +99,29 @@ int elastic_get_values(elastic_ctx* ctx, double* ad, double* al);
  * ELASTIC_MIXED (FP32 values and sums, result widened). */
 int elastic_spmv(elastic_ctx* ctx, int32_t precision, const double* x, double* y);
 
+/* ------------------------------------------------------- GPU assembly */
+/* Uploads the mesh for GPU assembly (after elastic_set_structure): element
+ * nodes ix(nen+1,numel), coordinates x(ndm,nnode), equation numbers
+ * id(ndf,nnode) as numeq left them, element type per material ie(numat) (2,3
+ * tria3, 4,5 quad4, 6 tet4, 7 hex8; one kind per mesh), material constants
+ * e(15,numat), element load numbers eload(7,numel) (may be NULL) and the load
+ * table (type, parcels, 3 values each; type 40 = traction vector, Pload.f:246).
+ * Builds the element colouring (Colormsh.f:9-308) and the CSRC slot map of
+ * every element. */
+int elastic_set_mesh(elastic_ctx* ctx, int32_t nnode, int32_t numel, int32_t nen, int32_t ndf,
+                     int32_t ndm, int32_t numat, const int32_t* ix, const double* x,
+                     const int32_t* id, const int32_t* ie, const double* e,
+                     const int32_t* eload, int32_t nloads, const int32_t* load_type,
+                     const int32_t* load_nparc, const double* load_vals);
+/* pform_mec (Pform.f:1-254) on the GPU: isw 2 with lhs assembles K into the
+ * device ad/al (zeroed first) and with rhs sets b = b - K_e u_e; isw 4 adds
+ * the element loads to b.  u(ndf,nnode) may be NULL (zero); b(neq) is in/out
+ * on the host.  Use elastic_get_values to copy K back. */
+int elastic_assemble(elastic_ctx* ctx, int32_t isw, int32_t lhs, int32_t rhs, const double* u,
+                     double* b);
+/* Average device time (ms) of one stiffness assembly (isw 2, lhs) over reps. */
+int elastic_time_assembly(elastic_ctx* ctx, int32_t reps, double* ms);
+
 /* ----------------------------------------------------------- solving */
 /* Solve A x = b.  Replaces solv -> pcg (Solv.f:13-139, Iters.f:252-524) for
  * solver 1, gmres (Iters.f:528-808) for 2 and pbicgstab (Iters.f:1079-1349)

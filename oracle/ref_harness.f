@@ -12,6 +12,7 @@ c *   mode 1 = PCG through solv (Solv.f:13), nth>1 -> pcg_omp          *
 c *   mode 2 = + gmres(k=50) called with its own signature (Iters.f:528)*
 c *   mode 4 = + pbicgstab with its own signature (Iters.f:1079)       *
 c *   mode 8 = + block_precond nblock=3 (Precond.f:163)                *
+c *   mode 16 = element loop by colours (omp_elmt, Pform.f:108-184)    *
 c *   (modes add up)                                                   *
 c * Output (raw little-endian stream files in outdir):                 *
 c *   meta.txt, inum.i4, id.i4, ia.i4, ja.i4, ad.f8, al.f8, b.f8,      *
@@ -78,8 +79,9 @@ c ... arguments
         call get_command_argument(7,arg)
         read(arg,*) maxmem
       endif
-      omp_elmt = .false.
+      omp_elmt = iand(mode,16) .ne. 0
       nth_elmt = 1
+      if (omp_elmt) nth_elmt = 2
       omp_solv = nth .gt. 1
       nth_solv = max(nth,1)
 c ... memory pool and deck (Main.f:300-330)

@@ -103,6 +103,14 @@ def main(only=None):
             out, stats = run_harness(dpath, tol, maxit, mode, wd)
             res = summarize(out, stats, full)
             res["tol"] = np.array(tol)
+            if full:
+                # the reference's coloured (OpenMP) element loop: same K and b
+                # up to the order in which element contributions are added
+                wd2 = os.path.join(tmp, name + "_omp")
+                os.makedirs(wd2)
+                out2, _ = run_harness(dpath, tol, maxit, 1 | 16, wd2)
+                for k in ("ad", "al", "b"):
+                    res[k + "_omp"] = out2[k]
             np.savez_compressed(os.path.join(OUT, name + ".npz"), **res)
             print(name, out["dims"].tolist(), stats)
         for name, rel in REF_DECKS.items():
