@@ -629,7 +629,7 @@ extern "C" int elastic_set_mesh(elastic_ctx* xc, int32_t nnode, int32_t numel, i
   if (!xc || nnode <= 0 || numel <= 0 || !ix || !x || !id || !ie || !e || numat <= 0)
     return ELASTIC_BAD_ARG;
   Context* c = &xc->c;
-  if (!c->tiles.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
+  if (!c->tiles.ntiles && !c->blk.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
   hipSetDevice(c->device);
   // element kind from the materials used (one kind per mesh)
   int kind = -1;
@@ -771,7 +771,7 @@ extern "C" int elastic_assemble(elastic_ctx* xc, int32_t isw, int32_t lhs, int32
     launch_recip<double>(c->neq, c->A64.ad, c->A64.m, s);
     launch_cast<double, float>(c->neq, c->A64.ad, c->A32.ad, s);
     launch_cast<double, float>(c->nad, c->A64.al, c->A32.al, s);
-    launch_cast<double, float>(c->neq, c->A64.m, c->A32.m, s);
+    launch_block_pack(c);
     ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
     c->values_set = true;
     c->bdiag_set = false;

@@ -49,19 +49,26 @@ static void free_tiles(DevTiles& t) {
   t = DevTiles();
 }
 
+static void free_blocks(DevBlocks& b) {
+  void* ptrs[] = {b.row0, b.nrows, b.nblk, b.ncols, b.ext0, b.seg0, b.rp0, b.blk0, b.word,
+                  b.rowptr, b.segptr, b.ext, b.inc_ptr, b.inc_idx, b.ia, b.v64, b.d64, b.v32,
+                  b.d32};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  b = DevBlocks();
+}
+
 static void free_system(Context* c) {
   free_tiles(c->tiles);
-  void* ptrs[] = {c->A64.ad, c->A64.al, c->A64.m, c->A64.bm, c->A32.ad, c->A32.al,
-                  c->A32.m,  c->A32.bm, c->epart64, c->epart32};
+  free_blocks(c->blk);
+  c->inc_ptr = c->inc_idx = nullptr;
+  void* ptrs[] = {c->A64.ad, c->A64.al, c->A64.m, c->A64.bm, c->A32.ad, c->A32.al, c->epart64};
   for (void* p : ptrs)
     if (p) hipFree(p);
   c->A64 = DevMatrix<double>();
   c->A32 = DevMatrix<float>();
   c->epart64 = nullptr;
-  c->epart32 = nullptr;
   for (auto& v : c->v64)
-    if (v) hipFree(v), v = nullptr;
-  for (auto& v : c->v32)
     if (v) hipFree(v), v = nullptr;
   c->values_set = c->values32 = c->bdiag_set = false;
 }
@@ -175,61 +182,105 @@ int elastic_set_structure(elastic_ctx* x, int32_t neq, int64_t nad, const int32_
   }
   hipStreamSynchronize(c->stream);
   free_system(c);
+  int nb = 1;
+  const char* lay = getenv("ELASTIC_LAYOUT");
+  if (c->allow_blocks && !(lay && std::string(lay) == "scalar")) nb = detect_node_blocks(neq, ia, ja);
+  if (const char* tb = getenv("ELASTIC_TILE_BLOCKS")) c->tile_blocks = std::max(32, atoi(tb));
   try {
-    build_tiles(neq, ia, ja, c->tile_nnz, c->layout);
+    if (nb > 1) build_block_tiles(neq, nb, ia, ja, c->tile_blocks, c->blayout);
+    else build_tiles(neq, ia, ja, c->tile_nnz, c->layout);
   } catch (const std::exception& e) {
     return set_error(c, ELASTIC_BAD_ARG, e.what());
   }
+  c->layout_nb = nb;
   x->ia.assign(ia, ia + neq + 1);
   x->ja.assign(ja, ja + nad);
   c->neq = neq;
   c->nad = nad;
-  TileLayout& L = c->layout;
-  DevTiles& t = c->tiles;
   int rc = 0;
-  rc = rc ? rc : upload(c, &t.row0, L.row0.data(), L.row0.size());
-  rc = rc ? rc : upload(c, &t.nrows, L.nrows.data(), L.nrows.size());
-  rc = rc ? rc : upload(c, &t.nnz, L.nnz.data(), L.nnz.size());
-  rc = rc ? rc : upload(c, &t.ncols, L.ncols.data(), L.ncols.size());
-  rc = rc ? rc : upload(c, &t.ext0, L.ext0.data(), L.ext0.size());
-  rc = rc ? rc : upload(c, &t.nz0, L.nz0.data(), L.nz0.size());
-  rc = rc ? rc : upload(c, &t.rp0, L.rp0.data(), L.rp0.size());
-  rc = rc ? rc : upload(c, &t.seg0, L.seg0.data(), L.seg0.size());
-  rc = rc ? rc : upload(c, &t.word, L.word.data(), L.word.size());
-  rc = rc ? rc : upload(c, &t.rowptr, L.rowptr.data(), L.rowptr.size());
-  rc = rc ? rc : upload(c, &t.segptr, L.segptr.data(), L.segptr.size());
-  rc = rc ? rc : upload(c, &t.ext, L.ext.data(), L.ext.size());
-  rc = rc ? rc : upload(c, &t.inc_ptr, L.inc_ptr.data(), L.inc_ptr.size());
-  rc = rc ? rc : upload(c, &t.inc_idx, L.inc_idx.data(), L.inc_idx.size());
-  t.ntiles = L.ntiles;
-  t.nnz_max = L.nnz_max;
-  t.ncols_max = L.ncols_max;
-  t.next_total = L.next_total;
+  int64_t nepart = 0;
+  if (nb == 1) {
+    TileLayout& L = c->layout;
+    DevTiles& t = c->tiles;
+    rc = rc ? rc : upload(c, &t.row0, L.row0.data(), L.row0.size());
+    rc = rc ? rc : upload(c, &t.nrows, L.nrows.data(), L.nrows.size());
+    rc = rc ? rc : upload(c, &t.nnz, L.nnz.data(), L.nnz.size());
+    rc = rc ? rc : upload(c, &t.ncols, L.ncols.data(), L.ncols.size());
+    rc = rc ? rc : upload(c, &t.ext0, L.ext0.data(), L.ext0.size());
+    rc = rc ? rc : upload(c, &t.nz0, L.nz0.data(), L.nz0.size());
+    rc = rc ? rc : upload(c, &t.rp0, L.rp0.data(), L.rp0.size());
+    rc = rc ? rc : upload(c, &t.seg0, L.seg0.data(), L.seg0.size());
+    rc = rc ? rc : upload(c, &t.word, L.word.data(), L.word.size());
+    rc = rc ? rc : upload(c, &t.rowptr, L.rowptr.data(), L.rowptr.size());
+    rc = rc ? rc : upload(c, &t.segptr, L.segptr.data(), L.segptr.size());
+    rc = rc ? rc : upload(c, &t.ext, L.ext.data(), L.ext.size());
+    rc = rc ? rc : upload(c, &t.inc_ptr, L.inc_ptr.data(), L.inc_ptr.size());
+    rc = rc ? rc : upload(c, &t.inc_idx, L.inc_idx.data(), L.inc_idx.size());
+    t.ntiles = L.ntiles;
+    t.nnz_max = L.nnz_max;
+    t.ncols_max = L.ncols_max;
+    t.next_total = L.next_total;
+    c->inc_ptr = t.inc_ptr;
+    c->inc_idx = t.inc_idx;
+    nepart = L.next_total;
+  } else {
+    BlockLayout& L = c->blayout;
+    DevBlocks& b = c->blk;
+    rc = rc ? rc : upload(c, &b.row0, L.row0.data(), L.row0.size());
+    rc = rc ? rc : upload(c, &b.nrows, L.nrows.data(), L.nrows.size());
+    rc = rc ? rc : upload(c, &b.nblk, L.nblk.data(), L.nblk.size());
+    rc = rc ? rc : upload(c, &b.ncols, L.ncols.data(), L.ncols.size());
+    rc = rc ? rc : upload(c, &b.ext0, L.ext0.data(), L.ext0.size());
+    rc = rc ? rc : upload(c, &b.seg0, L.seg0.data(), L.seg0.size());
+    rc = rc ? rc : upload(c, &b.rp0, L.rp0.data(), L.rp0.size());
+    rc = rc ? rc : upload(c, &b.blk0, L.blk0.data(), L.blk0.size());
+    rc = rc ? rc : upload(c, &b.word, L.word.data(), L.word.size());
+    rc = rc ? rc : upload(c, &b.rowptr, L.rowptr.data(), L.rowptr.size());
+    rc = rc ? rc : upload(c, &b.segptr, L.segptr.data(), L.segptr.size());
+    rc = rc ? rc : upload(c, &b.ext, L.ext.data(), L.ext.size());
+    rc = rc ? rc : upload(c, &b.inc_ptr, L.inc_ptr.data(), L.inc_ptr.size());
+    rc = rc ? rc : upload(c, &b.inc_idx, L.inc_idx.data(), L.inc_idx.size());
+    rc = rc ? rc : upload(c, &b.ia, ia, (size_t)neq + 1);
+    const size_t nvals = (size_t)L.nblocks * nb * nb, ndiag = (size_t)L.nn * (nb * (nb + 1) / 2);
+    rc = rc ? rc : dalloc(c, &b.v64, nvals);
+    rc = rc ? rc : dalloc(c, &b.v32, nvals);
+    rc = rc ? rc : dalloc(c, &b.d64, ndiag);
+    rc = rc ? rc : dalloc(c, &b.d32, ndiag);
+    b.nb = nb;
+    b.nn = L.nn;
+    b.ntiles = L.ntiles;
+    b.nblk_max = L.nblk_max;
+    b.ncols_max = L.ncols_max;
+    b.nblocks = L.nblocks;
+    b.next_nodes = L.next_nodes;
+    c->inc_ptr = b.inc_ptr;
+    c->inc_idx = b.inc_idx;
+    nepart = (int64_t)nb * L.next_nodes;
+  }
   rc = rc ? rc : dalloc(c, &c->A64.ad, neq);
   rc = rc ? rc : dalloc(c, &c->A64.al, (size_t)nad);
   rc = rc ? rc : dalloc(c, &c->A64.m, neq);
   rc = rc ? rc : dalloc(c, &c->A32.ad, neq);
   rc = rc ? rc : dalloc(c, &c->A32.al, (size_t)nad);
-  rc = rc ? rc : dalloc(c, &c->A32.m, neq);
-  rc = rc ? rc : dalloc(c, &c->epart64, (size_t)L.next_total);
-  rc = rc ? rc : dalloc(c, &c->epart32, (size_t)L.next_total);
+  rc = rc ? rc : dalloc(c, &c->epart64, (size_t)nepart);
   for (auto& v : c->v64) rc = rc ? rc : dalloc(c, &v, neq);
-  for (auto& v : c->v32) rc = rc ? rc : dalloc(c, &v, neq);
   if (rc) {
     free_system(c);
     return rc;
   }
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-  // the host layout is no longer needed except its sizes
-  L.word.clear();
-  L.word.shrink_to_fit();
+  // the host streams are no longer needed
+  c->layout.word.clear();
+  c->layout.word.shrink_to_fit();
+  c->blayout.word.clear();
+  c->blayout.word.shrink_to_fit();
   return ELASTIC_OK;
 }
 
 int elastic_set_values(elastic_ctx* x, const double* ad, const double* al) {
   if (!x || !ad || (x->c.nad > 0 && !al)) return ELASTIC_BAD_ARG;
   Context* c = &x->c;
-  if (!c->tiles.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
+  if (!c->tiles.ntiles && !c->blk.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
   hipSetDevice(c->device);
   c->zero_diag_row = -1;
   for (int i = 0; i < c->neq; ++i)
@@ -243,7 +294,7 @@ int elastic_set_values(elastic_ctx* x, const double* ad, const double* al) {
   launch_recip<double>(c->neq, c->A64.ad, c->A64.m, c->stream);
   launch_cast<double, float>(c->neq, c->A64.ad, c->A32.ad, c->stream);
   launch_cast<double, float>(c->nad, c->A64.al, c->A32.al, c->stream);
-  launch_cast<double, float>(c->neq, c->A64.m, c->A32.m, c->stream);
+  launch_block_pack(c);
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   c->values_set = true;
   c->bdiag_set = false;
@@ -262,16 +313,8 @@ int elastic_get_values(elastic_ctx* x, double* ad, double* al) {
 }
 
 static int spmv_resident(Context* c, int precision, const double* x64, double* y64) {
-  if (precision == ELASTIC_MIXED) {
-    launch_cast<double, float>(c->neq, x64, c->v32[VX], c->stream);
-    launch_spmv_tiles<float>(c->tiles, c->A32.ad, c->A32.al, c->v32[VX], c->v32[VQ], c->epart32,
-                             c->stream);
-    launch_merge<float>(c->tiles, c->neq, c->epart32, c->v32[VQ], c->stream);
-    launch_cast<float, double>(c->neq, c->v32[VQ], y64, c->stream);
-  } else {
-    launch_spmv_tiles<double>(c->tiles, c->A64.ad, c->A64.al, x64, y64, c->epart64, c->stream);
-    launch_merge<double>(c->tiles, c->neq, c->epart64, y64, c->stream);
-  }
+  launch_spmv(c, precision == ELASTIC_MIXED, x64, y64, c->epart64);
+  launch_merge(c, c->epart64, y64);
   return ELASTIC_OK;
 }
 
@@ -295,18 +338,8 @@ int elastic_time_spmv(elastic_ctx* x, int32_t precision, int32_t reps, double* t
   if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
   hipSetDevice(c->device);
   const bool f32 = precision == ELASTIC_MIXED;
-  auto tiles = [&]() {
-    if (f32)
-      launch_spmv_tiles<float>(c->tiles, c->A32.ad, c->A32.al, c->v32[VX], c->v32[VQ], c->epart32,
-                               c->stream);
-    else
-      launch_spmv_tiles<double>(c->tiles, c->A64.ad, c->A64.al, c->v64[VT], c->v64[VU],
-                                c->epart64, c->stream);
-  };
-  auto merge = [&]() {
-    if (f32) launch_merge<float>(c->tiles, c->neq, c->epart32, c->v32[VQ], c->stream);
-    else launch_merge<double>(c->tiles, c->neq, c->epart64, c->v64[VU], c->stream);
-  };
+  auto tiles = [&]() { launch_spmv(c, f32, c->v64[VT], c->v64[VU], c->epart64); };
+  auto merge = [&]() { launch_merge(c, c->epart64, c->v64[VU]); };
   for (int i = 0; i < 3; ++i) tiles(), merge();  // warm-up
   ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < reps; ++i) tiles();
@@ -326,17 +359,24 @@ int elastic_time_spmv(elastic_ctx* x, int32_t precision, int32_t reps, double* t
 int elastic_layout_info(elastic_ctx* x, int64_t* info) {
   if (!x || !info) return ELASTIC_BAD_ARG;
   Context* c = &x->c;
-  info[0] = c->tiles.ntiles;
-  info[1] = c->tiles.nnz_max;
-  info[2] = c->tiles.next_total;
-  info[3] = c->tiles.ncols_max;
+  if (c->layout_nb > 1) {
+    info[0] = c->blk.ntiles;
+    info[1] = c->blk.nblk_max;
+    info[2] = c->blk.next_nodes;
+    info[3] = c->layout_nb;
+  } else {
+    info[0] = c->tiles.ntiles;
+    info[1] = c->tiles.nnz_max;
+    info[2] = c->tiles.next_total;
+    info[3] = 1;
+  }
   return ELASTIC_OK;
 }
 
 int elastic_upload_rhs(elastic_ctx* x, const double* b) {
   if (!x || !b) return ELASTIC_BAD_ARG;
   Context* c = &x->c;
-  if (!c->tiles.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
+  if (!c->tiles.ntiles && !c->blk.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
   hipSetDevice(c->device);
   ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->v64[VB], b, sizeof(double) * c->neq, hipMemcpyHostToDevice, c->stream));
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -362,9 +402,8 @@ int ensure_bdiag(elastic_ctx* x);
 // statistics of Iters.f:435-457 on the resident x (FP64 operator)
 static int post_stats(Context* c, elastic_stats* st) {
   double* out = c->red64 + 8 * kRedBlocks;
-  launch_spmv_tiles<double>(c->tiles, c->A64.ad, c->A64.al, c->v64[VX], c->v64[VQ], c->epart64,
-                            c->stream);
-  launch_post(c->tiles, c->neq, c->epart64, c->v64[VX], c->v64[VQ], c->v64[VB], c->A64.m, c->red64,
+  launch_spmv(c, false, c->v64[VX], c->v64[VQ], c->epart64);
+  launch_post(c->inc_ptr, c->inc_idx, c->neq, c->epart64, c->v64[VX], c->v64[VQ], c->v64[VB], c->A64.m, c->red64,
               out, c->stream);
   double h[4];
   ELASTIC_HIP_CHECK(c, hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -381,70 +420,30 @@ static int solve_pcg(Context* c, int precond, int precision, double tol, int max
   hipStream_t s = c->stream;
   double* dslots = c->red64 + 8 * kRedBlocks + 8;
   const bool bd = precond == ELASTIC_PRECOND_BDIAG3;
-  int total_it = 0, outer = 0, spmvs = 0;
+  PcgVecs V{c->v64[VX], c->v64[VR], c->v64[VP],      c->v64[VQ], c->v64[VB],
+            c->A64.m,   bd ? c->A64.bm : nullptr, c->v64[VT], c->epart64};
   ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev0, s));
-  if (precision != ELASTIC_MIXED) {
-    PcgBuffers<double> B{c->v64[VX], c->v64[VR], c->v64[VP], c->v64[VQ], c->v64[VB],
-                         c->A64.m,   bd ? c->A64.bm : nullptr, c->A64.ad, c->A64.al, c->epart64};
-    int rc = pcg_device<double>(c, B, tol, maxit, dslots);
-    if (rc) return rc;
-    total_it = c->scal_host->iter;
-    spmvs = total_it;
-    st->conv = c->scal_host->conv;
-    outer = 1;
-    if (!c->scal_host->done) st->status = ELASTIC_MAXIT;
+  int rc;
+  if (precision == ELASTIC_MIXED) {
+    double delta = 0.1;
+    if (const char* e = getenv("ELASTIC_RELIABLE_DELTA")) delta = atof(e);
+    rc = pcg_run(c, V, true, tol, maxit, true, delta, dslots);
   } else {
-    // FP64 iterative refinement around FP32 PCG (inner relative tolerance)
-    double inner_tol = 1e-5;
-    if (const char* e = getenv("ELASTIC_INNER_TOL")) inner_tol = atof(e);
-    int max_outer = 60;
-    double* r64 = c->v64[VR];
-    double* q64 = c->v64[VQ];
-    // x64 = 0 ; r64 = b ; b32 = b ; b.M b
-    ELASTIC_HIP_CHECK(c, hipMemsetAsync(c->v64[VX], 0, sizeof(double) * c->neq, s));
-    ELASTIC_HIP_CHECK(c, hipMemsetAsync(q64, 0, sizeof(double) * c->neq, s));
-    ELASTIC_HIP_CHECK(c, hipMemsetAsync(c->epart64, 0, sizeof(double) * std::max<int64_t>(1, c->tiles.next_total), s));
-    launch_ir_residual(c->tiles, c->neq, c->epart64, q64, c->v64[VB], c->A64.m, r64,
-                       c->v32[VB], c->red64, s);
-    double* out = c->red64 + 8 * kRedBlocks;
-    launch_sum_parts(c->red64, 1, out, s);
-    double bmb = 0;
-    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(&bmb, out, sizeof(double), hipMemcpyDeviceToHost, s));
-    ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
-    const double conv = tol * std::sqrt(std::fabs(bmb));
-    st->conv = conv;
-    double rmr = bmb;
-    bool ok = (bmb == 0.0);
-    while (!ok && outer < max_outer && total_it < maxit) {
-      PcgBuffers<float> B{c->v32[VX], c->v32[VR], c->v32[VP], c->v32[VQ], c->v32[VB],
-                          c->A32.m,   bd ? c->A32.bm : nullptr, c->A32.ad, c->A32.al, c->epart32};
-      int rc = pcg_device<float>(c, B, inner_tol, maxit - total_it, dslots);
-      if (rc) return rc;
-      total_it += c->scal_host->iter;
-      spmvs += c->scal_host->iter;
-      ++outer;
-      launch_ir_correct(c->neq, c->v32[VX], c->v64[VX], s);
-      launch_spmv_tiles<double>(c->tiles, c->A64.ad, c->A64.al, c->v64[VX], q64, c->epart64, s);
-      ++spmvs;
-      launch_ir_residual(c->tiles, c->neq, c->epart64, q64, c->v64[VB], c->A64.m, r64,
-                         c->v32[VB], c->red64, s);
-      launch_sum_parts(c->red64, 1, out, s);
-      ELASTIC_HIP_CHECK(c, hipMemcpyAsync(&rmr, out, sizeof(double), hipMemcpyDeviceToHost, s));
-      ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
-      ok = std::sqrt(std::fabs(rmr)) < conv;
-      if (c->scal_host->iter == 0 && !ok) break;  // no progress possible
-    }
-    if (!ok) st->status = ELASTIC_MAXIT;
+    rc = pcg_run(c, V, false, tol, maxit, false, 0.0, dslots);
   }
+  if (rc) return rc;
   ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev1, s));
   ELASTIC_HIP_CHECK(c, hipEventSynchronize(c->ev1));
   float ms = 0;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  const SolveScalars& h = *c->scal_host;
   st->t_device_ms = ms;
-  st->iters = total_it;
-  st->outer = outer;
-  st->spmv_count = spmvs;
-  int rc = post_stats(c, st);
+  st->iters = h.iter;
+  st->outer = precision == ELASTIC_MIXED ? h.nupd : 1;
+  st->spmv_count = h.iter + (precision == ELASTIC_MIXED ? h.nupd : 0);
+  st->conv = h.conv;
+  if (!h.done) st->status = ELASTIC_MAXIT;
+  rc = post_stats(c, st);
   if (rc) return rc;
   st->rel_res = st->conv > 0 ? st->norm_m_r / (st->conv / tol) : 0.0;
   ELASTIC_HIP_CHECK(c, hipGetLastError());

@@ -60,6 +60,43 @@ struct TileLayout {
 // per tile (<= 3840 so that nrows + ncols fit the index fields).
 void build_tiles(int neq, const int32_t* ia1, const int32_t* ja1, int tile_nnz, TileLayout& L);
 
+// ---------------------------------------------------- node-blocked row tiles
+// (layout_block.cpp; see the comment there)
+constexpr int kBlkRowsMax = 127;
+constexpr int kBlkTileMax = 1024;
+
+struct BlockLayout {
+  int nb = 1, nn = 0, ntiles = 0, nblk_max = 0, ncols_max = 0;
+  int64_t nblocks = 0, next_nodes = 0;
+  std::vector<int32_t> row0, nrows, nblk, ncols, ext0, seg0, rp0;
+  std::vector<int64_t> blk0;
+  std::vector<uint32_t> word;
+  std::vector<uint16_t> rowptr, segptr;
+  std::vector<int32_t> ext, inc_ptr, inc_idx;
+};
+
+// 3 (or 2) when the CSRC pattern is a pattern of node blocks, else 1
+int detect_node_blocks(int neq, const int32_t* ia1, const int32_t* ja1);
+void build_block_tiles(int neq, int nb, const int32_t* ia1, const int32_t* ja1, int tile_blocks,
+                       BlockLayout& L);
+
+struct DevBlocks {
+  int nb = 1, nn = 0, ntiles = 0, nblk_max = 0, ncols_max = 0;
+  int64_t nblocks = 0, next_nodes = 0;
+  int32_t *row0 = nullptr, *nrows = nullptr, *nblk = nullptr, *ncols = nullptr, *ext0 = nullptr,
+          *seg0 = nullptr, *rp0 = nullptr;
+  int64_t* blk0 = nullptr;
+  uint32_t* word = nullptr;
+  uint16_t *rowptr = nullptr, *segptr = nullptr;
+  int32_t *ext = nullptr, *inc_ptr = nullptr, *inc_idx = nullptr;
+  int32_t* ia = nullptr;  // device copy of ia (value pack)
+  // values: plane p of block q of tile t at blk0[t]*nb*nb + p*nblk[t] + q;
+  // node diagonal blocks: entry c of node I at c*nn + I (nb diagonal entries,
+  // then the in-node lower entries a21, a31, a32)
+  double *v64 = nullptr, *d64 = nullptr;
+  float *v32 = nullptr, *d32 = nullptr;
+};
+
 // -------------------------------------------------------------- device state
 template <typename T>
 struct DevMatrix {
@@ -85,6 +122,9 @@ struct DevTiles {
 struct SolveScalars {
   double d, di, alpha, beta, conv, pq, rz0, norm_b;
   int32_t iter, done, maxit, status;
+  // reliable-update mixed precision (FP32 iteration, FP64 residual replacement)
+  double rmax, delta, true_rmr;
+  int32_t reliable, upd, skip64, nupd;
 };
 
 constexpr int kRedBlocks = 1024;  // fixed reduction grid (deterministic sums)
@@ -96,16 +136,21 @@ struct Context {
   int neq = 0;
   int64_t nad = 0;
   int tile_nnz = 2048;
+  int tile_blocks = 512;
+  int layout_nb = 1;           // 1: scalar tiles; 2 or 3: node-blocked tiles
+  bool allow_blocks = true;
   TileLayout layout;
   DevTiles tiles;
+  BlockLayout blayout;
+  DevBlocks blk;
+  const int32_t* inc_ptr = nullptr;  // merge lists of the active layout
+  const int32_t* inc_idx = nullptr;
   DevMatrix<double> A64;
   DevMatrix<float> A32;
   bool values_set = false, values32 = false, bdiag_set = false;
   // workspace
   double *epart64 = nullptr, *red64 = nullptr;
-  float* epart32 = nullptr;
   double *v64[8] = {nullptr};
-  float* v32[6] = {nullptr};
   SolveScalars* scal = nullptr;
   SolveScalars* scal_host = nullptr;  // pinned
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
@@ -129,31 +174,35 @@ void free_mesh(MeshDev* m);
   } while (0)
 
 // kernels (spmv.hip / vector.hip)
-template <typename T>
-void launch_spmv_tiles(const DevTiles& t, const T* ad, const T* al, const T* x, T* y, T* epart,
+template <typename TV, typename T>
+void launch_spmv_tiles(const DevTiles& t, const TV* ad, const TV* al, const T* x, T* y, T* epart,
                        hipStream_t s, const int32_t* skip = nullptr);
-template <typename T>
-void launch_merge(const DevTiles& t, int neq, const T* epart, T* y, hipStream_t s);
+void launch_merge(Context* c, const double* epart, double* y);
 template <typename S, typename D>
 void launch_cast(int64_t n, const S* a, D* b, hipStream_t s);
+// y = A x with the active layout; fp32 selects the FP32-value operator.  The
+// external partials are left in epart for the merge lists c->inc_ptr/inc_idx.
+void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epart,
+                 const int32_t* skip = nullptr);
+// gathers the CSRC values into the node-blocked planes (after every assembly)
+void launch_block_pack(Context* c);
 template <typename T>
 void launch_recip(int n, const T* ad, T* m, hipStream_t s);
-void launch_post(const DevTiles& t, int n, const double* epart, const double* x, double* y,
-                 const double* b, const double* m, double* part, double* out4, hipStream_t s);
+void launch_post(const int32_t* inc_ptr, const int32_t* inc_idx, int n, const double* epart,
+                 const double* x, double* y, const double* b, const double* m, double* part,
+                 double* out4, hipStream_t s);
 void launch_sum_parts(const double* part, int nsum, double* out, hipStream_t s);
-void launch_ir_correct(int n, const float* e, double* x, hipStream_t s);
-void launch_ir_residual(const DevTiles& t, int n, const double* epart, const double* y,
-                        const double* b, const double* m, double* r, float* r32, double* part,
-                        hipStream_t s);
 
-// pcg.hip: buffers of one PCG solve in precision T (all device pointers)
-template <typename T>
-struct PcgBuffers {
-  T *x, *r, *p, *q, *b, *m, *bm, *ad, *al, *epart;
+// pcg.hip: one PCG solve on device-resident FP64 vectors.  The operator's
+// values are TV (double, or float for the mixed-precision method); with
+// `reliable` the recursively updated residual is replaced by the FP64 true
+// residual b - A64 x (ad64/al64) whenever it has dropped by `delta`, and the
+// stopping test is made on that true residual.
+struct PcgVecs {
+  double *x, *r, *p, *q, *b, *m, *bm, *t, *epart;
 };
-template <typename T>
-int pcg_device(Context* c, const PcgBuffers<T>& B, double tol, int maxit, double* dslots);
-
+int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int maxit,
+            bool reliable, double delta, double* dslots);
 }  // namespace elastic
 
 // the opaque handle of the C-ABI

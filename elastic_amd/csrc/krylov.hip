@@ -60,11 +60,9 @@ int ensure_bdiag(elastic_ctx* x) {
   ELASTIC_HIP_CHECK(c, hipMemcpyAsync(dslots, slots.data(), sizeof(int64_t) * slots.size(),
                                       hipMemcpyHostToDevice, c->stream));
   if (!c->A64.bm) ELASTIC_HIP_CHECK(c, hipMalloc((void**)&c->A64.bm, sizeof(double) * 6 * nb));
-  if (!c->A32.bm) ELASTIC_HIP_CHECK(c, hipMalloc((void**)&c->A32.bm, sizeof(float) * 6 * nb));
   int g = std::min((nb + 255) / 256, 4096);
   hipLaunchKernelGGL(k_bdiag<double>, dim3(g), dim3(256), 0, c->stream, nb, c->A64.ad, c->A64.al,
                      dslots, c->A64.bm);
-  launch_cast<double, float>(6 * (int64_t)nb, c->A64.bm, c->A32.bm, c->stream);
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   hipFree(dslots);
   c->bdiag_set = true;

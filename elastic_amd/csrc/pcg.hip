@@ -1,19 +1,24 @@
-// Device-resident Krylov iterations on gfx950.
+// Device-resident PCG on gfx950.
 //
 // PCG follows Iters.f:252-524 step for step (x0 = 0, Jacobi or 3x3
 // block-Jacobi M, stop when sqrt|r.M r| < tol sqrt|b.M b|).  One iteration is
 // four kernels and no host synchronisation:
-//   spmv tiles (q = A p)   ->  merge + p.q   ->  x,r update + r.z  ->  p update
+//   spmv tiles (q = A p)  ->  merge + p.q  ->  x, r update + r.z  ->  p update
 // Dot products are reduced in two fixed-order stages (per-block partials over a
 // fixed grid of kRedBlocks blocks, then every consumer block sums the partials
-// in the same order), so each block sees the same alpha/beta and results are
-// bit-reproducible run to run.  Per-thread products are formed in the vector
-// type T; block and grid sums are kept in FP64 for both T = double and
-// T = float.  Scalars live in device memory (SolveScalars); kernels return at
-// once when the solve has converged, so the host only polls a flag every
-// `check_every` iterations.
+// in the same order), so every block sees the same alpha and beta and the
+// results are reproducible run to run.  Vectors, dots and scalars are FP64.
+// Scalars live in device memory (SolveScalars); kernels return at once when
+// the solve has converged, so the host only polls a flag every `check_every`
+// iterations.
+//
+// Mixed precision (TV = float): the operator of the iteration reads FP32
+// matrix values (products and sums in FP64).  Whenever the recursively updated
+// residual has dropped by `delta` since the last replacement, it is replaced by
+// the FP64 true residual r = b - A64 x (an FP64 product with the FP64 values)
+// and the search direction is kept, so the Krylov iteration is not restarted;
+// convergence is decided on that true residual only.
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 
 #include "elastic_internal.h"
@@ -34,11 +39,11 @@ __device__ inline double block_sum(double v, double* sh) {
   return r;  // valid in thread 0
 }
 
-// Reads the solve's done flag once per block (uniform branch for every wave).
-__device__ inline bool block_done(const SolveScalars* sc, double* sh) {
-  if (threadIdx.x == 0) sh[5] = (double)sc->done;
+// Reads a flag once per block (uniform branch for every wave).
+__device__ inline bool block_flag(const int32_t* f, double* sh) {
+  if (threadIdx.x == 0) sh[5] = (double)*f;
   __syncthreads();
-  bool d = sh[5] != 0.0;
+  const bool d = sh[5] != 0.0;
   __syncthreads();
   return d;
 }
@@ -58,205 +63,263 @@ __device__ inline double grid_sum(const double* __restrict__ part, double* sh) {
 // z = M^-1 r at row i: Jacobi (m = 1/ad) or, when bm != null, the 3x3 block
 // inverse of consecutive equation triples (Precond.f:419-540); rows past the
 // last full triple use 1/ad.
-template <typename T>
-__device__ inline T apply_m(int i, int n, const T* __restrict__ r, const T* __restrict__ m,
-                            const T* __restrict__ bm) {
+__device__ inline double apply_m(int i, int n, const double* __restrict__ r,
+                                 const double* __restrict__ m, const double* __restrict__ bm) {
   if (bm == nullptr) return r[i] * m[i];
   const int b = i / 3, nb = n / 3;
   if (b >= nb) return r[i] * m[i];
   const int i0 = 3 * b, c = i - i0;
-  const T* a = bm + 6 * b;  // c11 c22 c33 c21 c31 c32
-  const T x1 = r[i0], x2 = r[i0 + 1], x3 = r[i0 + 2];
+  const double* a = bm + 6 * b;  // c11 c22 c33 c21 c31 c32
+  const double x1 = r[i0], x2 = r[i0 + 1], x3 = r[i0 + 2];
   if (c == 0) return a[0] * x1 + a[3] * x2 + a[4] * x3;
   if (c == 1) return a[3] * x1 + a[1] * x2 + a[5] * x3;
   return a[4] * x1 + a[5] * x2 + a[2] * x3;
 }
 
+#define GRID_LOOP(i, n) \
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < (n); i += gridDim.x * kThreads)
+
 // x = 0, r = b, p = z = M^-1 b, partial b.z
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const T* __restrict__ b,
-                                                        const T* __restrict__ m,
-                                                        const T* __restrict__ bm, T* x, T* r,
-                                                        T* p, double* part) {
+__global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const double* __restrict__ b,
+                                                        const double* __restrict__ m,
+                                                        const double* __restrict__ bm, double* x,
+                                                        double* r, double* p, double* part) {
   __shared__ double sh[8];
   double acc = 0.0;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-    T zi = apply_m(i, n, b, m, bm);
-    x[i] = T(0);
+  GRID_LOOP(i, n) {
+    const double zi = apply_m(i, n, b, m, bm);
+    x[i] = 0.0;
     r[i] = b[i];
     p[i] = zi;
-    acc += (double)b[i] * (double)zi;
+    acc += b[i] * zi;
   }
-  double s = block_sum(acc, sh);
+  const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__global__ void k_pcg_init2(const double* __restrict__ part, double tol, double* dslot,
-                            SolveScalars* sc) {
+__global__ void k_pcg_init2(const double* __restrict__ part, double tol, double delta,
+                            int reliable, double* dslot, SolveScalars* sc) {
   __shared__ double sh[8];
-  double d = grid_sum(part, sh);
+  const double d = grid_sum(part, sh);
   if (threadIdx.x == 0) {
     *dslot = d;
     sc->conv = tol * sqrt(fabs(d));
     sc->norm_b = sqrt(fabs(d));
     sc->rz0 = d;
+    sc->rmax = sqrt(fabs(d));
+    sc->delta = delta;
+    sc->true_rmr = d;
     sc->iter = 0;
     sc->status = 0;
+    sc->reliable = reliable;
+    sc->upd = 0;
+    sc->skip64 = 1;
+    sc->nupd = 0;
     sc->done = (d == 0.0) ? 1 : 0;  // b = 0: x = 0 is the solution
   }
 }
 
 // q += merged external partials; partial p.q
-template <typename T>
 __global__ __launch_bounds__(kThreads) void k_merge_dot(int n, const int32_t* __restrict__ inc_ptr,
                                                          const int32_t* __restrict__ inc_idx,
-                                                         const T* __restrict__ epart, T* q,
-                                                         const T* __restrict__ p, double* part,
-                                                         const SolveScalars* sc) {
+                                                         const double* __restrict__ epart,
+                                                         double* q, const double* __restrict__ p,
+                                                         double* part, const SolveScalars* sc) {
   __shared__ double sh[8];
-  if (block_done(sc, sh)) return;
+  if (block_flag(&sc->done, sh)) return;
   double acc = 0.0;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-    T s = q[i];
+  GRID_LOOP(i, n) {
+    double s = q[i];
     for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
     q[i] = s;
-    acc += (double)p[i] * (double)s;
+    acc += p[i] * s;
   }
-  double s = block_sum(acc, sh);
+  const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// alpha = d / p.q ; x += alpha p ; r -= alpha q ; partial r.(M^-1 r)
-template <typename T>
+// alpha = d / p.q ; x += alpha p ; r -= alpha q ; with Jacobi also partial r.(m r)
 __global__ __launch_bounds__(kThreads) void k_pcg_update(int n, const double* __restrict__ part_pq,
-                                                          const double* __restrict__ dslot, T* x,
-                                                          const T* __restrict__ p, T* r,
-                                                          const T* __restrict__ q,
-                                                          double* part_rz, SolveScalars* sc) {
+                                                          const double* __restrict__ dslot,
+                                                          double* x, const double* __restrict__ p,
+                                                          double* r, const double* __restrict__ q,
+                                                          const double* __restrict__ m,
+                                                          int jacobi, double* part_rz,
+                                                          SolveScalars* sc) {
   __shared__ double sh[8];
-  if (block_done(sc, sh)) return;
+  if (block_flag(&sc->done, sh)) return;
   const double pq = grid_sum(part_pq, sh);
   const double alpha = *dslot / pq;
-  const T a = (T)alpha;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-    x[i] = x[i] + a * p[i];
-    r[i] = r[i] - a * q[i];
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc->alpha = alpha;
-    sc->pq = pq;
-  }
-  (void)part_rz;
-}
-
-// partial r.z (after every r entry of the update is written: separate kernel
-// for the block preconditioner, fused for Jacobi)
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_rz(int n, const T* __restrict__ r,
-                                                  const T* __restrict__ m,
-                                                  const T* __restrict__ bm, double* part,
-                                                  const SolveScalars* sc) {
-  __shared__ double sh[8];
-  if (block_done(sc, sh)) return;
   double acc = 0.0;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
-    acc += (double)r[i] * (double)apply_m(i, n, r, m, bm);
-  double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-// Jacobi-fused variant of update + r.z (z_i needs only r_i)
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_pcg_update_jacobi(
-    int n, const double* __restrict__ part_pq, const double* __restrict__ dslot, T* x,
-    const T* __restrict__ p, T* r, const T* __restrict__ q, const T* __restrict__ m,
-    double* part_rz, SolveScalars* sc) {
-  __shared__ double sh[8];
-  if (block_done(sc, sh)) return;
-  const double pq = grid_sum(part_pq, sh);
-  const double alpha = *dslot / pq;
-  const T a = (T)alpha;
-  double acc = 0.0;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-    x[i] = x[i] + a * p[i];
-    const T ri = r[i] - a * q[i];
+  GRID_LOOP(i, n) {
+    x[i] = x[i] + alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
     r[i] = ri;
-    acc += (double)ri * (double)(ri * m[i]);
+    acc += ri * (ri * m[i]);
   }
-  double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) part_rz[blockIdx.x] = s;
+  if (jacobi) {
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part_rz[blockIdx.x] = s;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->alpha = alpha;
     sc->pq = pq;
   }
+}
+
+// partial r.(M^-1 r) for the block preconditioner (after r is complete)
+__global__ __launch_bounds__(kThreads) void k_rz(int n, const double* __restrict__ r,
+                                                  const double* __restrict__ m,
+                                                  const double* __restrict__ bm, double* part,
+                                                  const int32_t* gate, const SolveScalars* sc) {
+  __shared__ double sh[8];
+  if (block_flag(&sc->done, sh)) return;
+  if (gate && !block_flag(gate, sh)) return;
+  double acc = 0.0;
+  GRID_LOOP(i, n) acc += r[i] * apply_m(i, n, r, m, bm);
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // di = r.z ; beta = di / d ; p = z + beta p ; bookkeeping in block 0
-template <typename T>
 __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* __restrict__ part_rz,
                                                            const double* __restrict__ dold,
-                                                           double* dnew, T* p,
-                                                           const T* __restrict__ r,
-                                                           const T* __restrict__ m,
-                                                           const T* __restrict__ bm,
+                                                           double* dnew, double* p,
+                                                           const double* __restrict__ r,
+                                                           const double* __restrict__ m,
+                                                           const double* __restrict__ bm,
                                                            SolveScalars* sc) {
   __shared__ double sh[8];
-  if (block_done(sc, sh)) return;
+  if (block_flag(&sc->done, sh)) return;
   const double di = grid_sum(part_rz, sh);
   const double beta = di / *dold;
-  const T bt = (T)beta;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
-    p[i] = apply_m(i, n, r, m, bm) + bt * p[i];
+  GRID_LOOP(i, n) p[i] = apply_m(i, n, r, m, bm) + beta * p[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *dnew = di;
     sc->beta = beta;
     sc->di = di;
     sc->iter += 1;
-    if (sqrt(fabs(di)) < sc->conv) sc->done = 1;
+    const double rn = sqrt(fabs(di));
+    if (!sc->reliable) {
+      if (rn < sc->conv) sc->done = 1;
+    } else if (rn < sc->delta * sc->rmax || rn < sc->conv) {
+      sc->upd = 1;  // true-residual replacement at the end of this chunk
+      sc->skip64 = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------- residual replacement
+__global__ void k_rel_force(SolveScalars* sc) {
+  if (threadIdx.x == 0 && !sc->done) {
+    sc->upd = 1;
+    sc->skip64 = 0;
+  }
+}
+
+// r = b - (t + merged partials); with Jacobi also partial r.(m r)
+__global__ __launch_bounds__(kThreads) void k_rel_resid(int n, const int32_t* __restrict__ inc_ptr,
+                                                         const int32_t* __restrict__ inc_idx,
+                                                         const double* __restrict__ epart,
+                                                         const double* __restrict__ t,
+                                                         const double* __restrict__ b,
+                                                         const double* __restrict__ m, double* r,
+                                                         int jacobi, double* part,
+                                                         const SolveScalars* sc) {
+  __shared__ double sh[8];
+  if (!block_flag(&sc->upd, sh) || block_flag(&sc->done, sh)) return;
+  double acc = 0.0;
+  GRID_LOOP(i, n) {
+    double s = t[i];
+    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
+    const double ri = b[i] - s;
+    r[i] = ri;
+    acc += ri * (ri * m[i]);
+  }
+  if (jacobi) {
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+  }
+}
+
+// true-residual test; otherwise the iteration resumes from the replaced
+// residual with its search direction kept
+__global__ void k_rel_finish(const double* __restrict__ part, double* dslot, SolveScalars* sc) {
+  __shared__ double sh[8];
+  if (!block_flag(&sc->upd, sh) || block_flag(&sc->done, sh)) return;
+  const double rz = grid_sum(part, sh);
+  if (threadIdx.x == 0) {
+    sc->true_rmr = rz;
+    sc->nupd += 1;
+    if (sqrt(fabs(rz)) < sc->conv) {
+      sc->done = 1;
+    } else {
+      *dslot = rz;
+      sc->rmax = sqrt(fabs(rz));
+      sc->upd = 0;
+      sc->skip64 = 1;
+    }
   }
 }
 
 // ------------------------------------------------------------- host side
-// Runs PCG on device-resident data.  Returns iterations in sc (device).
-template <typename T>
-int pcg_device(Context* c, const PcgBuffers<T>& B, double tol, int maxit, double* dslots) {
+int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int maxit,
+            bool reliable, double delta, double* dslots) {
   hipStream_t s = c->stream;
   const int n = c->neq;
   const dim3 G(kRedBlocks), TB(kThreads);
   double* part_a = c->red64;
   double* part_b = c->red64 + kRedBlocks;
-  hipLaunchKernelGGL(k_pcg_init<T>, G, TB, 0, s, n, B.b, B.m, B.bm, B.x, B.r, B.p, part_a);
-  hipLaunchKernelGGL(k_pcg_init2, dim3(1), TB, 0, s, part_a, tol, dslots, c->scal);
+  const int jac = V.bm == nullptr;
+  SolveScalars* sc = c->scal;
+  hipLaunchKernelGGL(k_pcg_init, G, TB, 0, s, n, V.b, V.m, V.bm, V.x, V.r, V.p, part_a);
+  hipLaunchKernelGGL(k_pcg_init2, dim3(1), TB, 0, s, part_a, tol, delta, reliable ? 1 : 0, dslots,
+                     sc);
+  auto replace = [&](int iters_so_far) {
+    double* dnext = dslots + (iters_so_far & 1);
+    launch_spmv(c, false, V.x, V.t, V.epart, &sc->skip64);
+    hipLaunchKernelGGL(k_rel_resid, G, TB, 0, s, n, c->inc_ptr, c->inc_idx, V.epart,
+                       V.t, V.b, V.m, V.r, jac, part_b, sc);
+    if (!jac) hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, &sc->upd, sc);
+    hipLaunchKernelGGL(k_rel_finish, dim3(1), TB, 0, s, part_b, dnext, sc);
+  };
   int it = 0;
+  const int every = std::max(2, c->check_every & ~1);
+  bool done = false;
   while (it < maxit) {
-    int chunk = std::min(c->check_every, maxit - it);
+    const int chunk = std::min(every, maxit - it);
     for (int j = 0; j < chunk; ++j, ++it) {
       double* dcur = dslots + (it & 1);
       double* dnxt = dslots + ((it + 1) & 1);
-      launch_spmv_tiles<T>(c->tiles, B.ad, B.al, B.p, B.q, B.epart, s, &c->scal->done);
-      hipLaunchKernelGGL(k_merge_dot<T>, G, TB, 0, s, n, c->tiles.inc_ptr, c->tiles.inc_idx,
-                         B.epart, B.q, B.p, part_a, c->scal);
-      if (B.bm == nullptr) {
-        hipLaunchKernelGGL(k_pcg_update_jacobi<T>, G, TB, 0, s, n, part_a, dcur, B.x, B.p, B.r,
-                           B.q, B.m, part_b, c->scal);
-      } else {
-        hipLaunchKernelGGL(k_pcg_update<T>, G, TB, 0, s, n, part_a, dcur, B.x, B.p, B.r, B.q,
-                           part_b, c->scal);
-        hipLaunchKernelGGL(k_rz<T>, G, TB, 0, s, n, B.r, B.m, B.bm, part_b, c->scal);
-      }
-      hipLaunchKernelGGL(k_pcg_pupdate<T>, G, TB, 0, s, n, part_b, dcur, dnxt, B.p, B.r, B.m,
-                         B.bm, c->scal);
+      launch_spmv(c, fp32_operator, V.p, V.q, V.epart, &sc->done);
+      hipLaunchKernelGGL(k_merge_dot, G, TB, 0, s, n, c->inc_ptr, c->inc_idx, V.epart,
+                         V.q, V.p, part_a, sc);
+      hipLaunchKernelGGL(k_pcg_update, G, TB, 0, s, n, part_a, dcur, V.x, V.p, V.r, V.q, V.m, jac,
+                         part_b, sc);
+      if (!jac) hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, nullptr, sc);
+      hipLaunchKernelGGL(k_pcg_pupdate, G, TB, 0, s, n, part_b, dcur, dnxt, V.p, V.r, V.m, V.bm,
+                         sc);
+      // the replacement kernels gate themselves on the flag set by k_pcg_pupdate
+      if (reliable && ((it + 1) & 1) == 0) replace(it + 1);
     }
-    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->scal_host, c->scal, sizeof(SolveScalars),
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->scal_host, sc, sizeof(SolveScalars),
                                         hipMemcpyDeviceToHost, s));
     ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
-    if (c->scal_host->done) break;
+    if (c->scal_host->done) {
+      done = true;
+      break;
+    }
+  }
+  if (!done && reliable) {  // maxit: leave the true residual of the final x
+    hipLaunchKernelGGL(k_rel_force, dim3(1), dim3(64), 0, s, sc);
+    replace(it);
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->scal_host, sc, sizeof(SolveScalars),
+                                        hipMemcpyDeviceToHost, s));
+    ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
   }
   ELASTIC_HIP_CHECK(c, hipGetLastError());
   return ELASTIC_OK;
 }
 
-template int pcg_device<double>(Context*, const PcgBuffers<double>&, double, int, double*);
-template int pcg_device<float>(Context*, const PcgBuffers<float>&, double, int, double*);
 
 }  // namespace elastic

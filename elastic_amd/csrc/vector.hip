@@ -74,34 +74,6 @@ __global__ __launch_bounds__(kT) void k_sum_parts(const double* __restrict__ par
   }
 }
 
-// Iterative refinement: x64 += e32 (widened)
-__global__ __launch_bounds__(kT) void k_ir_correct(int n, const float* __restrict__ e,
-                                                    double* __restrict__ x) {
-  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) x[i] += (double)e[i];
-}
-
-// r = b - (y + merged partials); r32 = r; partial r.m r (FP64)
-__global__ __launch_bounds__(kT) void k_ir_residual(int n, const int32_t* __restrict__ inc_ptr,
-                                                     const int32_t* __restrict__ inc_idx,
-                                                     const double* __restrict__ epart,
-                                                     const double* __restrict__ y,
-                                                     const double* __restrict__ b,
-                                                     const double* __restrict__ m,
-                                                     double* __restrict__ r,
-                                                     float* __restrict__ r32, double* part) {
-  __shared__ double sh[8];
-  double acc = 0.0;
-  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
-    double s = y[i];
-    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
-    const double ri = b[i] - s;
-    r[i] = ri;
-    r32[i] = (float)ri;
-    acc += ri * (ri * m[i]);
-  }
-  acc = blk_sum(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
-}
 
 // ------------------------------------------------------------- launchers
 static int grid_for(int64_t n) {
@@ -124,10 +96,11 @@ void launch_recip(int n, const T* ad, T* m, hipStream_t s) {
 template void launch_recip<double>(int, const double*, double*, hipStream_t);
 template void launch_recip<float>(int, const float*, float*, hipStream_t);
 
-void launch_post(const DevTiles& t, int n, const double* epart, const double* x, double* y,
-                 const double* b, const double* m, double* part, double* out4, hipStream_t s) {
-  hipLaunchKernelGGL(k_post, dim3(kRedBlocks), dim3(kT), 0, s, n, t.inc_ptr, t.inc_idx, epart, x, y,
-                     b, m, part);
+void launch_post(const int32_t* inc_ptr, const int32_t* inc_idx, int n, const double* epart,
+                 const double* x, double* y, const double* b, const double* m, double* part,
+                 double* out4, hipStream_t s) {
+  hipLaunchKernelGGL(k_post, dim3(kRedBlocks), dim3(kT), 0, s, n, inc_ptr, inc_idx, epart, x, y, b,
+                     m, part);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, s, part, 4, out4);
 }
 
@@ -135,15 +108,5 @@ void launch_sum_parts(const double* part, int nsum, double* out, hipStream_t s) 
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, s, part, nsum, out);
 }
 
-void launch_ir_correct(int n, const float* e, double* x, hipStream_t s) {
-  hipLaunchKernelGGL(k_ir_correct, dim3(grid_for(n)), dim3(kT), 0, s, n, e, x);
-}
-
-void launch_ir_residual(const DevTiles& t, int n, const double* epart, const double* y,
-                        const double* b, const double* m, double* r, float* r32, double* part,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(k_ir_residual, dim3(kRedBlocks), dim3(kT), 0, s, n, t.inc_ptr, t.inc_idx,
-                     epart, y, b, m, r, r32, part);
-}
 
 }  // namespace elastic

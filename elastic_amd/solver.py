@@ -71,8 +71,9 @@ class Context:
     def layout_info(self):
         info = np.zeros(4, np.int64)
         self._chk(self._L.elastic_layout_info(self._h, info))
+        # ncols_max holds the node block size of the active layout (1 = scalar tiles)
         return dict(ntiles=int(info[0]), nnz_max=int(info[1]), ext_partials=int(info[2]),
-                    ncols_max=int(info[3]))
+                    ncols_max=int(info[3]), nb=int(info[3]))
 
     # ---------------------------------------------------------- products
     def spmv(self, x: np.ndarray, precision: int = FP64) -> np.ndarray:

@@ -139,3 +139,75 @@ def test_maxit_is_reported(ctx, golden):
     assert e.value.code == 2
     x, st = ctx.solve(g["b"], maxit=3, allow_maxit=True)
     assert st["iters"] == 3
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_block_jacobi_pcg(ctx, golden, precision):
+    """3x3 block-Jacobi (Precond.f:163-540) as the PCG preconditioner: the
+    solution agrees with the reference's Jacobi-PCG solution."""
+    from elastic_amd.solver import BDIAG3
+
+    g = golden("hex8_5j")
+    load_system(ctx, g)
+    x, st = ctx.solve(g["b"], precond=BDIAG3, precision=precision, tol=1e-12)
+    assert rel(x, g["x"]) <= 1e-8
+    xj, sj = ctx.solve(g["b"], precision=precision, tol=1e-12)
+    assert st["iters"] < sj["iters"]
+
+
+def test_mixed_reliable_updates_keep_iterations(ctx, golden):
+    """FP32 iteration with FP64 residual replacement: iteration count close to
+    the FP64 PCG's (no restarts) and several replacements were made."""
+    from elastic_amd.solver import MIXED
+    from oracle import binding as O
+    from tests.cases import CASES
+
+    s = O.System(CASES["hex8_20"][0]())
+    ctx.set_structure(s.ia, s.ja)
+    ctx.set_values(s.ad, s.al[: s.nad])
+    x64, s64 = ctx.solve(s.b, tol=1e-10)
+    xm, sm = ctx.solve(s.b, tol=1e-10, precision=MIXED)
+    assert sm["iters"] <= 1.3 * s64["iters"] + 10
+    assert sm["outer"] >= 3
+    assert sm["norm_m_r"] < sm["conv"]
+
+
+@pytest.mark.parametrize("layout", ["scalar", "blocked"])
+@pytest.mark.parametrize("name", ["hex8_5j", "tet4_3j", "quad4_8", "tria3_8"])
+def test_layouts_agree(golden, name, layout, monkeypatch):
+    """Scalar CSRC tiles and node-blocked tiles give the same product and PCG."""
+    from elastic_amd.solver import MIXED, Context
+
+    if layout == "scalar":
+        monkeypatch.setenv("ELASTIC_LAYOUT", "scalar")
+    g = golden(name)
+    with Context(0) as c:
+        load_system(c, g)
+        nb = c.layout_info()["ncols_max"]
+        assert nb == (1 if layout == "scalar" else (3 if name[:3] in ("hex", "tet") else 2))
+        assert rel(c.spmv(g["spmv_v"]), g["spmv_y"]) <= 1e-13
+        assert rel(c.spmv(g["spmv_v"], precision=MIXED), g["spmv_y"]) <= 2e-6
+        x, st = c.solve(g["b"], tol=float(g["tol"]))
+        assert abs(st["iters"] - int(g["pcg_iters"])) <= 2
+        assert rel(x, g["x"]) <= 1e-10
+
+
+def test_partially_restrained_nodes_use_scalar_tiles():
+    """Rollers (only uz fixed on z=0) leave nodes with 2 free dofs: the node
+    blocks do not apply and the scalar tiles are used; results match the oracle."""
+    from elastic_amd import mesh as M
+    from elastic_amd.solver import Context
+    from oracle import binding as O
+
+    m = M.hex8_cube(4, jitter=0.1, seed=3)
+    z0 = m.x[:, 2] == 0.0
+    m.id[z0] = [0, 0, 1]
+    m.id[(m.x[:, 0] == 0.0) & z0] = [1, 1, 1]
+    s = O.System(m)
+    with Context(0) as c:
+        c.set_structure(s.ia, s.ja)
+        c.set_values(s.ad, s.al[: s.nad])
+        assert c.layout_info()["ncols_max"] == 1
+        x, st = c.solve(s.b, tol=1e-10)
+    xo, so = O.pcg(s.ia, s.ja, s.ad, s.al[: s.nad], s.b, tol=1e-10)
+    assert abs(st["iters"] - so["iters"]) <= 2 and rel(x, xo) <= 1e-10
