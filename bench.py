@@ -47,8 +47,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: multi-GPU sharding of this path is not built "
-              "(see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the solve path is not sharded across GPUs; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -108,6 +108,8 @@ def main():
     gbs32 = b32 / (ker32 * 1e-3) / 1e9
     gbs64 = b64 / (ker64 * 1e-3) / 1e9
     t_asm = sysm.time_assembly(reps=3)
+    nb = ctx.layout_info()["nb"]
+    copy = copy_probe(torch)
 
     cpu = None
     if not args.no_cpu:
@@ -144,36 +146,82 @@ def main():
         "assembly_ms": t_asm,
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv_tiles<float,double>"},
+                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": measured_traffic(neq, nad),
+                     "kernel": "k_spmv_blocks<float,double,%d>" % nb if nb > 1
+                               else "k_spmv_tiles<float,double>"},
+        "copy_probe": copy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
+def measured_traffic(neq: int, nad: int):
+    """HBM bytes per launch of the FP32 operator (FETCH_SIZE and WRITE_SIZE
+    counter runs scaled by the copy-probe calibration), recorded for this
+    workload in profiles/spmv_traffic_hex8_100.json; None for other sizes."""
+    path = os.path.join(ROOT, "profiles", "spmv_traffic_hex8_100.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    if ("neq %d" % neq) not in d["workload"] or ("nad %d" % nad) not in d["workload"]:
+        return None
+    return d["fp32"]["read_bytes"] + d["fp32"]["write_bytes"]
+
+
+def copy_probe(torch):
+    """A plain device copy of known size (y = -x over 2^27 doubles: 1 GiB read,
+    1 GiB written) timed with events on torch's stream; it is also the
+    calibration point of the FETCH_SIZE / WRITE_SIZE counter runs."""
+    n = 1 << 27
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.neg(x, out=y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 16 * n
+    del x, y
+    return {"bytes": nbytes, "ms": ms, "GBs": nbytes / (ms * 1e-3) / 1e9}
+
+
 def cpu_baseline(sysm, iters_fp64: int, seconds: float):
     """The oracle's OpenMP PCG (Iters_omp.f restated) on the same system, on the
-    host cores, for a bounded number of iterations; solved DOF/s extrapolated
-    with the FP64 iteration count."""
+    host cores: the whole solve when it takes at most about 2 x `seconds`,
+    else a bounded number of iterations extrapolated with the FP64 GPU
+    iteration count."""
     from oracle import binding as O
 
     nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
     ad, al = sysm.values()
     m = 1.0 / ad
     b = sysm.b
-    # calibrate with a few iterations, then run a sample of about `seconds`
+    # calibrate on 20 iterations; run the whole solve when it fits in about
+    # twice `seconds`, else a bounded sample extrapolated to the full count
     a = time.perf_counter()
-    O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=1e-30, maxit=3, m=m, nth=nth)
-    t3 = (time.perf_counter() - a) / 3.0
-    k = max(5, min(iters_fp64, int(seconds / max(t3, 1e-6))))
+    O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=1e-30, maxit=20, m=m, nth=nth)
+    t20 = (time.perf_counter() - a) / 20.0
+    full = t20 * iters_fp64 <= 2.0 * seconds
+    k = iters_fp64 + 100 if full else max(5, int(seconds / max(t20, 1e-6)))
     a = time.perf_counter()
-    _, st = O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=1e-30, maxit=k, m=m, nth=nth)
+    _, st = O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=TOL if full else 1e-30, maxit=k, m=m, nth=nth)
     t = time.perf_counter() - a
+    k = st["iters"] if full else k
     t_it = t / k
-    return {"value": sysm.neq / (t_it * iters_fp64), "unit": "DOF/s", "cores": nth,
-            "kind": "port",
-            "sample": "%d of the %d FP64 PCG iterations (OpenMP pcg_omp restatement, %d threads, "
-                      "%.1f s), extrapolated to a full solve" % (k, iters_fp64, nth, t),
+    if full:
+        sample = ("the whole FP64 solve to %g (%d iterations; OpenMP pcg_omp restatement, "
+                  "%d threads, %.1f s)" % (TOL, k, nth, t))
+        value = sysm.neq / t
+    else:
+        sample = ("%d of the %d FP64 PCG iterations (OpenMP pcg_omp restatement, %d threads, "
+                  "%.1f s), extrapolated to a full solve" % (k, iters_fp64, nth, t))
+        value = sysm.neq / (t_it * iters_fp64)
+    return {"value": value, "unit": "DOF/s", "cores": nth, "kind": "port", "sample": sample,
             "s_per_iter": t_it}
 
 
