@@ -43,14 +43,14 @@ extern "C" {
 #define ELASTIC_PRECOND_BDIAG3 6
 /* arithmetic */
 #define ELASTIC_FP64 64          /* FP64 values, vectors and sums */
-#define ELASTIC_MIXED 32         /* FP32 inner solves inside FP64 iterative refinement */
+#define ELASTIC_MIXED 32         /* FP32 matrix values, FP64 vectors, sums and true residual */
 
 typedef struct elastic_ctx elastic_ctx;
 
 /* Results of one solve, the quantities pcg prints (Iters.f:435-507). */
 typedef struct {
-  int32_t iters;        /* Krylov iterations (all inner iterations for MIXED) */
-  int32_t outer;        /* refinement steps (MIXED) or restart cycles (GMRES) */
+  int32_t iters;        /* Krylov iterations */
+  int32_t outer;        /* FP64 residual replacements (MIXED) or restart cycles (GMRES) */
   int32_t status;
   int32_t spmv_count;   /* products with A (any precision) */
   double xkx;           /* x . K x */
@@ -96,7 +96,8 @@ int elastic_set_values(elastic_ctx* ctx, const double* ad, const double* al);
  * caller's CSRC stays authoritative). */
 int elastic_get_values(elastic_ctx* ctx, double* ad, double* al);
 /* y = (D + L + L^T) x, Matvec.f:528-621.  precision ELASTIC_FP64 or
- * ELASTIC_MIXED (FP32 values and sums, result widened). */
+ * ELASTIC_MIXED (the FP32-value operator of the mixed method: values rounded
+ * to FP32, products and sums in FP64). */
 int elastic_spmv(elastic_ctx* ctx, int32_t precision, const double* x, double* y);
 
 /* ------------------------------------------------------- GPU assembly */
@@ -127,8 +128,10 @@ int elastic_time_assembly(elastic_ctx* ctx, int32_t reps, double* ms);
  * solver 1, gmres (Iters.f:528-808) for 2 and pbicgstab (Iters.f:1079-1349)
  * for 4.  x0 = 0 (fnew = .true.); the stopping rule is the reference's:
  * PCG ||r||_M < tol ||b||_M, GMRES ||M r|| < tol ||M b||, BiCGSTAB
- * ||r|| < tol ||b||.  MIXED runs FP32 inner solves and refines x in FP64
- * until the FP64 residual meets the same rule.  st may be NULL. */
+ * ||r|| < tol ||b||.  MIXED applies the FP32-value operator inside the
+ * iteration and replaces the recursive residual by the FP64 true residual
+ * b - A x whenever it has dropped by a factor 10 since the last replacement;
+ * the stopping rule is tested on that FP64 residual.  st may be NULL. */
 int elastic_solve(elastic_ctx* ctx, int32_t solver, int32_t precond, int32_t precision,
                   const double* b, double* x, double tol, int32_t maxit, int32_t ngram,
                   elastic_stats* st);
