@@ -70,6 +70,10 @@ static void free_system(Context* c) {
   c->epart64 = nullptr;
   for (auto& v : c->v64)
     if (v) hipFree(v), v = nullptr;
+  if (c->gm_basis) hipFree(c->gm_basis);
+  if (c->gm_small) hipFree(c->gm_small);
+  c->gm_basis = c->gm_small = nullptr;
+  c->gm_k = 0;
   c->values_set = c->values32 = c->bdiag_set = false;
 }
 
@@ -77,8 +81,6 @@ static void free_system(Context* c) {
 
 using namespace elastic;
 
-// vector slots
-enum { VX = 0, VR, VP, VQ, VB, VT, VU, VW };
 
 extern "C" {
 
@@ -395,23 +397,26 @@ int elastic_download_x(elastic_ctx* x, double* xv) {
 
 // ------------------------------------------------------------------ solves
 namespace elastic {
-int solve_krylov_other(Context* c, int solver, int precond, int precision, double tol, int maxit,
-                       int ngram, elastic_stats* st);
+
 int ensure_bdiag(elastic_ctx* x);
 
 // statistics of Iters.f:435-457 on the resident x (FP64 operator)
-static int post_stats(Context* c, elastic_stats* st) {
+// Post-solve figures of Iters.f:435-457 on the FP64 operator: x.Kx, ||x||,
+// ||b - Ax|| and ||b - Ax||_M; *bnorm2 / *bnormm receive ||b|| and ||b||_M.
+static int post_stats(Context* c, elastic_stats* st, double* bnorm2, double* bnormm) {
   double* out = c->red64 + 8 * kRedBlocks;
   launch_spmv(c, false, c->v64[VX], c->v64[VQ], c->epart64);
   launch_post(c->inc_ptr, c->inc_idx, c->neq, c->epart64, c->v64[VX], c->v64[VQ], c->v64[VB], c->A64.m, c->red64,
               out, c->stream);
-  double h[4];
+  double h[6];
   ELASTIC_HIP_CHECK(c, hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   st->xkx = h[0];
   st->norm_x = std::sqrt(h[1]);
   st->norm_m_r = std::sqrt(std::fabs(h[2]));
   st->norm_r = std::sqrt(h[3]);
+  *bnorm2 = std::sqrt(h[4]);
+  *bnormm = std::sqrt(std::fabs(h[5]));
   return ELASTIC_OK;
 }
 
@@ -443,10 +448,55 @@ static int solve_pcg(Context* c, int precond, int precision, double tol, int max
   st->spmv_count = h.iter + (precision == ELASTIC_MIXED ? h.nupd : 0);
   st->conv = h.conv;
   if (!h.done) st->status = ELASTIC_MAXIT;
-  rc = post_stats(c, st);
+  double bn2, bnm;
+  rc = post_stats(c, st, &bn2, &bnm);
   if (rc) return rc;
-  st->rel_res = st->conv > 0 ? st->norm_m_r / (st->conv / tol) : 0.0;
+  st->rel_res = bnm > 0 ? st->norm_m_r / bnm : 0.0;
   ELASTIC_HIP_CHECK(c, hipGetLastError());
+  return ELASTIC_OK;
+}
+
+// BiCGSTAB / GMRES(k): the iterations, then the same post-solve figures.
+// rel_res is ||b - Ax|| / ||b|| for BiCGSTAB (its stopping norm) and the
+// M-norm ratio for GMRES.
+static int solve_other(Context* c, int solver, int precond, int precision, double tol, int maxit,
+                       int ngram, elastic_stats* st) {
+  hipStream_t s = c->stream;
+  const bool bd = precond == ELASTIC_PRECOND_BDIAG3;
+  const bool mixed = precision == ELASTIC_MIXED;
+  ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev0, s));
+  KrylovResult kr;
+  int rc;
+  if (solver == ELASTIC_SOLVER_BICGSTAB) {
+    double delta = 0.1;
+    if (const char* e = getenv("ELASTIC_RELIABLE_DELTA")) delta = atof(e);
+    rc = bicgstab_run(c, bd, mixed, tol, maxit, delta, &kr);
+  } else if (solver == ELASTIC_SOLVER_GMRES) {
+    rc = gmres_run(c, bd, mixed, tol, maxit, ngram, &kr);
+  } else {
+    return set_error(c, ELASTIC_BAD_ARG, "solver " + std::to_string(solver) +
+                                             " not available (1 pcg, 2 gmres, 4 bicgstab)");
+  }
+  if (rc) return rc;
+  ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev1, s));
+  ELASTIC_HIP_CHECK(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  st->t_device_ms = ms;
+  st->iters = kr.iters;
+  st->outer = kr.outer;
+  st->status = kr.status;
+  st->conv = kr.conv;
+  st->spmv_count = solver == ELASTIC_SOLVER_BICGSTAB ? 2 * kr.iters + kr.nupd : kr.iters + kr.outer;
+  double bn2, bnm;
+  rc = post_stats(c, st, &bn2, &bnm);
+  if (rc) return rc;
+  if (solver == ELASTIC_SOLVER_BICGSTAB)
+    st->rel_res = bn2 > 0 ? st->norm_r / bn2 : 0.0;
+  else
+    st->rel_res = bnm > 0 ? st->norm_m_r / bnm : 0.0;
+  if (st->status == ELASTIC_BREAKDOWN)
+    return set_error(c, ELASTIC_BREAKDOWN, "zero denominator in the Krylov recurrence");
   return ELASTIC_OK;
 }
 
@@ -467,7 +517,7 @@ int solve_resident(elastic_ctx* x, int solver, int precond, int precision, doubl
   }
   memset(st, 0, sizeof(*st));
   if (solver == ELASTIC_SOLVER_PCG) return solve_pcg(c, precond, precision, tol, maxit, st);
-  return solve_krylov_other(c, solver, precond, precision, tol, maxit, ngram, st);
+  return solve_other(c, solver, precond, precision, tol, maxit, ngram, st);
 }
 
 }  // namespace elastic

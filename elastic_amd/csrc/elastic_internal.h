@@ -125,6 +125,8 @@ struct SolveScalars {
   // reliable-update mixed precision (FP32 iteration, FP64 residual replacement)
   double rmax, delta, true_rmr;
   int32_t reliable, upd, skip64, nupd;
+  // BiCGSTAB omega; GMRES: |e(i)| of the last inner step and its norm h(i+1,i)
+  double omega, gm_e, gm_h;
 };
 
 constexpr int kRedBlocks = 1024;  // fixed reduction grid (deterministic sums)
@@ -156,13 +158,35 @@ struct Context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   // assembly state (optional)
   void* mesh = nullptr;  // MeshDev (assembly.hip)
+  // GMRES(k) Krylov basis ((k+1) vectors of neq) and its small arrays
+  double* gm_basis = nullptr;
+  double* gm_small = nullptr;
+  int gm_k = 0;
   // autotuning/measurement
   double last_spmv_ms = 0.0;
   int check_every = 16;
   int zero_diag_row = -1;  // first row with ad == 0 (1-based), -1 if none
 };
 
+// device vector slots of a context (c->v64[...])
+enum VecSlot { VX = 0, VR, VP, VQ, VB, VT, VU, VW };
+
 int set_error(Context* c, int code, const std::string& msg);
+
+// krylov.hip: BiCGSTAB (Iters.f:1079-1349) and GMRES(k) (Iters.f:528-808) on
+// device-resident vectors (b in v64[VB], x left in v64[VX]).  fp32_operator
+// selects the FP32-value operator inside the iteration; the FP64 operator
+// then recomputes the true residual (BiCGSTAB: reliable replacement as in
+// PCG; GMRES: every restart) and convergence is decided on it.
+struct KrylovResult {
+  int iters = 0, outer = 0, nupd = 0, status = 0;
+  bool done = false;
+  double conv = 0.0;
+};
+int bicgstab_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit, double delta,
+                 KrylovResult* out);
+int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit, int k,
+              KrylovResult* out);
 struct MeshDev;
 void free_mesh(MeshDev* m);
 #define ELASTIC_HIP_CHECK(ctx, expr)                                                   \
@@ -190,7 +214,7 @@ template <typename T>
 void launch_recip(int n, const T* ad, T* m, hipStream_t s);
 void launch_post(const int32_t* inc_ptr, const int32_t* inc_idx, int n, const double* epart,
                  const double* x, double* y, const double* b, const double* m, double* part,
-                 double* out4, hipStream_t s);
+                 double* out6, hipStream_t s);
 void launch_sum_parts(const double* part, int nsum, double* out, hipStream_t s);
 
 // pcg.hip: one PCG solve on device-resident FP64 vectors.  The operator's

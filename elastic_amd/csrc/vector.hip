@@ -29,8 +29,8 @@ __global__ __launch_bounds__(kT) void k_recip(int n, const T* __restrict__ ad, T
   for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) m[i] = T(1) / ad[i];
 }
 
-// y += merged partials (when inc_ptr != null); then 4 partial sums:
-// x.y, x.x, (b-y).m(b-y), (b-y).(b-y)
+// y += merged partials (when inc_ptr != null); then 6 partial sums:
+// x.y, x.x, (b-y).m(b-y), (b-y).(b-y), b.b, b.(m b)
 __global__ __launch_bounds__(kT) void k_post(int n, const int32_t* __restrict__ inc_ptr,
                                               const int32_t* __restrict__ inc_idx,
                                               const double* __restrict__ epart,
@@ -38,27 +38,23 @@ __global__ __launch_bounds__(kT) void k_post(int n, const int32_t* __restrict__ 
                                               const double* __restrict__ b,
                                               const double* __restrict__ m, double* part) {
   __shared__ double sh[8];
-  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  double a[6] = {0, 0, 0, 0, 0, 0};
   for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
     double s = y[i];
     if (inc_ptr)
       for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
     y[i] = s;
-    const double xi = x[i], ri = b[i] - s;
-    a0 += xi * s;
-    a1 += xi * xi;
-    a2 += ri * (ri * m[i]);
-    a3 += ri * ri;
+    const double xi = x[i], bi = b[i], ri = bi - s;
+    a[0] += xi * s;
+    a[1] += xi * xi;
+    a[2] += ri * (ri * m[i]);
+    a[3] += ri * ri;
+    a[4] += bi * bi;
+    a[5] += bi * (bi * m[i]);
   }
-  a0 = blk_sum(a0, sh);
-  a1 = blk_sum(a1, sh);
-  a2 = blk_sum(a2, sh);
-  a3 = blk_sum(a3, sh);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = a0;
-    part[kRedBlocks + blockIdx.x] = a1;
-    part[2 * kRedBlocks + blockIdx.x] = a2;
-    part[3 * kRedBlocks + blockIdx.x] = a3;
+  for (int j = 0; j < 6; ++j) {
+    const double v = blk_sum(a[j], sh);
+    if (threadIdx.x == 0) part[j * kRedBlocks + blockIdx.x] = v;
   }
 }
 
@@ -98,10 +94,10 @@ template void launch_recip<float>(int, const float*, float*, hipStream_t);
 
 void launch_post(const int32_t* inc_ptr, const int32_t* inc_idx, int n, const double* epart,
                  const double* x, double* y, const double* b, const double* m, double* part,
-                 double* out4, hipStream_t s) {
+                 double* out6, hipStream_t s) {
   hipLaunchKernelGGL(k_post, dim3(kRedBlocks), dim3(kT), 0, s, n, inc_ptr, inc_idx, epart, x, y, b,
                      m, part);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, s, part, 4, out4);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, s, part, 6, out6);
 }
 
 void launch_sum_parts(const double* part, int nsum, double* out, hipStream_t s) {
