@@ -83,6 +83,14 @@ def lib():
                                    I32P, F64P]
     L.elastic_assemble.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, vp]
     L.elastic_time_assembly.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.elastic_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    L.elastic_batch_destroy.argtypes = [vp]
+    L.elastic_batch_last_error.argtypes = [vp]
+    L.elastic_batch_last_error.restype = C.c_char_p
+    L.elastic_batch_submit.argtypes = [vp, vp, C.c_int32, C.c_double, C.c_int32, C.c_int64]
+    L.elastic_batch_step.argtypes = [vp]
+    L.elastic_batch_poll.argtypes = [vp, I64P, vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.elastic_batch_active.argtypes = [vp, C.POINTER(C.c_int32)]
     _lib = L
     return L
 

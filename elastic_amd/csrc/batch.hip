@@ -1,0 +1,651 @@
+// Multi-instance PCG for serving: many independent systems (each held by its
+// own context: structure, values, b and vectors resident in HBM) iterate
+// together.  Every phase of a PCG iteration is ONE launch over all active
+// instances with a fixed grid that walks a device work list:
+//   SpMV          work item = (instance, tile)          block_tile of spmv_block.h
+//   vector phases work item = (instance, chunk of 2048 rows)
+// so a chunk of iterations is captured once as a HIP graph and replayed while
+// instances are admitted and retired: only the work lists and the instance
+// table (device memory) change between replays.
+//
+// The arithmetic of an instance never depends on the other instances: its
+// tiles, its chunks and the order of its reductions (per-chunk partials summed
+// in chunk order) are functions of its own size, so a result is bit for bit
+// the same alone and in any mix.  The method per instance is the PCG of
+// pcg.hip (Iters.f:252-524, Jacobi, x0 = 0, M-norm stopping rule), in FP64
+// or with the FP32-value operator and FP64 true-residual replacement.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_util.h"
+#include "spmv_block.h"
+
+namespace elastic {
+
+constexpr int kChunk = 2048;         // rows per vector work item (256 lanes x 8)
+constexpr int kBatchGrid = 2048;     // fixed grid of every batched phase
+constexpr int kMaxSlots = 255;       // slot index lives in the top 8 bits of a work item
+constexpr int kBatchPer = 2;         // node blocks per lane (tiles of <= 512 blocks)
+
+struct BInst {
+  BlkArgs a;
+  const double *v64, *d64;
+  const float *v32, *d32;
+  const int32_t *inc_ptr, *inc_idx;
+  double* epart;
+  double *x, *r, *p, *q, *b, *t;
+  const double* m;
+  double* part;   // 3 * nch partials: p.q, r.Mr, true-residual r.Mr
+  double* dslot;  // 2 ping-pong slots of r.Mr
+  SolveScalars* sc;
+  int n, nb, nch, fp32;
+};
+
+// ---------------------------------------------------------------- kernels
+__device__ inline int wl_slot(uint32_t e) { return (int)(e >> 24); }
+__device__ inline int wl_index(uint32_t e) { return (int)(e & 0xFFFFFFu); }
+
+// Sum of an instance's nch partials in chunk order (same value in every block).
+__device__ inline double inst_sum(const double* __restrict__ part, int nch, double* sh) {
+  double v = 0.0;
+  for (int q = threadIdx.x; q < nch; q += kThreads) v += part[q];
+  double r = block_sum(v, sh);
+  if (threadIdx.x == 0) sh[4] = r;
+  __syncthreads();
+  r = sh[4];
+  __syncthreads();
+  return r;
+}
+
+// mode 0: q = A p with the instance's operator (FP32 values when fp32), skipped
+// once done; mode 1: t = A64 x, only for instances that asked for a true
+// residual (reliable replacement)
+__global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts,
+                                                 const uint32_t* __restrict__ work,
+                                                 const int32_t* __restrict__ count, int mode) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int flag;
+  const int nw = *count;
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const uint32_t e = work[w];
+    const BInst& I = insts[wl_slot(e)];
+    const int t = wl_index(e);
+    __syncthreads();  // LDS of the previous tile is free, flag is free
+    if (threadIdx.x == 0) {
+      const SolveScalars* sc = I.sc;
+      flag = mode == 0 ? !sc->done : (sc->upd && !sc->done);
+    }
+    __syncthreads();
+    if (!flag) continue;
+    const bool f32 = mode == 0 && I.fp32;
+    if (mode == 0) {
+      if (I.nb == 3) {
+        if (f32) block_tile<float, double, 3, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
+        else block_tile<double, double, 3, kBatchPer>(I.a, t, I.v64, I.d64, I.p, I.q, I.epart, smem);
+      } else {
+        if (f32) block_tile<float, double, 2, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
+        else block_tile<double, double, 2, kBatchPer>(I.a, t, I.v64, I.d64, I.p, I.q, I.epart, smem);
+      }
+    } else {
+      if (I.nb == 3) block_tile<double, double, 3, kBatchPer>(I.a, t, I.v64, I.d64, I.x, I.t, I.epart, smem);
+      else block_tile<double, double, 2, kBatchPer>(I.a, t, I.v64, I.d64, I.x, I.t, I.epart, smem);
+    }
+  }
+}
+
+// reads an instance flag into a block-uniform value
+__device__ inline bool uniform_done(const BInst& I, double* sh, bool want_upd = false) {
+  if (threadIdx.x == 0) {
+    const SolveScalars* sc = I.sc;
+    sh[5] = want_upd ? (double)(!sc->upd || sc->done) : (double)sc->done;
+  }
+  __syncthreads();
+  const bool d = sh[5] != 0.0;
+  __syncthreads();
+  return d;
+}
+
+// q += merged partials; partial p.q
+__global__ __launch_bounds__(kThreads) void k_bt_merge_dot(const BInst* __restrict__ insts,
+                                                           const uint32_t* __restrict__ work,
+                                                           const int32_t* __restrict__ count) {
+  __shared__ double sh[8];
+  const int nw = *count;
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const uint32_t e = work[w];
+    const BInst& I = insts[wl_slot(e)];
+    const int c = wl_index(e);
+    if (uniform_done(I, sh)) continue;
+    const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
+    double acc = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
+      double s = I.q[i];
+      for (int k = I.inc_ptr[i]; k < I.inc_ptr[i + 1]; ++k) s += I.epart[I.inc_idx[k]];
+      I.q[i] = s;
+      acc += I.p[i] * s;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) I.part[c] = s;
+  }
+}
+
+// alpha = d / p.q ; x += alpha p ; r -= alpha q ; partial r.(m r)
+__global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict__ insts,
+                                                        const uint32_t* __restrict__ work,
+                                                        const int32_t* __restrict__ count,
+                                                        int cur) {
+  __shared__ double sh[8];
+  const int nw = *count;
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const uint32_t e = work[w];
+    const BInst& I = insts[wl_slot(e)];
+    const int c = wl_index(e);
+    if (uniform_done(I, sh)) continue;
+    const double pq = inst_sum(I.part, I.nch, sh);
+    const double alpha = I.dslot[cur] / pq;
+    const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
+    double acc = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
+      I.x[i] = I.x[i] + alpha * I.p[i];
+      const double ri = I.r[i] - alpha * I.q[i];
+      I.r[i] = ri;
+      acc += ri * (ri * I.m[i]);
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) I.part[I.nch + c] = s;
+    if (c == 0 && threadIdx.x == 0) {
+      I.sc->alpha = alpha;
+      I.sc->pq = pq;
+    }
+  }
+}
+
+// beta = r.Mr / d ; p = M r + beta p ; chunk 0 keeps the books
+__global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict__ insts,
+                                                         const uint32_t* __restrict__ work,
+                                                         const int32_t* __restrict__ count,
+                                                         int cur) {
+  __shared__ double sh[8];
+  const int nw = *count;
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const uint32_t e = work[w];
+    const BInst& I = insts[wl_slot(e)];
+    const int c = wl_index(e);
+    if (uniform_done(I, sh)) continue;
+    const double di = inst_sum(I.part + I.nch, I.nch, sh);
+    const double beta = di / I.dslot[cur];
+    const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
+    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) I.p[i] = I.r[i] * I.m[i] + beta * I.p[i];
+    if (c == 0 && threadIdx.x == 0) {
+      SolveScalars* sc = I.sc;
+      I.dslot[cur ^ 1] = di;
+      sc->beta = beta;
+      sc->di = di;
+      sc->iter += 1;
+      const double rn = sqrt(fabs(di));
+      if (!sc->reliable) {
+        if (rn < sc->conv) sc->done = 1;
+      } else if (rn < sc->delta * sc->rmax || rn < sc->conv) {
+        sc->upd = 1;
+        sc->skip64 = 0;
+      }
+      if (!sc->done && sc->iter >= sc->maxit) {
+        sc->done = 1;
+        sc->status = ELASTIC_MAXIT;
+      }
+    }
+  }
+}
+
+// true residual r = b - (t + merged partials) of instances that asked for it;
+// partial r.(m r)
+__global__ __launch_bounds__(kThreads) void k_bt_rel_resid(const BInst* __restrict__ insts,
+                                                           const uint32_t* __restrict__ work,
+                                                           const int32_t* __restrict__ count) {
+  __shared__ double sh[8];
+  const int nw = *count;
+  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+    const uint32_t e = work[w];
+    const BInst& I = insts[wl_slot(e)];
+    const int c = wl_index(e);
+    if (uniform_done(I, sh, true)) continue;
+    const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
+    double acc = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
+      double s = I.t[i];
+      for (int k = I.inc_ptr[i]; k < I.inc_ptr[i + 1]; ++k) s += I.epart[I.inc_idx[k]];
+      const double ri = I.b[i] - s;
+      I.r[i] = ri;
+      acc += ri * (ri * I.m[i]);
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) I.part[2 * I.nch + c] = s;
+  }
+}
+
+// per instance: accept the true residual (done) or resume from it
+__global__ __launch_bounds__(kThreads) void k_bt_rel_finish(const BInst* __restrict__ insts,
+                                                            const int32_t* __restrict__ slots,
+                                                            const int32_t* __restrict__ count,
+                                                            int nxt) {
+  __shared__ double sh[8];
+  const int ns = *count;
+  for (int w = blockIdx.x; w < ns; w += gridDim.x) {
+    const BInst& I = insts[slots[w]];
+    if (uniform_done(I, sh, true)) continue;
+    const double rz = inst_sum(I.part + 2 * I.nch, I.nch, sh);
+    if (threadIdx.x == 0) {
+      SolveScalars* sc = I.sc;
+      sc->true_rmr = rz;
+      sc->nupd += 1;
+      if (sqrt(fabs(rz)) < sc->conv) {
+        sc->done = 1;
+        sc->status = 0;
+      } else {
+        I.dslot[nxt] = rz;
+        sc->rmax = sqrt(fabs(rz));
+        sc->upd = 0;
+        sc->skip64 = 1;
+      }
+    }
+  }
+}
+
+// admission of one instance (eager, grid = its chunks): x = 0, r = b, p = M b,
+// partial b.Mb
+__global__ __launch_bounds__(kThreads) void k_bt_init(const BInst* __restrict__ insts, int slot) {
+  __shared__ double sh[8];
+  const BInst& I = insts[slot];
+  const int c = blockIdx.x;
+  const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
+  double acc = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const double bi = I.b[i], zi = bi * I.m[i];
+    I.x[i] = 0.0;
+    I.r[i] = bi;
+    I.p[i] = zi;
+    acc += bi * zi;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) I.part[I.nch + c] = s;
+}
+
+__global__ void k_bt_init2(const BInst* __restrict__ insts, int slot, double tol, int maxit,
+                           double delta) {
+  __shared__ double sh[8];
+  const BInst& I = insts[slot];
+  const double d = inst_sum(I.part + I.nch, I.nch, sh);
+  if (threadIdx.x == 0) {
+    SolveScalars* sc = I.sc;
+    I.dslot[0] = d;  // steps start at even iteration parity
+    I.dslot[1] = d;
+    sc->conv = tol * sqrt(fabs(d));
+    sc->norm_b = sqrt(fabs(d));
+    sc->rz0 = d;
+    sc->rmax = sqrt(fabs(d));
+    sc->delta = delta;
+    sc->true_rmr = d;
+    sc->iter = 0;
+    sc->maxit = maxit;
+    sc->status = 0;
+    sc->reliable = I.fp32;
+    sc->upd = 0;
+    sc->skip64 = 1;
+    sc->nupd = 0;
+    sc->done = (d == 0.0) ? 1 : 0;
+  }
+}
+
+// done flags / iterations of the active slots for the host
+struct BStatus {
+  int32_t done, status, iter, nupd;
+};
+__global__ void k_bt_status(const BInst* __restrict__ insts, const int32_t* __restrict__ slots,
+                            const int32_t* __restrict__ count, BStatus* out) {
+  const int ns = *count;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < ns; w += gridDim.x * blockDim.x) {
+    const SolveScalars* sc = insts[slots[w]].sc;
+    out[w] = BStatus{sc->done, sc->status, sc->iter, sc->nupd};
+  }
+}
+
+}  // namespace elastic
+
+using namespace elastic;
+
+// ------------------------------------------------------------------ host
+struct elastic_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int max_slots = 0, iters = 16;
+  bool use_graph = true;
+  double delta = 0.1;
+  struct Slot {
+    bool used = false;
+    elastic_ctx* ctx = nullptr;
+    int64_t tag = 0;
+    int ntiles = 0, nch = 0;
+    size_t lds = 0;
+    std::chrono::steady_clock::time_point t0;
+  };
+  std::vector<Slot> slots;
+  std::vector<BInst> h_insts;
+  BInst* d_insts = nullptr;
+  uint32_t *d_swork = nullptr, *d_vwork = nullptr;
+  int32_t *d_slist = nullptr, *d_counts = nullptr;  // counts: spmv, vec, slots
+  size_t swork_cap = 0, vwork_cap = 0;
+  std::vector<uint32_t> h_swork, h_vwork;
+  std::vector<int32_t> h_slist, h_counts;
+  BStatus *d_status = nullptr, *h_status = nullptr;  // h_status pinned
+  bool dirty = true, stepped = false;
+  size_t lds = 0, graph_lds = 0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+
+static int berr(elastic_batch* b, int code, const std::string& m) {
+  b->err = m;
+  return code;
+}
+#define BATCH_HIP(b, expr)                                                                   \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return berr((b), ELASTIC_HIP_ERROR, std::string(#expr) + ": " +   \
+                                                                   hipGetErrorString(e_));   \
+  } while (0)
+
+static size_t tile_lds(int nb, int ncols_max, int nblk_max) {
+  return sizeof(double) * nb * ((size_t)ncols_max + 2 * kBlkRowsMax + (size_t)nblk_max);
+}
+
+// one PCG iteration of every active instance; `it` is the iteration index in
+// the step (its parity selects the r.Mr slot)
+static void enqueue_iteration(elastic_batch* b, int it) {
+  hipStream_t s = b->stream;
+  const dim3 G(kBatchGrid), TB(kThreads);
+  const int cur = it & 1;
+  hipLaunchKernelGGL(k_bt_spmv, G, TB, b->lds, s, b->d_insts, b->d_swork, b->d_counts, 0);
+  hipLaunchKernelGGL(k_bt_merge_dot, G, TB, 0, s, b->d_insts, b->d_vwork, b->d_counts + 1);
+  hipLaunchKernelGGL(k_bt_update, G, TB, 0, s, b->d_insts, b->d_vwork, b->d_counts + 1, cur);
+  hipLaunchKernelGGL(k_bt_pupdate, G, TB, 0, s, b->d_insts, b->d_vwork, b->d_counts + 1, cur);
+  if (cur == 1) {  // every second iteration: true-residual replacement where asked
+    hipLaunchKernelGGL(k_bt_spmv, G, TB, b->lds, s, b->d_insts, b->d_swork, b->d_counts, 1);
+    hipLaunchKernelGGL(k_bt_rel_resid, G, TB, 0, s, b->d_insts, b->d_vwork, b->d_counts + 1);
+    hipLaunchKernelGGL(k_bt_rel_finish, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist,
+                       b->d_counts + 2, 0);
+  }
+}
+
+static int set_lds_limit(elastic_batch* b) {
+  if (b->lds > 64 * 1024)
+    BATCH_HIP(b, hipFuncSetAttribute((const void*)k_bt_spmv,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+  return ELASTIC_OK;
+}
+
+static int upload_lists(elastic_batch* b) {
+  b->h_swork.clear();
+  b->h_vwork.clear();
+  b->h_slist.clear();
+  size_t lds = 0;
+  for (int k = 0; k < b->max_slots; ++k) {
+    const auto& S = b->slots[k];
+    if (!S.used) continue;
+    b->h_slist.push_back(k);
+    for (int t = 0; t < S.ntiles; ++t) b->h_swork.push_back(((uint32_t)k << 24) | (uint32_t)t);
+    for (int c = 0; c < S.nch; ++c) b->h_vwork.push_back(((uint32_t)k << 24) | (uint32_t)c);
+    lds = std::max(lds, S.lds);
+  }
+  auto grow = [&](uint32_t*& p, size_t& cap, size_t need) -> int {
+    if (need <= cap) return ELASTIC_OK;
+    size_t nc = std::max(need, 2 * cap + 4096);
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc((void**)&p, nc * sizeof(uint32_t)) != hipSuccess)
+      return berr(b, ELASTIC_OOM, "batch work list");
+    cap = nc;
+    b->graph_lds = 0;  // pointers changed: capture again
+    return ELASTIC_OK;
+  };
+  int rc = grow(b->d_swork, b->swork_cap, b->h_swork.size());
+  if (rc) return rc;
+  rc = grow(b->d_vwork, b->vwork_cap, b->h_vwork.size());
+  if (rc) return rc;
+  b->h_counts = {(int32_t)b->h_swork.size(), (int32_t)b->h_vwork.size(),
+                 (int32_t)b->h_slist.size()};
+  hipStream_t s = b->stream;
+  if (!b->h_swork.empty())
+    BATCH_HIP(b, hipMemcpyAsync(b->d_swork, b->h_swork.data(), b->h_swork.size() * 4,
+                                hipMemcpyHostToDevice, s));
+  if (!b->h_vwork.empty())
+    BATCH_HIP(b, hipMemcpyAsync(b->d_vwork, b->h_vwork.data(), b->h_vwork.size() * 4,
+                                hipMemcpyHostToDevice, s));
+  if (!b->h_slist.empty())
+    BATCH_HIP(b, hipMemcpyAsync(b->d_slist, b->h_slist.data(), b->h_slist.size() * 4,
+                                hipMemcpyHostToDevice, s));
+  BATCH_HIP(b, hipMemcpyAsync(b->d_counts, b->h_counts.data(), 3 * 4, hipMemcpyHostToDevice, s));
+  // the host vectors must outlive the copies: wait here (admission time only)
+  BATCH_HIP(b, hipStreamSynchronize(s));
+  if (lds > b->lds) b->lds = lds;
+  b->dirty = false;
+  return ELASTIC_OK;
+}
+
+static int capture(elastic_batch* b) {
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  int rc = set_lds_limit(b);
+  if (rc) return rc;
+  BATCH_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int it = 0; it < b->iters; ++it) enqueue_iteration(b, it);
+  BATCH_HIP(b, hipStreamEndCapture(b->stream, &b->graph));
+  BATCH_HIP(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  b->graph_lds = b->lds;
+  return ELASTIC_OK;
+}
+
+extern "C" {
+
+int elastic_batch_create(elastic_batch** out, int32_t device, int32_t max_slots,
+                         int32_t iters_per_step, int32_t use_graph) {
+  if (!out || max_slots < 1 || max_slots > kMaxSlots || iters_per_step < 2) return ELASTIC_BAD_ARG;
+  *out = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return ELASTIC_HIP_ERROR;
+  auto* b = new elastic_batch();
+  b->device = device;
+  b->max_slots = max_slots;
+  b->iters = iters_per_step & ~1;  // even: steps start at even parity
+  b->use_graph = use_graph != 0;
+  if (const char* e = getenv("ELASTIC_RELIABLE_DELTA")) b->delta = atof(e);
+  b->slots.resize(max_slots);
+  b->h_insts.resize(max_slots);
+  std::memset(b->h_insts.data(), 0, sizeof(BInst) * max_slots);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc((void**)&b->d_insts, sizeof(BInst) * max_slots) != hipSuccess ||
+      hipMalloc((void**)&b->d_slist, sizeof(int32_t) * max_slots) != hipSuccess ||
+      hipMalloc((void**)&b->d_counts, sizeof(int32_t) * 4) != hipSuccess ||
+      hipMalloc((void**)&b->d_status, sizeof(BStatus) * max_slots) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_status, sizeof(BStatus) * max_slots) != hipSuccess) {
+    elastic_batch_destroy(b);
+    return ELASTIC_HIP_ERROR;
+  }
+  if (hipMemcpy(b->d_insts, b->h_insts.data(), sizeof(BInst) * max_slots,
+                hipMemcpyHostToDevice) != hipSuccess) {
+    elastic_batch_destroy(b);
+    return ELASTIC_HIP_ERROR;
+  }
+  *out = b;
+  return ELASTIC_OK;
+}
+
+int elastic_batch_destroy(elastic_batch* b) {
+  if (!b) return ELASTIC_OK;
+  hipSetDevice(b->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  for (void* p : {(void*)b->d_insts, (void*)b->d_swork, (void*)b->d_vwork, (void*)b->d_slist,
+                  (void*)b->d_counts, (void*)b->d_status})
+    if (p) hipFree(p);
+  if (b->h_status) hipHostFree(b->h_status);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return ELASTIC_OK;
+}
+
+const char* elastic_batch_last_error(const elastic_batch* b) {
+  return b ? b->err.c_str() : "null batch";
+}
+
+int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, double tol,
+                         int32_t maxit, int64_t tag) {
+  if (!b || !x) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  if (!c->values_set) return berr(b, ELASTIC_NOT_READY, "context has no values");
+  if (c->layout_nb < 2) return berr(b, ELASTIC_BAD_ARG, "batched solves need a node-blocked layout");
+  if (c->blk.nblk_max > 256 * kBatchPer)
+    return berr(b, ELASTIC_BAD_ARG, "tiles of more than 512 blocks");
+  if (c->zero_diag_row > 0) return berr(b, ELASTIC_ZERO_DIAG, "zero diagonal coefficient");
+  if (precision != ELASTIC_FP64 && precision != ELASTIC_MIXED) return ELASTIC_BAD_ARG;
+  if (maxit <= 0) return ELASTIC_BAD_ARG;
+  if (tol <= 0.0) tol = 2.220446049250313e-16;
+  const int nch = (c->neq + kChunk - 1) / kChunk;
+  if (3 * nch > 8 * kRedBlocks) return berr(b, ELASTIC_BAD_ARG, "system too large for a batch slot");
+  if (c->blk.ntiles >= (1 << 24) || nch >= (1 << 24)) return ELASTIC_BAD_ARG;
+  int k = -1;
+  for (int i = 0; i < b->max_slots; ++i)
+    if (!b->slots[i].used) {
+      k = i;
+      break;
+    }
+  if (k < 0) return berr(b, ELASTIC_NOT_READY, "no free slot");
+  hipSetDevice(b->device);
+  // the context's own work (uploads, assembly) must be complete
+  BATCH_HIP(b, hipStreamSynchronize(c->stream));
+  BInst I;
+  std::memset(&I, 0, sizeof(I));
+  I.a = blk_args(c->blk);
+  I.v64 = c->blk.v64;
+  I.d64 = c->blk.d64;
+  I.v32 = c->blk.v32;
+  I.d32 = c->blk.d32;
+  I.inc_ptr = c->inc_ptr;
+  I.inc_idx = c->inc_idx;
+  I.epart = c->epart64;
+  I.x = c->v64[VX];
+  I.r = c->v64[VR];
+  I.p = c->v64[VP];
+  I.q = c->v64[VQ];
+  I.b = c->v64[VB];
+  I.t = c->v64[VT];
+  I.m = c->A64.m;
+  I.part = c->red64;
+  I.dslot = c->red64 + 8 * kRedBlocks + 8;
+  I.sc = c->scal;
+  I.n = c->neq;
+  I.nb = c->layout_nb;
+  I.nch = nch;
+  I.fp32 = precision == ELASTIC_MIXED;
+  b->h_insts[k] = I;
+  BATCH_HIP(b, hipMemcpyAsync(b->d_insts + k, &b->h_insts[k], sizeof(BInst), hipMemcpyHostToDevice,
+                              b->stream));
+  hipLaunchKernelGGL(k_bt_init, dim3(nch), dim3(kThreads), 0, b->stream, b->d_insts, k);
+  hipLaunchKernelGGL(k_bt_init2, dim3(1), dim3(kThreads), 0, b->stream, b->d_insts, k, tol, maxit,
+                     b->delta);
+  auto& S = b->slots[k];
+  S.used = true;
+  S.ctx = x;
+  S.tag = tag;
+  S.ntiles = c->blk.ntiles;
+  S.nch = nch;
+  S.lds = tile_lds(c->layout_nb, c->blk.ncols_max, c->blk.nblk_max);
+  S.t0 = std::chrono::steady_clock::now();
+  b->dirty = true;
+  return ELASTIC_OK;
+}
+
+int elastic_batch_active(elastic_batch* b, int32_t* n) {
+  if (!b || !n) return ELASTIC_BAD_ARG;
+  int a = 0;
+  for (auto& S : b->slots) a += S.used;
+  *n = a;
+  return ELASTIC_OK;
+}
+
+int elastic_batch_step(elastic_batch* b) {
+  if (!b) return ELASTIC_BAD_ARG;
+  hipSetDevice(b->device);
+  if (b->dirty) {
+    int rc = upload_lists(b);
+    if (rc) return rc;
+  }
+  if (b->h_slist.empty()) return ELASTIC_OK;
+  if (b->use_graph) {
+    if (!b->gexec || b->graph_lds < b->lds) {
+      int rc = capture(b);
+      if (rc) return rc;
+    }
+    BATCH_HIP(b, hipGraphLaunch(b->gexec, b->stream));
+  } else {
+    int rc = set_lds_limit(b);
+    if (rc) return rc;
+    for (int it = 0; it < b->iters; ++it) enqueue_iteration(b, it);
+  }
+  hipLaunchKernelGGL(k_bt_status, dim3(1), dim3(256), 0, b->stream, b->d_insts, b->d_slist,
+                     b->d_counts + 2, b->d_status);
+  BATCH_HIP(b, hipMemcpyAsync(b->h_status, b->d_status, sizeof(BStatus) * b->h_slist.size(),
+                              hipMemcpyDeviceToHost, b->stream));
+  b->stepped = true;
+  return ELASTIC_OK;
+}
+
+int elastic_batch_poll(elastic_batch* b, int64_t* tags, elastic_stats* stats, int32_t cap,
+                       int32_t* ndone) {
+  if (!b || !ndone || (cap > 0 && !tags)) return ELASTIC_BAD_ARG;
+  *ndone = 0;
+  hipSetDevice(b->device);
+  BATCH_HIP(b, hipStreamSynchronize(b->stream));
+  BATCH_HIP(b, hipGetLastError());
+  if (!b->stepped) return ELASTIC_OK;
+  int nd = 0;
+  for (size_t w = 0; w < b->h_slist.size() && nd < cap; ++w) {
+    const BStatus& st = b->h_status[w];
+    if (!st.done) continue;
+    const int k = b->h_slist[w];
+    auto& S = b->slots[k];
+    if (!S.used) continue;
+    tags[nd] = S.tag;
+    if (stats) {
+      elastic_stats& o = stats[nd];
+      std::memset(&o, 0, sizeof(o));
+      o.iters = st.iter;
+      o.outer = st.nupd;
+      o.status = st.status;
+      o.spmv_count = st.iter + st.nupd;
+      o.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                              S.t0).count();
+      // post-solve figures on the instance's own context (FP64 operator)
+      Context* c = &S.ctx->c;
+      hipMemcpy(c->scal_host, c->scal, sizeof(SolveScalars), hipMemcpyDeviceToHost);
+      o.conv = c->scal_host->conv;
+      int rc = post_solve_stats(c, &o);
+      if (rc) return berr(b, rc, c->err);
+    }
+    S.used = false;
+    S.ctx = nullptr;
+    ++nd;
+  }
+  if (nd) b->dirty = true;
+  b->stepped = false;  // the statuses of this step are consumed
+  *ndone = nd;
+  return ELASTIC_OK;
+}
+
+}  // extern "C"

@@ -420,6 +420,15 @@ static int post_stats(Context* c, elastic_stats* st, double* bnorm2, double* bno
   return ELASTIC_OK;
 }
 
+// post-solve figures and the M-norm relative residual (used by the batch)
+int post_solve_stats(Context* c, elastic_stats* st) {
+  double bn2, bnm;
+  int rc = post_stats(c, st, &bn2, &bnm);
+  if (rc) return rc;
+  st->rel_res = bnm > 0 ? st->norm_m_r / bnm : 0.0;
+  return ELASTIC_OK;
+}
+
 static int solve_pcg(Context* c, int precond, int precision, double tol, int maxit,
                      elastic_stats* st) {
   hipStream_t s = c->stream;

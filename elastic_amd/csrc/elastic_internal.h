@@ -172,6 +172,8 @@ struct Context {
 enum VecSlot { VX = 0, VR, VP, VQ, VB, VT, VU, VW };
 
 int set_error(Context* c, int code, const std::string& msg);
+// x.Kx, ||x||, ||b - Ax||, ||b - Ax||_M and rel_res of the context's x and b
+int post_solve_stats(Context* c, elastic_stats* st);
 
 // krylov.hip: BiCGSTAB (Iters.f:1079-1349) and GMRES(k) (Iters.f:528-808) on
 // device-resident vectors (b in v64[VB], x left in v64[VX]).  fp32_operator

@@ -1,65 +1,9 @@
-// Node-blocked symmetric product y = (D + L + L^T) x on gfx950 (same product
-// as Matvec.f:528-621, on the block layout of layout_block.cpp).
-//
-// One workgroup per tile of node rows.  Every lane owns up to PER node blocks:
-// it loads their nb*nb values (one coalesced plane at a time) and the block
-// word, then forms A_IJ x_J for its row node and A_IJ^T x_I for its column
-// node from the LDS-staged x.  Row pieces are summed across the wavefront by a
-// segmented scan on the row key, transposed products land in LDS at their
-// column-sorted position and each local column node is summed by one lane in a
-// fixed order: deterministic, no atomics.  Values are TV (double, or float for
-// the mixed-precision operator); x, y and all sums are T = double.
-#include "elastic_internal.h"
+// Node-blocked symmetric product: the single-system kernel, the value pack
+// after assembly and the launch dispatch (the tile routine is in
+// spmv_block.h).
+#include "spmv_block.h"
 
 namespace elastic {
-
-struct BlkArgs {
-  const int32_t *row0, *nrows, *nblk, *ncols, *ext0, *seg0, *rp0;
-  const int64_t* blk0;
-  const uint32_t* word;
-  const uint16_t *rowptr, *segptr;
-  const int32_t* ext;
-  int nn, nblk_max, ncols_max;
-};
-
-static BlkArgs blk_args(const DevBlocks& b) {
-  return BlkArgs{b.row0, b.nrows,  b.nblk,   b.ncols, b.ext0, b.seg0,    b.rp0,
-                 b.blk0, b.word,   b.rowptr, b.segptr, b.ext, b.nn, b.nblk_max, b.ncols_max};
-}
-
-// Value planes of a tile: FP64 values are stored plane by plane (a lane's load
-// is 8 bytes); FP32 values are stored with planes 2k and 2k+1 interleaved so a
-// lane loads both with one 8-byte load (the last odd plane stays single).
-template <typename TV, int NB>
-struct Planes;
-template <int NB>
-struct Planes<double, NB> {
-  __device__ static inline void load(const double* v, int nbk, int q, double* out) {
-#pragma unroll
-    for (int p = 0; p < NB * NB; ++p) out[p] = __builtin_nontemporal_load(v + (int64_t)p * nbk + q);
-  }
-  __device__ __host__ static inline int64_t index(int nbk, int q, int p) { return (int64_t)p * nbk + q; }
-};
-template <int NB>
-struct Planes<float, NB> {
-  __device__ static inline void load(const float* v, int nbk, int q, float* out) {
-    constexpr int NP = NB * NB / 2;
-    typedef float f2 __attribute__((ext_vector_type(2)));
-    const f2* v2 = reinterpret_cast<const f2*>(v);
-#pragma unroll
-    for (int k = 0; k < NP; ++k) {
-      const f2 f = __builtin_nontemporal_load(v2 + (int64_t)k * nbk + q);
-      out[2 * k] = f.x;
-      out[2 * k + 1] = f.y;
-    }
-    if (NB * NB % 2) out[NB * NB - 1] = __builtin_nontemporal_load(v + (int64_t)2 * NP * nbk + q);
-  }
-  __device__ __host__ static inline int64_t index(int nbk, int q, int p) {
-    constexpr int NP = NB * NB / 2;
-    if (p < 2 * NP) return (int64_t)2 * ((int64_t)(p / 2) * nbk + q) + (p & 1);
-    return (int64_t)2 * NP * nbk + q;
-  }
-};
 
 template <typename TV, typename T, int NB, int PER>
 __global__ __launch_bounds__(256) void k_spmv_blocks(BlkArgs a, const TV* __restrict__ vals,
@@ -67,130 +11,9 @@ __global__ __launch_bounds__(256) void k_spmv_blocks(BlkArgs a, const TV* __rest
                                                       const T* __restrict__ x, T* __restrict__ y,
                                                       T* __restrict__ epart,
                                                       const int32_t* __restrict__ skip) {
-  constexpr int NB2 = NB * NB;
   extern __shared__ __align__(16) unsigned char smem[];
   if (skip != nullptr && *skip != 0) return;
-  T* xs = reinterpret_cast<T*>(smem);       // ncols_max * NB
-  T* pp = xs + a.ncols_max * NB;            // 2 * kBlkRowsMax * NB (row pieces)
-  T* tr = pp + 2 * kBlkRowsMax * NB;        // nblk_max * NB (transposed products)
-  const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
-  const int I0 = a.row0[t], nr = a.nrows[t], nbk = a.nblk[t], ncols = a.ncols[t];
-  const int ext0 = a.ext0[t];
-  const int64_t base = a.blk0[t];
-  const int32_t* ext = a.ext + ext0;
-  const TV* v = vals + base * NB2;
-  uint32_t wr[PER];
-  TV vr[PER][NB2];
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int q = tid + 256 * i;
-    if (q < nbk) {
-      wr[i] = __builtin_nontemporal_load(a.word + base + q);
-      Planes<TV, NB>::load(v, nbk, q, vr[i]);
-    } else {
-      wr[i] = 0xFFFFFFFFu;
-#pragma unroll
-      for (int p = 0; p < NB2; ++p) vr[i][p] = TV(0);
-    }
-  }
-  for (int c = tid; c < ncols * NB; c += 256) {
-    const int node = c / NB, comp = c - node * NB;
-    const int g = node < nr ? I0 + node : ext[node - nr];
-    xs[c] = x[(int64_t)NB * g + comp];
-  }
-  __syncthreads();
-  const uint16_t* rp = a.rowptr + a.rp0[t];
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int q = tid + 256 * i;
-    const bool valid = q < nbk;
-    const uint32_t w = wr[i];
-    const int key = valid ? (int)(w >> 25) : 255;
-    const int lc = (w >> 10) & 0x7FFF, tp = w & 0x3FF;
-    T yl[NB];
-#pragma unroll
-    for (int r = 0; r < NB; ++r) yl[r] = T(0);
-    if (valid) {
-      T xc[NB], xr[NB];
-#pragma unroll
-      for (int r = 0; r < NB; ++r) {
-        xc[r] = xs[lc * NB + r];
-        xr[r] = xs[key * NB + r];
-      }
-#pragma unroll
-      for (int r = 0; r < NB; ++r) {
-        T s = T(0);
-#pragma unroll
-        for (int cc = 0; cc < NB; ++cc) s += (T)vr[i][r * NB + cc] * xc[cc];
-        yl[r] = s;
-      }
-#pragma unroll
-      for (int cc = 0; cc < NB; ++cc) {
-        T s = T(0);
-#pragma unroll
-        for (int r = 0; r < NB; ++r) s += (T)vr[i][r * NB + cc] * xr[r];
-        tr[tp * NB + cc] = s;
-      }
-    }
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int ok = __shfl_up(key, d, 64);
-#pragma unroll
-      for (int r = 0; r < NB; ++r) {
-        const T o = __shfl_up(yl[r], d, 64);
-        if (lane >= d && ok == key) yl[r] += o;
-      }
-    }
-    // a node row has at most 64 blocks (checked when the layout is built), so
-    // it ends in its own chunk or in the next one: two pieces at most
-    const bool row_end = valid && (q + 1 == (int)rp[key + 1]);
-    if (valid && (lane == 63 || row_end)) {
-      const int slot = 2 * key + (row_end ? 1 : 0);
-#pragma unroll
-      for (int r = 0; r < NB; ++r) pp[slot * NB + r] = yl[r];
-    }
-  }
-  __syncthreads();
-  const uint16_t* sp = a.segptr + a.seg0[t];
-  for (int c = tid; c < ncols; c += 256) {
-    T s[NB];
-#pragma unroll
-    for (int r = 0; r < NB; ++r) s[r] = T(0);
-    if (c < nr) {
-      // node diagonal block (nb diagonal entries, then a21, a31, a32)
-      const int g = I0 + c;
-      T d[NB * (NB + 1) / 2];
-#pragma unroll
-      for (int k = 0; k < NB * (NB + 1) / 2; ++k) d[k] = (T)diag[(int64_t)k * a.nn + g];
-      const T* xi = xs + c * NB;
-      if constexpr (NB == 3) {
-        s[0] = d[0] * xi[0] + d[3] * xi[1] + d[4] * xi[2];
-        s[1] = d[3] * xi[0] + d[1] * xi[1] + d[5] * xi[2];
-        s[2] = d[4] * xi[0] + d[5] * xi[1] + d[2] * xi[2];
-      } else {
-        s[0] = d[0] * xi[0] + d[2] * xi[1];
-        s[1] = d[2] * xi[0] + d[1] * xi[1];
-      }
-      const int b0 = rp[c], b1 = rp[c + 1];
-      if (b1 > b0) {
-        if ((b0 | 63) < b1 - 1)  // the row crosses a chunk boundary
-#pragma unroll
-          for (int r = 0; r < NB; ++r) s[r] += pp[(2 * c) * NB + r];
-#pragma unroll
-        for (int r = 0; r < NB; ++r) s[r] += pp[(2 * c + 1) * NB + r];
-      }
-    }
-    for (int q = sp[c]; q < sp[c + 1]; ++q)
-#pragma unroll
-      for (int r = 0; r < NB; ++r) s[r] += tr[q * NB + r];
-    if (c < nr) {
-#pragma unroll
-      for (int r = 0; r < NB; ++r) y[(int64_t)NB * (I0 + c) + r] = s[r];
-    } else {
-#pragma unroll
-      for (int r = 0; r < NB; ++r) epart[(int64_t)NB * (ext0 + c - nr) + r] = s[r];
-    }
-  }
+  block_tile<TV, T, NB, PER>(a, blockIdx.x, vals, diag, x, y, epart, smem);
 }
 
 // Gathers the CSRC al/ad into the block planes (one workgroup per tile) and

@@ -171,6 +171,36 @@ int elastic_layout_info(elastic_ctx* ctx, int64_t* info4);
  * elastic_set_structure. */
 int elastic_set_tile_nnz(elastic_ctx* ctx, int32_t tile_nnz);
 
+/* ------------------------------------------------------ batched serving */
+/* No counterpart in the reference (its solver serves one system per call).
+ * A batch iterates many independent systems together: every PCG phase is one
+ * launch over all active instances, and a step of iters_per_step iterations
+ * is captured once as a HIP graph (use_graph != 0) and replayed while
+ * instances join and leave.  Each instance is a context holding its
+ * structure, values (node-blocked layout) and b (elastic_upload_rhs); its
+ * method is the PCG of elastic_solve (Iters.f:252-524, Jacobi), FP64 or
+ * MIXED, and its result is bit for bit the same alone and in any mix.  x is
+ * left in the context (elastic_download_x).  One host thread drives a batch;
+ * other threads may prepare contexts meanwhile. */
+typedef struct elastic_batch elastic_batch;
+int elastic_batch_create(elastic_batch** b, int32_t device, int32_t max_slots,
+                         int32_t iters_per_step, int32_t use_graph);
+int elastic_batch_destroy(elastic_batch* b);
+const char* elastic_batch_last_error(const elastic_batch* b);
+/* Admits ctx as a new instance (x0 = 0, stop when ||r||_M < tol ||b||_M or
+ * at maxit); ELASTIC_NOT_READY when every slot is taken.  The context must
+ * not be used by the caller until the instance is retired by a poll. */
+int elastic_batch_submit(elastic_batch* b, elastic_ctx* ctx, int32_t precision, double tol,
+                         int32_t maxit, int64_t tag);
+/* Enqueues one step for every active instance (asynchronous). */
+int elastic_batch_step(elastic_batch* b);
+/* Waits for the last step and retires finished instances: their tags (and,
+ * when stats != NULL, their statistics, with t_total_ms = time since submit)
+ * go to the first *ndone entries; their slots are free again. */
+int elastic_batch_poll(elastic_batch* b, int64_t* tags, elastic_stats* stats, int32_t cap,
+                       int32_t* ndone);
+int elastic_batch_active(elastic_batch* b, int32_t* nactive);
+
 #ifdef __cplusplus
 }
 #endif
