@@ -78,6 +78,9 @@ def lib():
                                     C.POINTER(C.c_double)]
     L.elastic_layout_info.argtypes = [vp, I64P]
     L.elastic_set_tile_nnz.argtypes = [vp, C.c_int32]
+    L.elastic_set_tile_blocks.argtypes = [vp, C.c_int32]
+    L.elastic_autotune.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double)]
     L.elastic_set_mesh.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                    C.c_int32, I32P, F64P, I32P, I32P, F64P, I32P, C.c_int32, I32P,
                                    I32P, F64P]

@@ -768,10 +768,7 @@ extern "C" int elastic_assemble(elastic_ctx* xc, int32_t isw, int32_t lhs, int32
         c->zero_diag_row = i + 1;
         break;
       }
-    launch_recip<double>(c->neq, c->A64.ad, c->A64.m, s);
-    launch_cast<double, float>(c->neq, c->A64.ad, c->A32.ad, s);
-    launch_cast<double, float>(c->nad, c->A64.al, c->A32.al, s);
-    launch_block_pack(c);
+    derive_operator(c);
     ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
     c->values_set = true;
     c->bdiag_set = false;

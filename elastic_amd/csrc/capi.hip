@@ -58,16 +58,24 @@ static void free_blocks(DevBlocks& b) {
   b = DevBlocks();
 }
 
-static void free_system(Context* c) {
+// device layout of the product (tiles or node blocks, FP32 copies, partials)
+static void free_layout(Context* c) {
   free_tiles(c->tiles);
   free_blocks(c->blk);
   c->inc_ptr = c->inc_idx = nullptr;
-  void* ptrs[] = {c->A64.ad, c->A64.al, c->A64.m, c->A64.bm, c->A32.ad, c->A32.al, c->epart64};
+  void* ptrs[] = {c->A32.ad, c->A32.al, c->epart64};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  c->A32 = DevMatrix<float>();
+  c->epart64 = nullptr;
+}
+
+static void free_system(Context* c) {
+  free_layout(c);
+  void* ptrs[] = {c->A64.ad, c->A64.al, c->A64.m, c->A64.bm};
   for (void* p : ptrs)
     if (p) hipFree(p);
   c->A64 = DevMatrix<double>();
-  c->A32 = DevMatrix<float>();
-  c->epart64 = nullptr;
   for (auto& v : c->v64)
     if (v) hipFree(v), v = nullptr;
   if (c->gm_basis) hipFree(c->gm_basis);
@@ -82,112 +90,18 @@ static void free_system(Context* c) {
 using namespace elastic;
 
 
-extern "C" {
-
-int elastic_create(elastic_ctx** out, int32_t device) {
-  if (!out) return ELASTIC_BAD_ARG;
-  *out = nullptr;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ELASTIC_HIP_ERROR;
-  if (device < 0 || device >= ndev) return ELASTIC_BAD_ARG;
-  if (hipSetDevice(device) != hipSuccess) return ELASTIC_HIP_ERROR;
-  auto* x = new elastic_ctx();
+// Builds the device layout of the product from x->ia / x->ja (node-blocked
+// when the pattern allows it, else scalar row tiles) with the context's tile
+// sizes; values are packed separately (derive_operator).
+static int build_layout(elastic_ctx* x) {
   Context* c = &x->c;
-  c->device = device;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-      hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
-      hipMalloc((void**)&c->scal, sizeof(SolveScalars)) != hipSuccess ||
-      hipHostMalloc((void**)&c->scal_host, sizeof(SolveScalars), hipHostMallocDefault) != hipSuccess ||
-      hipMalloc((void**)&c->red64, sizeof(double) * (8 * kRedBlocks + 64)) != hipSuccess) {
-    delete x;
-    return ELASTIC_HIP_ERROR;
-  }
-  const char* ce = getenv("ELASTIC_CHECK_EVERY");
-  if (ce) c->check_every = std::max(1, atoi(ce));
-  *out = x;
-  return ELASTIC_OK;
-}
-
-int elastic_destroy(elastic_ctx* x) {
-  if (!x) return ELASTIC_OK;
-  Context* c = &x->c;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
-  free_system(c);
-  if (c->mesh) free_mesh((MeshDev*)c->mesh);
-  c->mesh = nullptr;
-  if (c->scal) hipFree(c->scal);
-  if (c->scal_host) hipHostFree(c->scal_host);
-  if (c->red64) hipFree(c->red64);
-  for (hipEvent_t e : {c->ev0, c->ev1, c->ev2, c->ev3})
-    if (e) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
-  delete x;
-  return ELASTIC_OK;
-}
-
-const char* elastic_last_error(const elastic_ctx* x) { return x ? x->c.err.c_str() : "null context"; }
-
-// ------------------------------------------------------------ host set-up
-int elastic_reord(int32_t nnode, int32_t numel, int32_t nen, const int32_t* ix, int32_t* perm) {
-  if (nnode <= 0 || numel < 0 || nen <= 0 || !ix || !perm) return ELASTIC_BAD_ARG;
-  try {
-    rcm_order(nnode, numel, nen, ix, perm);
-  } catch (const std::bad_alloc&) {
-    return ELASTIC_OOM;
-  }
-  return ELASTIC_OK;
-}
-
-int elastic_numeq(int32_t nnode, int32_t ndf, const int32_t* perm, int32_t* id, int32_t* neq) {
-  if (nnode <= 0 || ndf <= 0 || !perm || !id || !neq) return ELASTIC_BAD_ARG;
-  *neq = number_equations(id, perm, nnode, ndf);
-  return ELASTIC_OK;
-}
-
-int elastic_csrstruct(int32_t nnode, int32_t numel, int32_t nen, int32_t ndf, int32_t neq,
-                      const int32_t* ix, const int32_t* id, int32_t* ia, int32_t* ja,
-                      int64_t* nad) {
-  if (nnode <= 0 || neq < 0 || !ix || !id || !ia || !nad) return ELASTIC_BAD_ARG;
-  try {
-    std::vector<int32_t> vi, vj;
-    csrc_pattern(id, ix, nnode, numel, nen, ndf, neq, vi, vj);
-    std::copy(vi.begin(), vi.end(), ia);
-    *nad = (int64_t)vj.size();
-    if (ja) std::copy(vj.begin(), vj.end(), ja);
-  } catch (const std::bad_alloc&) {
-    return ELASTIC_OOM;
-  }
-  return ELASTIC_OK;
-}
-
-// ----------------------------------------------------------- device system
-int elastic_set_tile_nnz(elastic_ctx* x, int32_t tile_nnz) {
-  if (!x || tile_nnz < 64 || tile_nnz > kTileColsMax - kTileRowsMax) return ELASTIC_BAD_ARG;
-  x->c.tile_nnz = tile_nnz;
-  return ELASTIC_OK;
-}
-
-int elastic_set_structure(elastic_ctx* x, int32_t neq, int64_t nad, const int32_t* ia,
-                          const int32_t* ja) {
-  if (!x || neq <= 0 || nad < 0 || !ia || (nad > 0 && !ja)) return ELASTIC_BAD_ARG;
-  Context* c = &x->c;
-  hipSetDevice(c->device);
-  if (ia[0] != 1 || (int64_t)ia[neq] - 1 != nad)
-    return set_error(c, ELASTIC_BAD_ARG, "ia must start at 1 and end at nad+1");
-  for (int i = 0; i < neq; ++i) {
-    if (ia[i + 1] < ia[i]) return set_error(c, ELASTIC_BAD_ARG, "ia not monotone");
-    for (int64_t k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
-      if (ja[k] < 1 || ja[k] > i || (k > ia[i] - 1 && ja[k] <= ja[k - 1]))
-        return set_error(c, ELASTIC_BAD_ARG, "ja must be strictly lower and ascending per row");
-  }
-  hipStreamSynchronize(c->stream);
-  free_system(c);
+  const int neq = c->neq;
+  const int64_t nad = c->nad;
+  const int32_t* ia = x->ia.data();
+  const int32_t* ja = x->ja.data();
   int nb = 1;
   const char* lay = getenv("ELASTIC_LAYOUT");
   if (c->allow_blocks && !(lay && std::string(lay) == "scalar")) nb = detect_node_blocks(neq, ia, ja);
-  if (const char* tb = getenv("ELASTIC_TILE_BLOCKS")) c->tile_blocks = std::max(32, atoi(tb));
   try {
     if (nb > 1) build_block_tiles(neq, nb, ia, ja, c->tile_blocks, c->blayout);
     else build_tiles(neq, ia, ja, c->tile_nnz, c->layout);
@@ -195,10 +109,6 @@ int elastic_set_structure(elastic_ctx* x, int32_t neq, int64_t nad, const int32_
     return set_error(c, ELASTIC_BAD_ARG, e.what());
   }
   c->layout_nb = nb;
-  x->ia.assign(ia, ia + neq + 1);
-  x->ja.assign(ja, ja + nad);
-  c->neq = neq;
-  c->nad = nad;
   int rc = 0;
   int64_t nepart = 0;
   if (nb == 1) {
@@ -259,23 +169,188 @@ int elastic_set_structure(elastic_ctx* x, int32_t neq, int64_t nad, const int32_
     c->inc_idx = b.inc_idx;
     nepart = (int64_t)nb * L.next_nodes;
   }
-  rc = rc ? rc : dalloc(c, &c->A64.ad, neq);
-  rc = rc ? rc : dalloc(c, &c->A64.al, (size_t)nad);
-  rc = rc ? rc : dalloc(c, &c->A64.m, neq);
-  rc = rc ? rc : dalloc(c, &c->A32.ad, neq);
-  rc = rc ? rc : dalloc(c, &c->A32.al, (size_t)nad);
-  rc = rc ? rc : dalloc(c, &c->epart64, (size_t)nepart);
-  for (auto& v : c->v64) rc = rc ? rc : dalloc(c, &v, neq);
-  if (rc) {
-    free_system(c);
-    return rc;
+  if (nb == 1) {  // FP32 operator of the scalar tiles (the blocked layout has its planes)
+    rc = rc ? rc : dalloc(c, &c->A32.ad, neq);
+    rc = rc ? rc : dalloc(c, &c->A32.al, (size_t)nad);
   }
+  rc = rc ? rc : dalloc(c, &c->epart64, (size_t)nepart);
+  if (rc) return rc;
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   // the host streams are no longer needed
   c->layout.word.clear();
   c->layout.word.shrink_to_fit();
   c->blayout.word.clear();
   c->blayout.word.shrink_to_fit();
+  return ELASTIC_OK;
+}
+
+extern "C" {
+
+int elastic_create(elastic_ctx** out, int32_t device) {
+  if (!out) return ELASTIC_BAD_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ELASTIC_HIP_ERROR;
+  if (device < 0 || device >= ndev) return ELASTIC_BAD_ARG;
+  if (hipSetDevice(device) != hipSuccess) return ELASTIC_HIP_ERROR;
+  auto* x = new elastic_ctx();
+  Context* c = &x->c;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
+      hipMalloc((void**)&c->scal, sizeof(SolveScalars)) != hipSuccess ||
+      hipHostMalloc((void**)&c->scal_host, sizeof(SolveScalars), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void**)&c->red64, sizeof(double) * (8 * kRedBlocks + 64)) != hipSuccess) {
+    delete x;
+    return ELASTIC_HIP_ERROR;
+  }
+  if (const char* tb = getenv("ELASTIC_TILE_BLOCKS"))
+    c->tile_blocks = std::max(32, std::min(kBlkTileMax, atoi(tb)));
+  const char* ce = getenv("ELASTIC_CHECK_EVERY");
+  if (ce) c->check_every = std::max(1, atoi(ce));
+  *out = x;
+  return ELASTIC_OK;
+}
+
+int elastic_destroy(elastic_ctx* x) {
+  if (!x) return ELASTIC_OK;
+  Context* c = &x->c;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_system(c);
+  if (c->mesh) free_mesh((MeshDev*)c->mesh);
+  c->mesh = nullptr;
+  if (c->scal) hipFree(c->scal);
+  if (c->scal_host) hipHostFree(c->scal_host);
+  if (c->red64) hipFree(c->red64);
+  for (hipEvent_t e : {c->ev0, c->ev1, c->ev2, c->ev3})
+    if (e) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete x;
+  return ELASTIC_OK;
+}
+
+const char* elastic_last_error(const elastic_ctx* x) { return x ? x->c.err.c_str() : "null context"; }
+
+// ------------------------------------------------------------ host set-up
+int elastic_reord(int32_t nnode, int32_t numel, int32_t nen, const int32_t* ix, int32_t* perm) {
+  if (nnode <= 0 || numel < 0 || nen <= 0 || !ix || !perm) return ELASTIC_BAD_ARG;
+  try {
+    rcm_order(nnode, numel, nen, ix, perm);
+  } catch (const std::bad_alloc&) {
+    return ELASTIC_OOM;
+  }
+  return ELASTIC_OK;
+}
+
+int elastic_numeq(int32_t nnode, int32_t ndf, const int32_t* perm, int32_t* id, int32_t* neq) {
+  if (nnode <= 0 || ndf <= 0 || !perm || !id || !neq) return ELASTIC_BAD_ARG;
+  *neq = number_equations(id, perm, nnode, ndf);
+  return ELASTIC_OK;
+}
+
+int elastic_csrstruct(int32_t nnode, int32_t numel, int32_t nen, int32_t ndf, int32_t neq,
+                      const int32_t* ix, const int32_t* id, int32_t* ia, int32_t* ja,
+                      int64_t* nad) {
+  if (nnode <= 0 || neq < 0 || !ix || !id || !ia || !nad) return ELASTIC_BAD_ARG;
+  try {
+    std::vector<int32_t> vi, vj;
+    csrc_pattern(id, ix, nnode, numel, nen, ndf, neq, vi, vj);
+    std::copy(vi.begin(), vi.end(), ia);
+    *nad = (int64_t)vj.size();
+    if (ja) std::copy(vj.begin(), vj.end(), ja);
+  } catch (const std::bad_alloc&) {
+    return ELASTIC_OOM;
+  }
+  return ELASTIC_OK;
+}
+
+// ----------------------------------------------------------- device system
+int elastic_set_tile_blocks(elastic_ctx* x, int32_t tile_blocks) {
+  if (!x || tile_blocks < 32 || tile_blocks > kBlkTileMax) return ELASTIC_BAD_ARG;
+  x->c.tile_blocks = tile_blocks;
+  return ELASTIC_OK;
+}
+
+int elastic_set_tile_nnz(elastic_ctx* x, int32_t tile_nnz) {
+  if (!x || tile_nnz < 64 || tile_nnz > kTileColsMax - kTileRowsMax) return ELASTIC_BAD_ARG;
+  x->c.tile_nnz = tile_nnz;
+  return ELASTIC_OK;
+}
+
+int elastic_set_structure(elastic_ctx* x, int32_t neq, int64_t nad, const int32_t* ia,
+                          const int32_t* ja) {
+  if (!x || neq <= 0 || nad < 0 || !ia || (nad > 0 && !ja)) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  hipSetDevice(c->device);
+  if (ia[0] != 1 || (int64_t)ia[neq] - 1 != nad)
+    return set_error(c, ELASTIC_BAD_ARG, "ia must start at 1 and end at nad+1");
+  for (int i = 0; i < neq; ++i) {
+    if (ia[i + 1] < ia[i]) return set_error(c, ELASTIC_BAD_ARG, "ia not monotone");
+    for (int64_t k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
+      if (ja[k] < 1 || ja[k] > i || (k > ia[i] - 1 && ja[k] <= ja[k - 1]))
+        return set_error(c, ELASTIC_BAD_ARG, "ja must be strictly lower and ascending per row");
+  }
+  hipStreamSynchronize(c->stream);
+  free_system(c);
+  x->ia.assign(ia, ia + neq + 1);
+  x->ja.assign(ja, ja + nad);
+  c->neq = neq;
+  c->nad = nad;
+  int rc = build_layout(x);
+  if (rc) {
+    free_system(c);
+    return rc;
+  }
+  rc = rc ? rc : dalloc(c, &c->A64.ad, neq);
+  rc = rc ? rc : dalloc(c, &c->A64.al, (size_t)nad);
+  rc = rc ? rc : dalloc(c, &c->A64.m, neq);
+  for (auto& v : c->v64) rc = rc ? rc : dalloc(c, &v, neq);
+  if (rc) {
+    free_system(c);
+    return rc;
+  }
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return ELASTIC_OK;
+}
+
+// Autotuner: times the product (tile kernel + merge, HIP events) for each
+// candidate tile size on the resident values and keeps the fastest layout.
+int elastic_autotune(elastic_ctx* x, int32_t precision, int32_t reps, int32_t* best,
+                     double* best_ms) {
+  if (!x || reps <= 0) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
+  hipSetDevice(c->device);
+  const bool blocked = c->layout_nb > 1;
+  const std::vector<int> cands = blocked ? std::vector<int>{128, 256, 384, 512, 768, 1024}
+                                         : std::vector<int>{1024, 2048, 3072, 3840};
+  int arg = -1;
+  double tmin = 1e300;
+  auto apply = [&](int v) -> int {
+    if (blocked) c->tile_blocks = v;
+    else c->tile_nnz = v;
+    ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    free_layout(c);
+    int rc = build_layout(x);
+    if (rc) return rc;
+    derive_operator(c);
+    ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return ELASTIC_OK;
+  };
+  for (int v : cands) {
+    int rc = apply(v);
+    if (rc) return rc;
+    double tot = 0, ker = 0;
+    rc = elastic_time_spmv(x, precision, reps, &tot, &ker);
+    if (rc) return rc;
+    if (tot < tmin) tmin = tot, arg = v;
+  }
+  int rc = apply(arg);
+  if (rc) return rc;
+  if (best) *best = arg;
+  if (best_ms) *best_ms = tmin;
   return ELASTIC_OK;
 }
 
@@ -293,10 +368,7 @@ int elastic_set_values(elastic_ctx* x, const double* ad, const double* al) {
   ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->A64.ad, ad, sizeof(double) * c->neq, hipMemcpyHostToDevice, c->stream));
   if (c->nad)
     ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->A64.al, al, sizeof(double) * c->nad, hipMemcpyHostToDevice, c->stream));
-  launch_recip<double>(c->neq, c->A64.ad, c->A64.m, c->stream);
-  launch_cast<double, float>(c->neq, c->A64.ad, c->A32.ad, c->stream);
-  launch_cast<double, float>(c->nad, c->A64.al, c->A32.al, c->stream);
-  launch_block_pack(c);
+  derive_operator(c);
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   c->values_set = true;
   c->bdiag_set = false;
@@ -418,6 +490,16 @@ static int post_stats(Context* c, elastic_stats* st, double* bnorm2, double* bno
   *bnorm2 = std::sqrt(h[4]);
   *bnormm = std::sqrt(std::fabs(h[5]));
   return ELASTIC_OK;
+}
+
+void derive_operator(Context* c) {
+  launch_recip<double>(c->neq, c->A64.ad, c->A64.m, c->stream);
+  if (c->layout_nb == 1) {
+    launch_cast<double, float>(c->neq, c->A64.ad, c->A32.ad, c->stream);
+    launch_cast<double, float>(c->nad, c->A64.al, c->A32.al, c->stream);
+  } else {
+    launch_block_pack(c);
+  }
 }
 
 // post-solve figures and the M-norm relative residual (used by the batch)

@@ -172,6 +172,9 @@ struct Context {
 enum VecSlot { VX = 0, VR, VP, VQ, VB, VT, VU, VW };
 
 int set_error(Context* c, int code, const std::string& msg);
+// arrays derived from new A64 values (asynchronous on c->stream): the
+// Jacobi inverse, the FP32 operator (scalar tiles) or the block planes
+void derive_operator(Context* c);
 // x.Kx, ||x||, ||b - Ax||, ||b - Ax||_M and rel_res of the context's x and b
 int post_solve_stats(Context* c, elastic_stats* st);
 

@@ -75,6 +75,16 @@ class Context:
         return dict(ntiles=int(info[0]), nnz_max=int(info[1]), ext_partials=int(info[2]),
                     ncols_max=int(info[3]), nb=int(info[3]))
 
+    def set_tile_blocks(self, tile_blocks: int):
+        self._chk(self._L.elastic_set_tile_blocks(self._h, tile_blocks))
+
+    def autotune(self, precision: int = FP64, reps: int = 20):
+        """Picks the fastest tile size for the resident values (rebuilds the
+        layout); returns (tile size, product time in ms)."""
+        best, ms = C.c_int32(), C.c_double()
+        self._chk(self._L.elastic_autotune(self._h, precision, reps, C.byref(best), C.byref(ms)))
+        return best.value, ms.value
+
     # ---------------------------------------------------------- products
     def spmv(self, x: np.ndarray, precision: int = FP64) -> np.ndarray:
         x = np.ascontiguousarray(x, np.float64)

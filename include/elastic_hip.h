@@ -167,9 +167,18 @@ int elastic_time_spmv(elastic_ctx* ctx, int32_t precision, int32_t reps, double*
                       double* kernel_ms);
 /* Layout facts: ntiles, max tile nnz, external partials, merge entries. */
 int elastic_layout_info(elastic_ctx* ctx, int64_t* info4);
-/* Tile size (nonzeros per tile, 256..3840); takes effect at the next
- * elastic_set_structure. */
+/* Tile size of the scalar row tiles (nonzeros per tile, 64..3840); takes
+ * effect at the next elastic_set_structure. */
 int elastic_set_tile_nnz(elastic_ctx* ctx, int32_t tile_nnz);
+/* Tile size of the node-blocked layout (node blocks per tile, 32..1024);
+ * takes effect at the next elastic_set_structure. */
+int elastic_set_tile_blocks(elastic_ctx* ctx, int32_t tile_blocks);
+/* Autotuner (no counterpart in the reference): with values resident, times
+ * the product of the given precision for each candidate tile size (HIP
+ * events, reps launches each), rebuilds the layout with the fastest and
+ * returns its size and time.  Not while the context is in a batch. */
+int elastic_autotune(elastic_ctx* ctx, int32_t precision, int32_t reps, int32_t* best,
+                     double* best_ms);
 
 /* ------------------------------------------------------ batched serving */
 /* No counterpart in the reference (its solver serves one system per call).

@@ -211,3 +211,20 @@ def test_partially_restrained_nodes_use_scalar_tiles():
         x, st = c.solve(s.b, tol=1e-10)
     xo, so = O.pcg(s.ia, s.ja, s.ad, s.al[: s.nad], s.b, tol=1e-10)
     assert abs(st["iters"] - so["iters"]) <= 2 and rel(x, xo) <= 1e-10
+
+
+@pytest.mark.parametrize("name", ["hex8_5j", "quad4_8"])
+def test_autotune_keeps_results(golden, name):
+    """The autotuner rebuilds the layout with the fastest tile size; the
+    product and the solve stay within the same bars."""
+    from elastic_amd.solver import Context
+
+    g = golden(name)
+    with Context(0) as c:
+        load_system(c, g)
+        best, ms = c.autotune(precision=MIXED, reps=5)
+        assert best in (128, 256, 384, 512, 768, 1024) and ms > 0
+        assert rel(c.spmv(g["spmv_v"]), g["spmv_y"]) <= 1e-13
+        x, st = c.solve(g["b"], tol=float(g["tol"]))
+        assert abs(st["iters"] - int(g["pcg_iters"])) <= 2
+        assert rel(x, g["x"]) <= 1e-10
