@@ -217,7 +217,7 @@ def test_partially_restrained_nodes_use_scalar_tiles():
 def test_autotune_keeps_results(golden, name):
     """The autotuner rebuilds the layout with the fastest tile size; the
     product and the solve stay within the same bars."""
-    from elastic_amd.solver import Context
+    from elastic_amd.solver import MIXED, Context
 
     g = golden(name)
     with Context(0) as c:
