@@ -45,6 +45,10 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="cells per cube edge")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving", type=int, default=512,
+                    help="requests of the plane-strain serving stream (0: skip)")
+    ap.add_argument("--serving-compare", type=int, default=48,
+                    help="requests in the solve-only batched vs sequential comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve path is not sharded across GPUs; "
@@ -115,6 +119,12 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(sysm, st64["iters"], args.cpu_seconds)
 
+    serving = None
+    if args.serving > 0:
+        ctx.close()
+        del sysm, ctx
+        serving = serving_bench(args.serving, args.serving_compare)
+
     value = neq * args.steps / t_total
     out = {
         "metric": "solved DOF/s + p50/p99 solve latency at 1e-8 rel err; CSRC SpMV GB/s vs HBM",
@@ -151,8 +161,79 @@ def main():
                                else "k_spmv_tiles<float,double>"},
         "copy_probe": copy,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def serving_bench(nreq: int, ncompare: int):
+    """The plane-strain request stream (half quad4, half tria3, side U{32..512},
+    Poisson arrivals at 200 requests/s) served by the batched mixed-precision
+    PCG with continuous admission: throughput and p50/p99 latency from arrival
+    to x on the host (host set-up and GPU assembly included).  Then the
+    solve-only comparison on the first `ncompare` requests with every system
+    resident: batched mixed-precision PCG against plain FP64 PCG solved one
+    after the other (the drop-in path), both to the same tolerance."""
+    from elastic_amd.serving import Batch, Server, request_stream
+    from elastic_amd.solver import FP64, MIXED
+    from elastic_amd.system import System
+
+    workers = min(16, os.cpu_count() or 1)
+    srv = Server(max_slots=64, iters_per_step=16, prep_workers=workers, precision=MIXED,
+                 tol=TOL, hbm_budget=64e9)
+    a = time.perf_counter()
+    res = srv.run(request_stream(nreq, rate=200.0), realtime=True)
+    print("serving stream: %d requests in %.1f s" % (res["requests"], time.perf_counter() - a),
+          file=sys.stderr, flush=True)
+    stream = {k: v for k, v in res.items() if k != "done"}
+    stream["workload"] = ("%d plane-strain requests (even quad4, odd tria3, side U{32..512}), "
+                          "Poisson arrivals 200/s, mixed precision to %g" % (nreq, TOL))
+
+    # solve-only comparison on resident systems
+    from elastic_amd.serving import MeshRequest
+
+    systems = []
+    for k in range(ncompare):
+        s_ = System(MeshRequest(k)())
+        s_.ctx.upload_rhs(s_.b)
+        systems.append(s_)
+    dof = float(sum(s_.neq for s_ in systems))
+    torch_sync()
+    a = time.perf_counter()
+    it64 = 0
+    for s_ in systems:
+        st = s_.ctx.solve_resident(precision=FP64, tol=TOL, maxit=50000)
+        it64 += st["iters"]
+    t_seq = time.perf_counter() - a
+    x64 = [s_.ctx.download_x() for s_ in systems]
+    a = time.perf_counter()
+    done, itb = {}, 0
+    with Batch(0, max_slots=64, iters_per_step=16) as b:
+        queue = list(range(len(systems)))
+        while len(done) < len(systems):
+            while queue and b.submit(systems[queue[0]].ctx, MIXED, TOL, 50000, queue[0]):
+                queue.pop(0)
+            b.step()
+            for tag, st in b.poll():
+                done[tag] = st
+                itb += st["iters"]
+    t_bat = time.perf_counter() - a
+    err = max(float(np.linalg.norm(s_.ctx.download_x() - x64[i]) / np.linalg.norm(x64[i]))
+              for i, s_ in enumerate(systems))
+    for s_ in systems:
+        s_.ctx.close()
+    compare = {"requests": ncompare, "dof": dof,
+               "sequential_fp64": {"s": t_seq, "dof_per_s": dof / t_seq, "iters": it64},
+               "batched_mixed": {"s": t_bat, "dof_per_s": dof / t_bat, "iters": itb},
+               "speedup": t_seq / t_bat,
+               "max_rel_err_vs_fp64": err}
+    return {"stream": stream, "solve_only": compare}
+
+
+def torch_sync():
+    import torch
+
+    torch.cuda.synchronize()
 
 
 def measured_traffic(neq: int, nad: int):

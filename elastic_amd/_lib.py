@@ -15,10 +15,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libelastic_hip.so")
 HEADER = os.path.join(HERE, "..", "include", "elastic_hip.h")
 
-OK, ZERO_DIAG, MAXIT, NEG_JACOBIAN, OOM, HIP_ERROR, BAD_ARG, NOT_READY, BREAKDOWN = range(9)
+OK, ZERO_DIAG, MAXIT, NEG_JACOBIAN, OOM, HIP_ERROR, BAD_ARG, NOT_READY, BREAKDOWN, BUSY = range(10)
 STATUS = {0: "ok", 1: "zero diagonal", 2: "maxit reached", 3: "non-positive Jacobian",
           4: "out of device memory", 5: "HIP error", 6: "bad argument", 7: "not ready",
-          8: "Krylov breakdown"}
+          8: "Krylov breakdown", 9: "busy"}
 PCG, GMRES, BICGSTAB = 1, 2, 4
 DIAG, BDIAG3 = 2, 6
 FP64, MIXED = 64, 32
@@ -79,6 +79,7 @@ def lib():
     L.elastic_layout_info.argtypes = [vp, I64P]
     L.elastic_set_tile_nnz.argtypes = [vp, C.c_int32]
     L.elastic_set_tile_blocks.argtypes = [vp, C.c_int32]
+    L.elastic_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
     L.elastic_autotune.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_double)]
     L.elastic_set_mesh.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,

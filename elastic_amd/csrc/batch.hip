@@ -525,7 +525,7 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
       k = i;
       break;
     }
-  if (k < 0) return berr(b, ELASTIC_NOT_READY, "no free slot");
+  if (k < 0) return berr(b, ELASTIC_BUSY, "no free slot");
   hipSetDevice(b->device);
   // the context's own work (uploads, assembly) must be complete
   BATCH_HIP(b, hipStreamSynchronize(c->stream));

@@ -205,6 +205,8 @@ int elastic_create(elastic_ctx** out, int32_t device) {
     delete x;
     return ELASTIC_HIP_ERROR;
   }
+  if (const char* ac = getenv("ELASTIC_ACCUM")) c->accum32 = atoi(ac) == 32;
+  if (const char* pg = getenv("ELASTIC_PIPE_GRID")) c->pipe_grid64 = c->pipe_grid32 = atoi(pg);
   if (const char* tb = getenv("ELASTIC_TILE_BLOCKS"))
     c->tile_blocks = std::max(32, std::min(kBlkTileMax, atoi(tb)));
   const char* ce = getenv("ELASTIC_CHECK_EVERY");
@@ -351,6 +353,46 @@ int elastic_autotune(elastic_ctx* x, int32_t precision, int32_t reps, int32_t* b
   if (rc) return rc;
   if (best) *best = arg;
   if (best_ms) *best_ms = tmin;
+  return ELASTIC_OK;
+}
+
+// Tuning knobs of a context (no counterpart in the reference).
+int elastic_set_option(elastic_ctx* x, const char* name, int64_t value) {
+  if (!x || !name) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  const std::string n(name);
+  if (n == "pipe_grid") {
+    if (value < 0 || value > (1 << 20)) return ELASTIC_BAD_ARG;
+    c->pipe_grid64 = c->pipe_grid32 = (int)value;
+  } else if (n == "pipe_grid64") {
+    if (value < 0 || value > (1 << 20)) return ELASTIC_BAD_ARG;
+    c->pipe_grid64 = (int)value;
+  } else if (n == "pipe_grid32") {
+    if (value < 0 || value > (1 << 20)) return ELASTIC_BAD_ARG;
+    c->pipe_grid32 = (int)value;
+  } else if (n == "accum32") {
+    c->accum32 = value != 0;
+  } else if (n == "check_every") {
+    if (value < 1 || value > 4096) return ELASTIC_BAD_ARG;
+    c->check_every = (int)value;
+  } else if (n == "tile_blocks" || n == "tile_nnz") {
+    const bool blocks = n == "tile_blocks";
+    if (blocks && (value < 32 || value > kBlkTileMax)) return ELASTIC_BAD_ARG;
+    if (!blocks && (value < 64 || value > kTileColsMax - kTileRowsMax)) return ELASTIC_BAD_ARG;
+    if (blocks) c->tile_blocks = (int)value;
+    else c->tile_nnz = (int)value;
+    if (c->neq > 0 && (c->tiles.ntiles || c->blk.ntiles)) {  // rebuild the layout now
+      hipSetDevice(c->device);
+      ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+      free_layout(c);
+      int rc = build_layout(x);
+      if (rc) return rc;
+      if (c->values_set) derive_operator(c);
+      ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    }
+  } else {
+    return set_error(c, ELASTIC_BAD_ARG, "unknown option " + n);
+  }
   return ELASTIC_OK;
 }
 

@@ -10,7 +10,7 @@
 // segmented scan on the row key, transposed products land in LDS at their
 // column-sorted position and each local column node is summed by one lane in a
 // fixed order: deterministic, no atomics.  Values are TV (double, or float for
-// the mixed-precision operator); x, y and all sums are T = double.
+// the mixed-precision operator); staged x and sums are T.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -67,13 +67,59 @@ struct Planes<float, NB> {
   }
 };
 
-// One tile of the product (a workgroup of 256 lanes); smem holds
-// 8 * NB * (ncols_max + 2 * kBlkRowsMax + nblk_max) bytes of the layout.
-template <typename TV, typename T, int NB, int PER>
-__device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
-                                           const TV* __restrict__ diag, const T* __restrict__ x,
-                                           T* __restrict__ y, T* __restrict__ epart,
-                                           unsigned char* smem) {
+// Registers of one tile: the words and values of the lane's PER blocks and,
+// for lanes below the tile's row count, the diagonal block of that row node.
+template <typename TV, int NB, int PER>
+struct TileRegs {
+  uint32_t wr[PER];
+  TV vr[PER][NB * NB];
+  TV dg[NB * (NB + 1) / 2];
+};
+
+// Issues the loads of tile t (values nontemporal: read once per product).
+template <typename TV, int NB, int PER>
+__device__ __forceinline__ void load_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
+                                          const TV* __restrict__ diag, TileRegs<TV, NB, PER>& R) {
+  constexpr int NB2 = NB * NB;
+  const int tid = threadIdx.x;
+  const int nbk = a.nblk[t], nr = a.nrows[t];
+  const int64_t base = a.blk0[t];
+  const TV* v = vals + base * NB2;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int q = tid + 256 * i;
+    if (q < nbk) {
+      R.wr[i] = __builtin_nontemporal_load(a.word + base + q);
+      Planes<TV, NB>::load(v, nbk, q, R.vr[i]);
+    } else {
+      R.wr[i] = 0xFFFFFFFFu;
+#pragma unroll
+      for (int p = 0; p < NB2; ++p) R.vr[i][p] = TV(0);
+    }
+  }
+  if (tid < nr) {
+    const int g = a.row0[t] + tid;
+#pragma unroll
+    for (int k = 0; k < NB * (NB + 1) / 2; ++k) R.dg[k] = diag[(int64_t)k * a.nn + g];
+  }
+}
+
+// The product of tile t from its registers (a workgroup of 256 lanes); smem
+// holds sizeof(T) * NB * (ncols_max + 2 * kBlkRowsMax + nblk_max) bytes.  T is
+// the type of the staged x, the products and the sums (double, or float for
+// the FP32 operator with FP32 accumulation); x, y and the external partials
+// are FP64 in memory either way.  Starts with a barrier, so tiles can follow
+// each other in one workgroup.
+struct NoHook {
+  __device__ void operator()() const {}
+};
+
+template <typename TV, typename T, int NB, int PER, typename Hook = NoHook>
+__device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
+                                             const TileRegs<TV, NB, PER>& R,
+                                             const double* __restrict__ x, double* __restrict__ y,
+                                             double* __restrict__ epart, unsigned char* smem,
+                                             Hook hook = Hook()) {
   constexpr int NB2 = NB * NB;
   T* xs = reinterpret_cast<T*>(smem);       // ncols_max * NB
   T* pp = xs + a.ncols_max * NB;            // 2 * kBlkRowsMax * NB (row pieces)
@@ -81,27 +127,35 @@ __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __
   const int tid = threadIdx.x, lane = tid & 63;
   const int I0 = a.row0[t], nr = a.nrows[t], nbk = a.nblk[t], ncols = a.ncols[t];
   const int ext0 = a.ext0[t];
-  const int64_t base = a.blk0[t];
   const int32_t* ext = a.ext + ext0;
-  const TV* v = vals + base * NB2;
-  uint32_t wr[PER];
-  TV vr[PER][NB2];
+  const uint32_t* wr = R.wr;
+  const auto& vr = R.vr;
+  // x of the tile's local columns: the first 8 * 256 entries are loaded into
+  // registers, then `hook` runs (it may issue loads of a next tile, which then
+  // stay in flight behind these), then they go to LDS
+  constexpr int XR = 8;
+  const int nx = ncols * NB;
+  double xv[XR];
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int q = tid + 256 * i;
-    if (q < nbk) {
-      wr[i] = __builtin_nontemporal_load(a.word + base + q);
-      Planes<TV, NB>::load(v, nbk, q, vr[i]);
-    } else {
-      wr[i] = 0xFFFFFFFFu;
-#pragma unroll
-      for (int p = 0; p < NB2; ++p) vr[i][p] = TV(0);
+  for (int k = 0; k < XR; ++k) {
+    const int c = tid + 256 * k;
+    if (c < nx) {
+      const int node = c / NB, comp = c - node * NB;
+      const int g = node < nr ? I0 + node : ext[node - nr];
+      xv[k] = x[(int64_t)NB * g + comp];
     }
   }
-  for (int c = tid; c < ncols * NB; c += 256) {
+  hook();
+  __syncthreads();  // the LDS of a previous tile is free
+#pragma unroll
+  for (int k = 0; k < XR; ++k) {
+    const int c = tid + 256 * k;
+    if (c < nx) xs[c] = (T)xv[k];
+  }
+  for (int c = tid + 256 * XR; c < nx; c += 256) {
     const int node = c / NB, comp = c - node * NB;
     const int g = node < nr ? I0 + node : ext[node - nr];
-    xs[c] = x[(int64_t)NB * g + comp];
+    xs[c] = (T)x[(int64_t)NB * g + comp];
   }
   __syncthreads();
   const uint16_t* rp = a.rowptr + a.rp0[t];
@@ -163,10 +217,10 @@ __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __
     for (int r = 0; r < NB; ++r) s[r] = T(0);
     if (c < nr) {
       // node diagonal block (nb diagonal entries, then a21, a31, a32)
-      const int g = I0 + c;
+      // c < nr <= 127 < 256, so c == tid: the lane loaded this node's diagonal
       T d[NB * (NB + 1) / 2];
 #pragma unroll
-      for (int k = 0; k < NB * (NB + 1) / 2; ++k) d[k] = (T)diag[(int64_t)k * a.nn + g];
+      for (int k = 0; k < NB * (NB + 1) / 2; ++k) d[k] = (T)R.dg[k];
       const T* xi = xs + c * NB;
       if constexpr (NB == 3) {
         s[0] = d[0] * xi[0] + d[3] * xi[1] + d[4] * xi[2];
@@ -190,12 +244,23 @@ __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __
       for (int r = 0; r < NB; ++r) s[r] += tr[q * NB + r];
     if (c < nr) {
 #pragma unroll
-      for (int r = 0; r < NB; ++r) y[(int64_t)NB * (I0 + c) + r] = s[r];
+      for (int r = 0; r < NB; ++r) y[(int64_t)NB * (I0 + c) + r] = (double)s[r];
     } else {
 #pragma unroll
-      for (int r = 0; r < NB; ++r) epart[(int64_t)NB * (ext0 + c - nr) + r] = s[r];
+      for (int r = 0; r < NB; ++r) epart[(int64_t)NB * (ext0 + c - nr) + r] = (double)s[r];
     }
   }
+}
+
+// One tile: its loads, then its product.
+template <typename TV, typename T, int NB, int PER>
+__device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
+                                           const TV* __restrict__ diag,
+                                           const double* __restrict__ x, double* __restrict__ y,
+                                           double* __restrict__ epart, unsigned char* smem) {
+  TileRegs<TV, NB, PER> R;
+  load_tile<TV, NB, PER>(a, t, vals, diag, R);
+  compute_tile<TV, T, NB, PER>(a, t, R, x, y, epart, smem);
 }
 
 }  // namespace elastic

@@ -12,7 +12,7 @@ module elastic_hip
   integer(c_int32_t), parameter :: ELASTIC_OK = 0, ELASTIC_ZERO_DIAG = 1, &
        ELASTIC_MAXIT = 2, ELASTIC_NEG_JACOBIAN = 3, ELASTIC_OOM = 4, &
        ELASTIC_HIP_ERROR = 5, ELASTIC_BAD_ARG = 6, ELASTIC_NOT_READY = 7, &
-       ELASTIC_BREAKDOWN = 8
+       ELASTIC_BREAKDOWN = 8, ELASTIC_BUSY = 9
   integer(c_int32_t), parameter :: ELASTIC_SOLVER_PCG = 1, ELASTIC_SOLVER_GMRES = 2, &
        ELASTIC_SOLVER_BICGSTAB = 4
   integer(c_int32_t), parameter :: ELASTIC_PRECOND_NONE = 1, ELASTIC_PRECOND_DIAG = 2, &

@@ -52,7 +52,7 @@ class Batch:
                tag: int = 0) -> bool:
         """Admits a prepared context; False when every slot is taken."""
         rc = self._L.elastic_batch_submit(self._h, ctx._h, precision, tol, maxit, tag)
-        return self._chk(rc, allow=(_lib.NOT_READY,)) == _lib.OK
+        return self._chk(rc, allow=(_lib.BUSY,)) == _lib.OK
 
     def step(self):
         self._chk(self._L.elastic_batch_step(self._h))

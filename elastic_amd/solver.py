@@ -75,6 +75,10 @@ class Context:
         return dict(ntiles=int(info[0]), nnz_max=int(info[1]), ext_partials=int(info[2]),
                     ncols_max=int(info[3]), nb=int(info[3]))
 
+    def set_option(self, name: str, value: int):
+        """Tuning knob of the context (see elastic_set_option)."""
+        self._chk(self._L.elastic_set_option(self._h, name.encode(), int(value)))
+
     def set_tile_blocks(self, tile_blocks: int):
         self._chk(self._L.elastic_set_tile_blocks(self._h, tile_blocks))
 

@@ -32,6 +32,7 @@ extern "C" {
 #define ELASTIC_BAD_ARG 6
 #define ELASTIC_NOT_READY 7      /* values or structure not set */
 #define ELASTIC_BREAKDOWN 8      /* zero denominator inside a Krylov method */
+#define ELASTIC_BUSY 9           /* every slot of a batch is taken (try again later) */
 
 /* solver codes (Main.f:193, Solv.f:467-498) */
 #define ELASTIC_SOLVER_PCG 1
@@ -173,6 +174,13 @@ int elastic_set_tile_nnz(elastic_ctx* ctx, int32_t tile_nnz);
 /* Tile size of the node-blocked layout (node blocks per tile, 32..1024);
  * takes effect at the next elastic_set_structure. */
 int elastic_set_tile_blocks(elastic_ctx* ctx, int32_t tile_blocks);
+/* Tuning knobs (no counterpart in the reference): "tile_blocks" /
+ * "tile_nnz" (tile size; rebuilds the layout at once when a structure is
+ * set), "pipe_grid" / "pipe_grid64" / "pipe_grid32" (> 0: persistent
+ * pipelined product with that many workgroups; 0: one workgroup per tile),
+ * "accum32" (FP32 operator with FP32 staging and sums), "check_every"
+ * (iterations between convergence polls). */
+int elastic_set_option(elastic_ctx* ctx, const char* name, int64_t value);
 /* Autotuner (no counterpart in the reference): with values resident, times
  * the product of the given precision for each candidate tile size (HIP
  * events, reps launches each), rebuilds the layout with the fastest and
@@ -197,7 +205,7 @@ int elastic_batch_create(elastic_batch** b, int32_t device, int32_t max_slots,
 int elastic_batch_destroy(elastic_batch* b);
 const char* elastic_batch_last_error(const elastic_batch* b);
 /* Admits ctx as a new instance (x0 = 0, stop when ||r||_M < tol ||b||_M or
- * at maxit); ELASTIC_NOT_READY when every slot is taken.  The context must
+ * at maxit); ELASTIC_BUSY when every slot is taken.  The context must
  * not be used by the caller until the instance is retired by a poll. */
 int elastic_batch_submit(elastic_batch* b, elastic_ctx* ctx, int32_t precision, double tol,
                          int32_t maxit, int64_t tag);
