@@ -87,6 +87,14 @@ def lib():
                                    I32P, F64P]
     L.elastic_assemble.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, vp]
     L.elastic_time_assembly.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.elastic_j2_setup.argtypes = [vp, C.c_double, C.c_double, vp]
+    L.elastic_j2_increment.argtypes = [vp, C.c_double]
+    L.elastic_j2_assemble.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.elastic_j2_correct.argtypes = [vp]
+    L.elastic_j2_commit.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.elastic_j2_get_u.argtypes = [vp, F64P]
+    L.elastic_j2_get_residual.argtypes = [vp, F64P]
+    L.elastic_j2_get_state.argtypes = [vp, C.c_int32, vp, vp]
     L.elastic_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, C.c_int32, C.c_int32]
     L.elastic_batch_destroy.argtypes = [vp]
     L.elastic_batch_last_error.argtypes = [vp]

@@ -802,3 +802,410 @@ extern "C" int elastic_time_assembly(elastic_ctx* xc, int32_t reps, double* ms) 
   *ms = t / reps;
   return ELASTIC_OK;
 }
+
+// ======================================================= J2 plasticity (hex8)
+// Small-strain J2 plasticity with linear isotropic hardening (current yield
+// stress sy + H * alpha, alpha the equivalent plastic strain, the hardening
+// law of Plasticity01.f:322, 389), integrated by the closed-form radial
+// return with its consistent tangent (Simo & Hughes 1998, Box 3.2).  The
+// reference reaches no plasticity from its driver, so there is no reference
+// result to match; Voigt order xx, yy, zz, xy, yz, zx, engineering shears.
+// One 64-lane wavefront per element of a colour: lanes 0..7 integrate the 8
+// Gauss points (same points, shape functions and Jacobian as elmt07_mec),
+// then all lanes form K_t = sum B^T C B det and f = sum B^T sigma det, and the
+// element is added to ad/al and b (b -= f) through the slot map.
+namespace elastic {
+
+struct J2Dev {
+  const double *ep_n, *al_n;  // committed state: 6 + 1 per Gauss point
+  double *ep_t, *al_t;        // state of the last assembly
+  const double* u;            // nodal displacements ndf*nnode
+  double sy, H;
+};
+
+__device__ inline void j2_point(const double* eps, const double* epn, double aln, double E,
+                                double nu, double sy, double H, double* sig, double* C,
+                                double* ep, double* al) {
+  const double mu = E / (2.0 * (1.0 + nu));
+  const double kap = E / (3.0 * (1.0 - 2.0 * nu));
+  const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  double ee[6];
+  for (int k = 0; k < 6; ++k) ee[k] = eps[k] - epn[k];
+  const double tr = ee[0] + ee[1] + ee[2];
+  double st[6];
+  for (int k = 0; k < 3; ++k) st[k] = lam * tr + 2.0 * mu * ee[k];
+  for (int k = 3; k < 6; ++k) st[k] = mu * ee[k];
+  const double p = (st[0] + st[1] + st[2]) / 3.0;
+  double s[6] = {st[0] - p, st[1] - p, st[2] - p, st[3], st[4], st[5]};
+  const double sn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] +
+                         2.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+  const double sq23 = sqrt(2.0 / 3.0);
+  const double f = sn - sq23 * (sy + H * aln);
+  const double w3[6] = {1.0, 1.0, 1.0, 0.5, 0.5, 0.5};
+  if (f <= 0.0) {
+    for (int k = 0; k < 6; ++k) sig[k] = st[k], ep[k] = epn[k];
+    *al = aln;
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c)
+        C[r * 6 + c] = (r < 3 && c < 3 ? lam : 0.0) + (r == c ? 2.0 * mu * w3[r] : 0.0);
+    return;
+  }
+  const double dg = f / (2.0 * mu + 2.0 * H / 3.0);
+  double n[6];
+  for (int k = 0; k < 6; ++k) n[k] = s[k] / sn;
+  for (int k = 0; k < 6; ++k) sig[k] = st[k] - 2.0 * mu * dg * n[k];
+  for (int k = 0; k < 3; ++k) ep[k] = epn[k] + dg * n[k];
+  for (int k = 3; k < 6; ++k) ep[k] = epn[k] + dg * 2.0 * n[k];
+  *al = aln + sq23 * dg;
+  const double th = 1.0 - 2.0 * mu * dg / sn;
+  const double thb = 1.0 / (1.0 + H / (3.0 * mu)) - (1.0 - th);
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) {
+      const double pd = (r == c ? w3[r] : 0.0) - (r < 3 && c < 3 ? 1.0 / 3.0 : 0.0);
+      C[r * 6 + c] = (r < 3 && c < 3 ? kap : 0.0) + 2.0 * mu * th * pd -
+                     2.0 * mu * thb * n[r] * n[c];
+    }
+}
+
+// B column of dof r of a node: rows and the derivative each row takes
+__device__ inline void bcol(int r, int* row, int* der) {
+  if (r == 0) row[0] = 0, der[0] = 0, row[1] = 3, der[1] = 1, row[2] = 5, der[2] = 2;
+  else if (r == 1) row[0] = 1, der[0] = 1, row[1] = 3, der[1] = 0, row[2] = 4, der[2] = 2;
+  else row[0] = 2, der[0] = 2, row[1] = 4, der[1] = 1, row[2] = 5, der[2] = 0;
+}
+
+__global__ __launch_bounds__(64) void k_j2_hex8(AsmArgs a, J2Dev j, int c0, int lhs) {
+  constexpr int NEN = 8, NST = 24, NPAIR = NST * (NST + 1) / 2, NENT = NST * NST;
+  __shared__ double xl[NEN * 3], ul[NST], dX[8][3][NEN], Cg[8][36], Sg[8][6], s[NENT];
+  __shared__ int32_t ldv[NST];
+  __shared__ int bad;
+  const int el = a.order[c0 + blockIdx.x];
+  const int tid = threadIdx.x;
+  const int32_t* ixe = a.ix + (size_t)el * (NEN + 1);
+  if (tid == 0) bad = 0;
+  if (tid < NEN * 3) xl[tid] = a.x[(size_t)(ixe[tid / 3] - 1) * 3 + tid % 3];
+  if (tid < NST) {
+    const size_t g = (size_t)(ixe[tid / 3] - 1) * 3 + tid % 3;
+    ldv[tid] = a.id[g];
+    ul[tid] = j.u[g];
+  }
+  const double* em = a.e + (size_t)(ixe[NEN] - 1) * 15;
+  const double E = em[0], nu = em[1];
+  __syncthreads();
+  if (tid < 8) {
+    const int gp = tid, lx = gp & 1, ly = (gp >> 1) & 1, lz = gp >> 2;
+    double h[NEN], hx[NEN], hy[NEN], hz[NEN], det;
+    sfhexa8_d(kPG2[lx], kPG2[ly], kPG2[lz], h, hx, hy, hz);
+    if (!jacob3d_d(hx, hy, hz, xl, NEN, &det)) bad = 1;
+    double eps[6] = {0, 0, 0, 0, 0, 0};
+    for (int n = 0; n < NEN; ++n) {
+      dX[gp][0][n] = hx[n];
+      dX[gp][1][n] = hy[n];
+      dX[gp][2][n] = hz[n];
+      const double ux = ul[3 * n], uy = ul[3 * n + 1], uz = ul[3 * n + 2];
+      eps[0] += hx[n] * ux;
+      eps[1] += hy[n] * uy;
+      eps[2] += hz[n] * uz;
+      eps[3] += hy[n] * ux + hx[n] * uy;
+      eps[4] += hz[n] * uy + hy[n] * uz;
+      eps[5] += hz[n] * ux + hx[n] * uz;
+    }
+    const size_t q = (size_t)el * 8 + gp;
+    double epn[6], sig[6], C[36], ep[6], al;
+    for (int k = 0; k < 6; ++k) epn[k] = j.ep_n[q * 6 + k];
+    j2_point(eps, epn, j.al_n[q], E, nu, j.sy, j.H, sig, C, ep, &al);
+    for (int k = 0; k < 6; ++k) Sg[gp][k] = sig[k] * det;
+    for (int k = 0; k < 36; ++k) Cg[gp][k] = C[k] * det;
+    for (int k = 0; k < 6; ++k) j.ep_t[q * 6 + k] = ep[k];
+    j.al_t[q] = al;
+  }
+  __syncthreads();
+  if (bad) {
+    if (tid == 0) a.err[0] = el + 1;
+    return;
+  }
+  // internal force, b -= f
+  if (tid < NST) {
+    const int n = tid / 3, r = tid % 3;
+    int row[3], der[3];
+    bcol(r, row, der);
+    double f = 0.0;
+    for (int gp = 0; gp < 8; ++gp)
+      for (int k = 0; k < 3; ++k) f += dX[gp][der[k]][n] * Sg[gp][row[k]];
+    const int k = ldv[tid];
+    if (k > 0 && k <= a.neq) a.b[k - 1] = a.b[k - 1] - f;
+  }
+  if (!lhs) return;
+  for (int en = tid; en < NENT; en += 64) {
+    const int i = en % NST, jj = en / NST;
+    const int na = i / 3, r = i % 3, nb = jj / 3, c = jj % 3;
+    int ri[3], di[3], rj[3], dj[3];
+    bcol(r, ri, di);
+    bcol(c, rj, dj);
+    double acc = 0.0;
+    for (int gp = 0; gp < 8; ++gp)
+      for (int p = 0; p < 3; ++p) {
+        const double bi = dX[gp][di[p]][na];
+        double t = 0.0;
+        for (int q2 = 0; q2 < 3; ++q2) t += Cg[gp][ri[p] * 6 + rj[q2]] * dX[gp][dj[q2]][nb];
+        acc += bi * t;
+      }
+    s[en] = acc;  // s(i,j) at i + NST*j
+  }
+  __syncthreads();
+  const int32_t* sl = a.slots + (size_t)el * NPAIR;
+  for (int pp = tid; pp < NPAIR; pp += 64) {
+    const int32_t t = sl[pp];
+    if (t == -1) continue;
+    int i = (int)((sqrtf(8.0f * pp + 1.0f) - 1.0f) * 0.5f);
+    while ((i + 1) * (i + 2) / 2 <= pp) ++i;
+    while (i * (i + 1) / 2 > pp) --i;
+    const int jx = pp - i * (i + 1) / 2;
+    if (t <= -2) {
+      const int k = -t - 2;
+      a.ad[k] = a.ad[k] + s[i * NST + i];
+    } else {
+      const bool rev = (t >> 30) & 1;
+      const int64_t slot = t & ((1 << 30) - 1);
+      a.al[slot] = a.al[slot] + (rev ? s[i * NST + jx] : s[jx * NST + i]);
+    }
+  }
+}
+
+// u(dof) = lambda * ubar(dof) on restrained dofs (id == 0)
+__global__ void k_j2_prescribe(int n, const int32_t* __restrict__ id, const double* __restrict__ ubar,
+                               double lambda, double* u) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (id[i] == 0) u[i] = lambda * ubar[i];
+}
+
+// u(dof) += x(id(dof)) on free dofs
+__global__ void k_j2_correct(int n, const int32_t* __restrict__ id, const double* __restrict__ x,
+                             double* u) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (id[i] > 0) u[i] = u[i] + x[id[i] - 1];
+}
+
+// per-block partials: sum of b^2 (mode 0); max alpha and count of alpha > 0 (mode 1)
+__global__ __launch_bounds__(256) void k_j2_reduce(int64_t n, const double* __restrict__ v, int mode,
+                                                   double* part) {
+  __shared__ double sh[256], sc[256];
+  double acc = 0.0, cnt = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double x = v[i];
+    if (mode == 0) acc += x * x;
+    else {
+      acc = fmax(acc, x);
+      cnt += x > 0.0 ? 1.0 : 0.0;
+    }
+  }
+  sh[threadIdx.x] = acc;
+  sc[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sh[threadIdx.x] = mode == 0 ? sh[threadIdx.x] + sh[threadIdx.x + o]
+                                  : fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+      sc[threadIdx.x] += sc[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0];
+    part[2 * blockIdx.x + 1] = sc[0];
+  }
+}
+
+struct J2State {
+  double *ep_n = nullptr, *al_n = nullptr, *ep_t = nullptr, *al_t = nullptr, *u = nullptr,
+         *ubar = nullptr, *part = nullptr;
+  double sy = 0, H = 0;
+  int64_t ngp = 0;
+};
+
+void free_j2(void* p) {
+  J2State* j = (J2State*)p;
+  if (!j) return;
+  for (void* q : {(void*)j->ep_n, (void*)j->al_n, (void*)j->ep_t, (void*)j->al_t, (void*)j->u,
+                  (void*)j->ubar, (void*)j->part})
+    if (q) hipFree(q);
+  delete j;
+}
+
+constexpr int kJ2RedBlocks = 512;
+
+static int j2_reduce(Context* c, J2State* j, const double* v, int64_t n, int mode, double* r0,
+                     double* r1) {
+  hipLaunchKernelGGL(k_j2_reduce, dim3(kJ2RedBlocks), dim3(256), 0, c->stream, n, v, mode, j->part);
+  std::vector<double> h(2 * kJ2RedBlocks);
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(h.data(), j->part, sizeof(double) * h.size(),
+                                      hipMemcpyDeviceToHost, c->stream));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  double a = 0.0, b = 0.0;
+  for (int k = 0; k < kJ2RedBlocks; ++k) {
+    a = mode == 0 ? a + h[2 * k] : std::max(a, h[2 * k]);
+    b += h[2 * k + 1];
+  }
+  *r0 = a;
+  if (r1) *r1 = b;
+  return ELASTIC_OK;
+}
+
+}  // namespace elastic
+
+extern "C" int elastic_j2_setup(elastic_ctx* xc, double sy, double H, const double* ubar) {
+  if (!xc || sy <= 0.0 || H < 0.0) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  MeshDev* m = (MeshDev*)c->mesh;
+  if (!m) return set_error(c, ELASTIC_NOT_READY, "mesh not set");
+  if (m->kind != K_HEX8) return set_error(c, ELASTIC_BAD_ARG, "J2 plasticity is built for hex8");
+  hipSetDevice(c->device);
+  if (c->j2) free_j2(c->j2);
+  J2State* j = new J2State();
+  c->j2 = j;
+  j->sy = sy;
+  j->H = H;
+  j->ngp = (int64_t)m->numel * 8;
+  const size_t nu = (size_t)m->nnode * 3;
+  if (hipMalloc((void**)&j->ep_n, sizeof(double) * 6 * j->ngp) != hipSuccess ||
+      hipMalloc((void**)&j->ep_t, sizeof(double) * 6 * j->ngp) != hipSuccess ||
+      hipMalloc((void**)&j->al_n, sizeof(double) * j->ngp) != hipSuccess ||
+      hipMalloc((void**)&j->al_t, sizeof(double) * j->ngp) != hipSuccess ||
+      hipMalloc((void**)&j->u, sizeof(double) * nu) != hipSuccess ||
+      hipMalloc((void**)&j->ubar, sizeof(double) * nu) != hipSuccess ||
+      hipMalloc((void**)&j->part, sizeof(double) * 2 * kJ2RedBlocks) != hipSuccess) {
+    (void)hipGetLastError();
+    free_j2(j);
+    c->j2 = nullptr;
+    return set_error(c, ELASTIC_OOM, "J2 state");
+  }
+  hipStream_t s = c->stream;
+  ELASTIC_HIP_CHECK(c, hipMemsetAsync(j->ep_n, 0, sizeof(double) * 6 * j->ngp, s));
+  ELASTIC_HIP_CHECK(c, hipMemsetAsync(j->al_n, 0, sizeof(double) * j->ngp, s));
+  ELASTIC_HIP_CHECK(c, hipMemsetAsync(j->u, 0, sizeof(double) * nu, s));
+  if (ubar) ELASTIC_HIP_CHECK(c, hipMemcpyAsync(j->ubar, ubar, sizeof(double) * nu, hipMemcpyHostToDevice, s));
+  else ELASTIC_HIP_CHECK(c, hipMemsetAsync(j->ubar, 0, sizeof(double) * nu, s));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
+  return ELASTIC_OK;
+}
+
+extern "C" int elastic_j2_increment(elastic_ctx* xc, double lambda) {
+  if (!xc) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  MeshDev* m = (MeshDev*)c->mesh;
+  J2State* j = (J2State*)c->j2;
+  if (!m || !j) return set_error(c, ELASTIC_NOT_READY, "J2 not set up");
+  hipSetDevice(c->device);
+  const int n = m->nnode * 3;
+  hipLaunchKernelGGL(k_j2_prescribe, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, m->id,
+                     (const double*)j->ubar, lambda, j->u);
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return ELASTIC_OK;
+}
+
+extern "C" int elastic_j2_assemble(elastic_ctx* xc, int32_t lhs, double* rnorm) {
+  if (!xc) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  MeshDev* m = (MeshDev*)c->mesh;
+  J2State* j = (J2State*)c->j2;
+  if (!m || !j) return set_error(c, ELASTIC_NOT_READY, "J2 not set up");
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  AsmArgs a = make_args(c, m);
+  a.b = c->v64[VB];  // the residual stays on the device, as the solve's b
+  J2Dev d{j->ep_n, j->al_n, j->ep_t, j->al_t, j->u, j->sy, j->H};
+  ELASTIC_HIP_CHECK(c, hipMemsetAsync(m->err, 0, sizeof(int32_t), s));
+  ELASTIC_HIP_CHECK(c, hipMemsetAsync(a.b, 0, sizeof(double) * c->neq, s));
+  if (lhs) {
+    ELASTIC_HIP_CHECK(c, hipMemsetAsync(c->A64.ad, 0, sizeof(double) * c->neq, s));
+    ELASTIC_HIP_CHECK(c, hipMemsetAsync(c->A64.al, 0, sizeof(double) * std::max<int64_t>(c->nad, 1), s));
+  }
+  for (int k = 0; k < m->ncolors; ++k) {
+    const int c0 = m->color_start[k], cnt = m->color_start[k + 1] - c0;
+    if (cnt > 0) hipLaunchKernelGGL(k_j2_hex8, dim3(cnt), dim3(64), 0, s, a, d, c0, lhs);
+  }
+  int32_t err = 0;
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(&err, m->err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
+  ELASTIC_HIP_CHECK(c, hipGetLastError());
+  if (err) return set_error(c, ELASTIC_NEG_JACOBIAN, "non-positive Jacobian in element " + std::to_string(err));
+  if (lhs) {
+    derive_operator(c);
+    c->values_set = true;
+    c->bdiag_set = false;
+    c->zero_diag_row = -1;
+  }
+  double r2 = 0.0;
+  int rc = j2_reduce(c, j, a.b, c->neq, 0, &r2, nullptr);
+  if (rc) return rc;
+  if (rnorm) *rnorm = std::sqrt(r2);
+  return ELASTIC_OK;
+}
+
+extern "C" int elastic_j2_correct(elastic_ctx* xc) {
+  if (!xc) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  MeshDev* m = (MeshDev*)c->mesh;
+  J2State* j = (J2State*)c->j2;
+  if (!m || !j) return set_error(c, ELASTIC_NOT_READY, "J2 not set up");
+  hipSetDevice(c->device);
+  const int n = m->nnode * 3;
+  hipLaunchKernelGGL(k_j2_correct, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, m->id,
+                     (const double*)c->v64[VX], j->u);
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return ELASTIC_OK;
+}
+
+extern "C" int elastic_j2_commit(elastic_ctx* xc, double* alpha_max, int64_t* nplastic) {
+  if (!xc) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  J2State* j = (J2State*)c->j2;
+  if (!j) return set_error(c, ELASTIC_NOT_READY, "J2 not set up");
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(j->ep_n, j->ep_t, sizeof(double) * 6 * j->ngp, hipMemcpyDeviceToDevice, s));
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(j->al_n, j->al_t, sizeof(double) * j->ngp, hipMemcpyDeviceToDevice, s));
+  double amax = 0.0, cnt = 0.0;
+  int rc = j2_reduce(c, j, j->al_n, j->ngp, 1, &amax, &cnt);
+  if (rc) return rc;
+  if (alpha_max) *alpha_max = amax;
+  if (nplastic) *nplastic = (int64_t)cnt;
+  return ELASTIC_OK;
+}
+
+extern "C" int elastic_j2_get_u(elastic_ctx* xc, double* u) {
+  if (!xc || !u) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  MeshDev* m = (MeshDev*)c->mesh;
+  J2State* j = (J2State*)c->j2;
+  if (!m || !j) return set_error(c, ELASTIC_NOT_READY, "J2 not set up");
+  hipSetDevice(c->device);
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(u, j->u, sizeof(double) * m->nnode * 3, hipMemcpyDeviceToHost,
+                                      c->stream));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return ELASTIC_OK;
+}
+
+extern "C" int elastic_j2_get_state(elastic_ctx* xc, int32_t trial, double* ep, double* alpha) {
+  if (!xc) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  J2State* j = (J2State*)c->j2;
+  if (!j) return set_error(c, ELASTIC_NOT_READY, "J2 not set up");
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  if (ep) ELASTIC_HIP_CHECK(c, hipMemcpyAsync(ep, trial ? j->ep_t : j->ep_n, sizeof(double) * 6 * j->ngp, hipMemcpyDeviceToHost, s));
+  if (alpha) ELASTIC_HIP_CHECK(c, hipMemcpyAsync(alpha, trial ? j->al_t : j->al_n, sizeof(double) * j->ngp, hipMemcpyDeviceToHost, s));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
+  return ELASTIC_OK;
+}
+
+extern "C" int elastic_j2_get_residual(elastic_ctx* xc, double* r) {
+  if (!xc || !r) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  if (!c->j2) return set_error(c, ELASTIC_NOT_READY, "J2 not set up");
+  hipSetDevice(c->device);
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(r, c->v64[VB], sizeof(double) * c->neq, hipMemcpyDeviceToHost,
+                                      c->stream));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return ELASTIC_OK;
+}

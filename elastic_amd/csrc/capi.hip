@@ -223,6 +223,8 @@ int elastic_destroy(elastic_ctx* x) {
   free_system(c);
   if (c->mesh) free_mesh((MeshDev*)c->mesh);
   c->mesh = nullptr;
+  if (c->j2) free_j2(c->j2);
+  c->j2 = nullptr;
   if (c->scal) hipFree(c->scal);
   if (c->scal_host) hipHostFree(c->scal_host);
   if (c->red64) hipFree(c->red64);

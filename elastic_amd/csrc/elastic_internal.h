@@ -161,6 +161,7 @@ struct Context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   // assembly state (optional)
   void* mesh = nullptr;  // MeshDev (assembly.hip)
+  void* j2 = nullptr;    // J2 plasticity state (assembly.hip)
   // GMRES(k) Krylov basis ((k+1) vectors of neq) and its small arrays
   double* gm_basis = nullptr;
   double* gm_small = nullptr;
@@ -197,6 +198,7 @@ int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit,
               KrylovResult* out);
 struct MeshDev;
 void free_mesh(MeshDev* m);
+void free_j2(void* j);
 #define ELASTIC_HIP_CHECK(ctx, expr)                                                   \
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \

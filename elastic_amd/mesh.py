@@ -197,6 +197,22 @@ def tet4_cube(N: int, jitter: float = 0.0, seed: int = 20261203, logE=(-1.0, 1.0
                 np.zeros((nnode, 3)), [(40, list(map(float, traction)))], name=f"tet4_{N}")
 
 
+def hex8_block_pull(N: int, stretch: float, E: float = 1.0, nu: float = 0.3):
+    """Unit cube of N^3 hex8 held between two rigid grips: z=0 clamped, z=1
+    restrained in all directions and moved by (0, 0, stretch) at load factor
+    1 (every node fully free or fully restrained, so the node-blocked
+    layout applies).  Returns (mesh, ubar(nnode, 3))."""
+    m = hex8_cube(N, E=E, nu=nu)
+    top = m.x[:, 2] == 1.0
+    m.id[top] = 1
+    m.eload[:] = 0
+    m.loads = []
+    ubar = np.zeros((m.nnode, 3))
+    ubar[top, 2] = stretch
+    m.name = f"hex8_pull_{N}"
+    return m, ubar
+
+
 # ------------------------------------------------------------------ 2-D squares
 def _square(n: int, tria: bool, E: float, nu: float, traction, edge: str, plane_stress: bool,
             thickness: float = 0.0) -> Mesh:

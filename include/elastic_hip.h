@@ -124,6 +124,38 @@ int elastic_assemble(elastic_ctx* ctx, int32_t isw, int32_t lhs, int32_t rhs, co
 /* Average device time (ms) of one stiffness assembly (isw 2, lhs) over reps. */
 int elastic_time_assembly(elastic_ctx* ctx, int32_t reps, double* ms);
 
+/* ------------------------------------------- J2 plasticity (hex8, GPU) */
+/* No counterpart on the reference's called path (Plasticity01.f is not
+ * reached from its driver).  Small-strain J2 plasticity with linear isotropic
+ * hardening (yield stress sy + H * equivalent plastic strain, the law of
+ * Plasticity01.f:322), closed-form radial return and consistent tangent,
+ * for Newton steps that re-assemble K_t on the GPU and solve with
+ * elastic_solve_resident or a batch.  State lives on the device: committed
+ * and trial plastic strain (6) and equivalent plastic strain per Gauss point,
+ * nodal displacements u(3,nnode).  After elastic_set_mesh (hex8 mesh, E and
+ * nu from its material): */
+/* allocates the state (zero) and stores ubar(3,nnode), the prescribed
+ * displacements of restrained dofs at load factor 1. */
+int elastic_j2_setup(elastic_ctx* ctx, double sy, double H, const double* ubar);
+/* u = lambda * ubar on restrained dofs (free dofs keep their values). */
+int elastic_j2_increment(elastic_ctx* ctx, double lambda);
+/* Return map at every Gauss point from u and the committed state; K_t into
+ * the device ad/al when lhs (and the operator derived from it), the residual
+ * b = -f_int into the device b of elastic_solve_resident; *rnorm = ||b||. */
+int elastic_j2_assemble(elastic_ctx* ctx, int32_t lhs, double* rnorm);
+/* u += x (the device x of the last resident solve) on free dofs. */
+int elastic_j2_correct(elastic_ctx* ctx);
+/* committed state <- state of the last assembly; returns the largest
+ * equivalent plastic strain and the number of yielded Gauss points. */
+int elastic_j2_commit(elastic_ctx* ctx, double* alpha_max, int64_t* nplastic);
+int elastic_j2_get_u(elastic_ctx* ctx, double* u);
+/* the residual b = -f_int of the last elastic_j2_assemble (neq values) */
+int elastic_j2_get_residual(elastic_ctx* ctx, double* r);
+/* plastic strains (6 per Gauss point, element-major) and equivalent plastic
+ * strains of the committed (trial = 0) or last assembled (trial = 1) state;
+ * either pointer may be NULL. */
+int elastic_j2_get_state(elastic_ctx* ctx, int32_t trial, double* ep, double* alpha);
+
 /* ----------------------------------------------------------- solving */
 /* Solve A x = b.  Replaces solv -> pcg (Solv.f:13-139, Iters.f:252-524) for
  * solver 1, gmres (Iters.f:528-808) for 2 and pbicgstab (Iters.f:1079-1349)

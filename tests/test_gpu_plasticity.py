@@ -1,0 +1,152 @@
+"""J2 plasticity on the GPU against the numpy oracle (oracle/j2.py, pinned by
+closed forms and finite differences in tests/test_j2_oracle.py; parity
+against the reference is unpinned: its plasticity is not on a called path).
+
+Bars (written here):
+* one assembly from a yielding state: K_t (ad, al), the residual -f_int and
+  the trial state within 1e-11 relative of the oracle's element loop;
+* Newton on a pulled block through the yield point: u within 1e-8 relative of
+  the oracle's Newton with dense solves, each increment converging in <= 8
+  iterations with the last two residuals dropping at least quadratically;
+* several load paths (different yield stresses) solved together in a batch
+  give bit-identical u to the same paths solved one at a time in the batch.
+"""
+import numpy as np
+import pytest
+
+from oracle import j2
+
+pytestmark = pytest.mark.gpu
+
+E, NU, H = 1.0, 0.3, 0.05
+
+
+def block(N=3, stretch=5e-3, jitter=0.1):
+    from elastic_amd import mesh as M
+
+    m = M.hex8_cube(N, jitter=jitter, seed=M.BASE_SEED + 5)
+    top = m.x[:, 2] == 1.0
+    m.id[top] = 1
+    m.eload[:] = 0
+    m.loads = []
+    ubar = np.zeros((m.nnode, 3))
+    ubar[top, 2] = stretch
+    return m, ubar
+
+
+def oracle_assemble(path, u, ep_n, al_n, sy):
+    """Dense K_t over free equations, R = -f_int, trial state."""
+    m, idn, neq = path.mesh, path.sys.idn, path.sys.neq
+    K = np.zeros((neq, neq))
+    R = np.zeros(neq)
+    ep_t, al_t = np.zeros_like(ep_n), np.zeros_like(al_n)
+    for e in range(m.numel):
+        nodes = m.ix[e, :8] - 1
+        xl = m.x[nodes]
+        ue = u[nodes].ravel()
+        f, Ke, ep_t[e], al_t[e] = j2.element(xl, ue, ep_n[e], al_n[e], E, NU, sy, H)
+        eq = idn[nodes].ravel() - 1
+        for a in range(24):
+            if eq[a] < 0:
+                continue
+            R[eq[a]] -= f[a]
+            for b in range(24):
+                if eq[b] >= 0:
+                    K[eq[a], eq[b]] += Ke[a, b]
+    return K, R, ep_t, al_t
+
+
+def dense_from_csrc(ia, ja, ad, al):
+    n = ad.shape[0]
+    K = np.diag(ad.copy())
+    for i in range(n):
+        for k in range(ia[i] - 1, ia[i + 1] - 1):
+            K[i, ja[k] - 1] = al[k]
+            K[ja[k] - 1, i] = al[k]
+    return K
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_j2_assembly_matches_oracle():
+    from elastic_amd.plasticity import PlasticPath
+
+    m, ubar = block()
+    p = PlasticPath(m, ubar, 1e-3, H)
+    p.increment(1.0)
+    rn = p.assemble()
+    u = p.u()
+    ngp = m.numel
+    K, R, ep_t, al_t = oracle_assemble(p, u, np.zeros((ngp, 8, 6)), np.zeros((ngp, 8)), 1e-3)
+    ad, al = p.ctx.get_values()
+    Kg = dense_from_csrc(p.sys.ia, p.sys.ja, ad, al)
+    assert rel(Kg, K) <= 1e-11
+    r = p.residual()
+    assert rel(r, R) <= 1e-11 and abs(rn - np.linalg.norm(R)) <= 1e-11 * np.linalg.norm(R)
+    gep, gal = p.state(trial=True)
+    assert (al_t > 0).any()
+    assert rel(gal, al_t) <= 1e-11 and rel(gep, ep_t) <= 1e-11
+    p.close()
+
+
+def oracle_newton(path, ubar, sy, increments, rtol):
+    m, idn, neq = path.mesh, path.sys.idn, path.sys.neq
+    u = np.zeros((m.nnode, 3))
+    ep = np.zeros((m.numel, 8, 6))
+    al = np.zeros((m.numel, 8))
+    restrained = idn <= 0
+    free = idn > 0
+    for inc in range(1, increments + 1):
+        u[restrained] = (inc / increments) * ubar[restrained]
+        r0 = None
+        for it in range(30):
+            K, R, ep_t, al_t = oracle_assemble(path, u, ep, al, sy)
+            r = np.linalg.norm(R)
+            r0 = r if r0 is None else r0
+            if r <= rtol * r0 or r == 0.0:
+                ep, al = ep_t, al_t
+                break
+            du = np.linalg.solve(K, R)
+            u[free] += du[idn[free] - 1]
+    return u
+
+
+def test_j2_newton_matches_oracle():
+    from elastic_amd.plasticity import PlasticPath, run_paths
+
+    m, ubar = block(N=3, stretch=4e-3)
+    p = PlasticPath(m, ubar, 1e-3, H)
+    hist = run_paths([p], increments=4, rtol=1e-9, precision=64, tol=1e-13)
+    uo = oracle_newton(p, ubar, 1e-3, 4, 1e-9)
+    assert rel(p.u(), uo) <= 1e-8
+    a, n = p.commit()
+    assert n > 0 and a > 0
+    for inc in range(1, 5):
+        rs = [r for (i, it, r) in hist[0] if i == inc]
+        assert len(rs) <= 8
+        if len(rs) >= 4:  # the tail of Newton is quadratic
+            assert rs[-1] <= 10 * rs[-2] ** 2 / rs[0] + 1e-14 * rs[0]
+    p.close()
+
+
+def test_j2_batched_paths_equal_alone():
+    from elastic_amd.plasticity import PlasticPath, run_paths
+    from elastic_amd.serving import Batch
+
+    m, ubar = block(N=4, stretch=4e-3)
+    sys_y = [5e-4, 1e-3, 2e-3]
+    alone = []
+    for sy in sys_y:
+        p = PlasticPath(m, ubar, sy, H)
+        with Batch(0, max_slots=2, iters_per_step=8) as b:
+            run_paths([p], increments=3, rtol=1e-8, batch=b)
+        alone.append(p.u())
+        p.close()
+    paths = [PlasticPath(m, ubar, sy, H) for sy in sys_y]
+    with Batch(0, max_slots=3, iters_per_step=8) as b:
+        run_paths(paths, increments=3, rtol=1e-8, batch=b)
+    for i, p in enumerate(paths):
+        assert np.array_equal(p.u(), alone[i])
+        p.close()
