@@ -13,7 +13,8 @@
 // in chunk order) are functions of its own size, so a result is bit for bit
 // the same alone and in any mix.  The method per instance is the PCG of
 // pcg.hip (Iters.f:252-524, Jacobi, x0 = 0, M-norm stopping rule), in FP64
-// or with the FP32-value operator and FP64 true-residual replacement.
+// or with the FP32 operator (FP32 values, staging and sums) and FP64
+// true-residual replacement.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -83,10 +84,10 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
     const bool f32 = mode == 0 && I.fp32;
     if (mode == 0) {
       if (I.nb == 3) {
-        if (f32) block_tile<float, double, 3, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
+        if (f32) block_tile<float, float, 3, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
         else block_tile<double, double, 3, kBatchPer>(I.a, t, I.v64, I.d64, I.p, I.q, I.epart, smem);
       } else {
-        if (f32) block_tile<float, double, 2, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
+        if (f32) block_tile<float, float, 2, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
         else block_tile<double, double, 2, kBatchPer>(I.a, t, I.v64, I.d64, I.p, I.q, I.epart, smem);
       }
     } else {

@@ -206,7 +206,6 @@ int elastic_create(elastic_ctx** out, int32_t device) {
     return ELASTIC_HIP_ERROR;
   }
   if (const char* ac = getenv("ELASTIC_ACCUM")) c->accum32 = atoi(ac) == 32;
-  if (const char* pg = getenv("ELASTIC_PIPE_GRID")) c->pipe_grid64 = c->pipe_grid32 = atoi(pg);
   if (const char* tb = getenv("ELASTIC_TILE_BLOCKS"))
     c->tile_blocks = std::max(32, std::min(kBlkTileMax, atoi(tb)));
   const char* ce = getenv("ELASTIC_CHECK_EVERY");
@@ -363,16 +362,7 @@ int elastic_set_option(elastic_ctx* x, const char* name, int64_t value) {
   if (!x || !name) return ELASTIC_BAD_ARG;
   Context* c = &x->c;
   const std::string n(name);
-  if (n == "pipe_grid") {
-    if (value < 0 || value > (1 << 20)) return ELASTIC_BAD_ARG;
-    c->pipe_grid64 = c->pipe_grid32 = (int)value;
-  } else if (n == "pipe_grid64") {
-    if (value < 0 || value > (1 << 20)) return ELASTIC_BAD_ARG;
-    c->pipe_grid64 = (int)value;
-  } else if (n == "pipe_grid32") {
-    if (value < 0 || value > (1 << 20)) return ELASTIC_BAD_ARG;
-    c->pipe_grid32 = (int)value;
-  } else if (n == "accum32") {
+  if (n == "accum32") {
     c->accum32 = value != 0;
   } else if (n == "check_every") {
     if (value < 1 || value > 4096) return ELASTIC_BAD_ARG;

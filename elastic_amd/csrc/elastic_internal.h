@@ -140,9 +140,7 @@ struct Context {
   int tile_nnz = 2048;
   int tile_blocks = 512;
   int layout_nb = 1;           // 1: scalar tiles; 2 or 3: node-blocked tiles
-  bool accum32 = false;        // FP32 operator: FP32 staging and sums (else FP64)
-  int pipe_grid64 = 0;         // > 0: persistent pipelined product with this grid (FP64)
-  int pipe_grid32 = 0;         // same for the FP32 operator
+  bool accum32 = true;         // FP32 operator: FP32 staging and sums (else FP64)
   bool allow_blocks = true;
   TileLayout layout;
   DevTiles tiles;

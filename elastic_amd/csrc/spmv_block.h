@@ -131,7 +131,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   const uint32_t* wr = R.wr;
   const auto& vr = R.vr;
   // x of the tile's local columns: the first 8 * 256 entries are loaded into
-  // registers, then `hook` runs (it may issue loads of a next tile, which then
+  // registers, then `hook` runs (a caller may issue more loads there, which
   // stay in flight behind these), then they go to LDS
   constexpr int XR = 8;
   const int nx = ncols * NB;

@@ -17,45 +17,6 @@ __global__ __launch_bounds__(256) void k_spmv_blocks(BlkArgs a, const TV* __rest
   block_tile<TV, T, NB, PER>(a, blockIdx.x, vals, diag, x, y, epart, smem);
 }
 
-// Persistent variant: a fixed grid (a multiple of 8) walks the tiles.
-// Workgroup b runs on XCD b % 8 (round-robin dispatch) and takes, in turn,
-// the tiles of that XCD's contiguous eighth of the tile range, so the x
-// gathers of neighbouring tiles share one L2.  The loads of its next tile are
-// issued right after the current tile's x gathers, and stay in flight through
-// the current tile's products and sums.
-template <typename TV, typename T, int NB, int PER>
-__global__ __launch_bounds__(256) void k_spmv_blocks_pipe(BlkArgs a, int ntiles,
-                                                           const TV* __restrict__ vals,
-                                                           const TV* __restrict__ diag,
-                                                           const double* __restrict__ x,
-                                                           double* __restrict__ y,
-                                                           double* __restrict__ epart,
-                                                           const int32_t* __restrict__ skip) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  if (skip != nullptr && *skip != 0) return;
-  constexpr int kXcd = 8;
-  const int xcd = blockIdx.x % kXcd, step = gridDim.x / kXcd;
-  const int t1 = (int)((int64_t)ntiles * (xcd + 1) / kXcd);
-  int t = (int)((int64_t)ntiles * xcd / kXcd) + (int)(blockIdx.x / kXcd);
-  if (t >= t1) return;
-  TileRegs<TV, NB, PER> A, B;
-  load_tile<TV, NB, PER>(a, t, vals, diag, A);
-  while (true) {
-    int tn = t + step;
-    compute_tile<TV, T, NB, PER>(a, t, A, x, y, epart, smem, [&]() {
-      if (tn < t1) load_tile<TV, NB, PER>(a, tn, vals, diag, B);
-    });
-    if (tn >= t1) break;
-    t = tn;
-    tn = t + step;
-    compute_tile<TV, T, NB, PER>(a, t, B, x, y, epart, smem, [&]() {
-      if (tn < t1) load_tile<TV, NB, PER>(a, tn, vals, diag, A);
-    });
-    if (tn >= t1) break;
-    t = tn;
-  }
-}
-
 // Gathers the CSRC al/ad into the block planes (one workgroup per tile) and
 // the node diagonal blocks (grid over nodes).
 template <typename TV, int NB>
@@ -106,10 +67,9 @@ __global__ __launch_bounds__(256) void k_pack_diag(int nn, const int32_t* __rest
 template <typename TV, typename TA, int NB>
 static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* diag,
                                const double* x, double* y, double* epart, hipStream_t s,
-                               const int32_t* skip, int pipe_grid) {
+                               const int32_t* skip) {
   const size_t lds = sizeof(TA) * NB * ((size_t)b.ncols_max + 2 * kBlkRowsMax + (size_t)b.nblk_max);
   const int per = (b.nblk_max + 255) / 256;
-  const int grid = pipe_grid > 0 ? std::max(8, (std::min(pipe_grid, b.ntiles + 7) / 8) * 8) : 0;
 #define ELASTIC_BLK_LAUNCH(K, P, G, ...)                                                        \
   do {                                                                                          \
     static bool raised = false;                                                                 \
@@ -120,15 +80,9 @@ static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* dia
     }                                                                                           \
     hipLaunchKernelGGL((K<TV, TA, NB, P>), dim3(G), dim3(256), lds, s, __VA_ARGS__);            \
   } while (0)
-  if (grid > 0) {
-    if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks_pipe, 1, grid, blk_args(b), b.ntiles, vals, diag, x, y, epart, skip);
-    else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks_pipe, 2, grid, blk_args(b), b.ntiles, vals, diag, x, y, epart, skip);
-    else ELASTIC_BLK_LAUNCH(k_spmv_blocks_pipe, 4, grid, blk_args(b), b.ntiles, vals, diag, x, y, epart, skip);
-  } else {
-    if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 1, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip);
-    else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip);
-    else ELASTIC_BLK_LAUNCH(k_spmv_blocks, 4, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip);
-  }
+  if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 1, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip);
+  else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip);
+  else ELASTIC_BLK_LAUNCH(k_spmv_blocks, 4, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip);
 #undef ELASTIC_BLK_LAUNCH
 }
 
@@ -136,15 +90,14 @@ void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epar
                  const int32_t* skip) {
   hipStream_t s = c->stream;
   const DevBlocks& b = c->blk;
-  const int pg = fp32 ? c->pipe_grid32 : c->pipe_grid64;
   if (c->layout_nb == 3) {
-    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 3>(b, b.v32, b.d32, x, y, epart, s, skip, pg);
-    else if (fp32) spmv_blocks_launch<float, double, 3>(b, b.v32, b.d32, x, y, epart, s, skip, pg);
-    else spmv_blocks_launch<double, double, 3>(b, b.v64, b.d64, x, y, epart, s, skip, pg);
+    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 3>(b, b.v32, b.d32, x, y, epart, s, skip);
+    else if (fp32) spmv_blocks_launch<float, double, 3>(b, b.v32, b.d32, x, y, epart, s, skip);
+    else spmv_blocks_launch<double, double, 3>(b, b.v64, b.d64, x, y, epart, s, skip);
   } else if (c->layout_nb == 2) {
-    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 2>(b, b.v32, b.d32, x, y, epart, s, skip, pg);
-    else if (fp32) spmv_blocks_launch<float, double, 2>(b, b.v32, b.d32, x, y, epart, s, skip, pg);
-    else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip, pg);
+    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 2>(b, b.v32, b.d32, x, y, epart, s, skip);
+    else if (fp32) spmv_blocks_launch<float, double, 2>(b, b.v32, b.d32, x, y, epart, s, skip);
+    else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip);
   } else {
     if (fp32)
       launch_spmv_tiles<float, double>(c->tiles, c->A32.ad, c->A32.al, x, y, epart, s, skip);
