@@ -35,7 +35,7 @@ struct BInst {
   BlkArgs a;
   const double *v64, *d64;
   const float *v32, *d32;
-  const int32_t *inc_ptr, *inc_idx;
+  MergeArgs mg;   // external partials of the instance's product -> rows
   double* epart;
   double *x, *r, *p, *q, *b, *t;
   const double* m;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_merge_dot(const BInst* __restri
     double acc = 0.0;
     for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
       double s = I.q[i];
-      for (int k = I.inc_ptr[i]; k < I.inc_ptr[i + 1]; ++k) s += I.epart[I.inc_idx[k]];
+      s = merge_row(I.mg, i, s);
       I.q[i] = s;
       acc += I.p[i] * s;
     }
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_resid(const BInst* __restri
     double acc = 0.0;
     for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
       double s = I.t[i];
-      for (int k = I.inc_ptr[i]; k < I.inc_ptr[i + 1]; ++k) s += I.epart[I.inc_idx[k]];
+      s = merge_row(I.mg, i, s);
       const double ri = I.b[i] - s;
       I.r[i] = ri;
       acc += ri * (ri * I.m[i]);
@@ -537,8 +537,7 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   I.d64 = c->blk.d64;
   I.v32 = c->blk.v32;
   I.d32 = c->blk.d32;
-  I.inc_ptr = c->inc_ptr;
-  I.inc_idx = c->inc_idx;
+  I.mg = merge_of(c, c->epart64);
   I.epart = c->epart64;
   I.x = c->v64[VX];
   I.r = c->v64[VR];

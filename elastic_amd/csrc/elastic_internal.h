@@ -73,6 +73,7 @@ struct BlockLayout {
   std::vector<uint32_t> word;
   std::vector<uint16_t> rowptr, segptr;
   std::vector<int32_t> ext, inc_ptr, inc_idx;
+  std::vector<int32_t> dpos;  // destination slot of each (tile, external node) partial
 };
 
 // 3 (or 2) when the CSRC pattern is a pattern of node blocks, else 1
@@ -88,7 +89,7 @@ struct DevBlocks {
   int64_t* blk0 = nullptr;
   uint32_t* word = nullptr;
   uint16_t *rowptr = nullptr, *segptr = nullptr;
-  int32_t *ext = nullptr, *inc_ptr = nullptr, *inc_idx = nullptr;
+  int32_t *ext = nullptr, *inc_ptr = nullptr, *inc_idx = nullptr, *dpos = nullptr;
   int32_t* ia = nullptr;  // device copy of ia (value pack)
   // values: plane p of block q of tile t at blk0[t]*nb*nb + p*nblk[t] + q;
   // node diagonal blocks: entry c of node I at c*nn + I (nb diagonal entries,
@@ -170,10 +171,35 @@ struct Context {
   int zero_diag_row = -1;  // first row with ad == 0 (1-based), -1 if none
 };
 
+// How the external partials of a product reach their rows.
+//   nb == 1 (scalar tiles): row i adds epart[idx[k]], k in ptr[i]..ptr[i+1]-1
+//   nb > 1 (node blocks): the partials are stored in destination order, so
+//     row i = nb*J + r adds epart[nb*k + r], k in ptr[J]..ptr[J+1]-1
+// Either way the partials of a row are added in tile order.
+struct MergeArgs {
+  const int32_t* ptr;
+  const int32_t* idx;
+  const double* epart;
+  int nb;
+};
+__device__ inline double merge_row(const MergeArgs& m, int i, double s) {
+  if (!m.ptr) return s;
+  if (m.nb <= 1) {
+    for (int k = m.ptr[i]; k < m.ptr[i + 1]; ++k) s += m.epart[m.idx[k]];
+    return s;
+  }
+  const int J = i / m.nb, r = i - J * m.nb;
+  for (int k = m.ptr[J]; k < m.ptr[J + 1]; ++k) s += m.epart[(int64_t)m.nb * k + r];
+  return s;
+}
+
 // device vector slots of a context (c->v64[...])
 enum VecSlot { VX = 0, VR, VP, VQ, VB, VT, VU, VW };
 
 int set_error(Context* c, int code, const std::string& msg);
+inline MergeArgs merge_of(const Context* c, const double* epart) {
+  return MergeArgs{c->inc_ptr, c->inc_idx, epart, c->layout_nb};
+}
 // arrays derived from new A64 values (asynchronous on c->stream): the
 // Jacobi inverse, the FP32 operator (scalar tiles) or the block planes
 void derive_operator(Context* c);
@@ -220,9 +246,8 @@ void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epar
 void launch_block_pack(Context* c);
 template <typename T>
 void launch_recip(int n, const T* ad, T* m, hipStream_t s);
-void launch_post(const int32_t* inc_ptr, const int32_t* inc_idx, int n, const double* epart,
-                 const double* x, double* y, const double* b, const double* m, double* part,
-                 double* out6, hipStream_t s);
+void launch_post(const MergeArgs& mg, int n, const double* x, double* y, const double* b,
+                 const double* m, double* part, double* out6, hipStream_t s);
 void launch_sum_parts(const double* part, int nsum, double* out, hipStream_t s);
 
 // pcg.hip: one PCG solve on device-resident FP64 vectors.  The operator's

@@ -132,9 +132,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_m(int n, const double* __res
 }
 
 // y += merged partials; partial y.a (and y.a2 when a2 != null)
-__global__ __launch_bounds__(kThreads) void k_merge_dot2(int n, const int32_t* __restrict__ inc_ptr,
-                                                          const int32_t* __restrict__ inc_idx,
-                                                          const double* __restrict__ epart,
+__global__ __launch_bounds__(kThreads) void k_merge_dot2(int n, MergeArgs mg,
                                                           double* y, const double* a, double* part_a,
                                                           const double* a2, double* part_b,
                                                           const SolveScalars* sc) {
@@ -143,7 +141,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_dot2(int n, const int32_t* _
   double s1 = 0.0, s2 = 0.0;
   GRID_LOOP(i, n) {
     double s = y[i];
-    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
+    s = merge_row(mg, i, s);
     y[i] = s;
     s1 += s * a[i];
     if (a2) s2 += s * (a2 == y ? s : a2[i]);
@@ -247,9 +245,7 @@ __global__ void k_bi_check(const double* __restrict__ part_rr, const double* __r
 
 // true residual: r = b - (t + merged partials); z = M r (Jacobi) when asked;
 // partials r.r and r.r0
-__global__ __launch_bounds__(kThreads) void k_bi_rresid(int n, const int32_t* __restrict__ inc_ptr,
-                                                         const int32_t* __restrict__ inc_idx,
-                                                         const double* __restrict__ epart,
+__global__ __launch_bounds__(kThreads) void k_bi_rresid(int n, MergeArgs mg,
                                                          const double* __restrict__ t,
                                                          const double* __restrict__ b, double* r,
                                                          double* z, const double* __restrict__ m,
@@ -261,7 +257,7 @@ __global__ __launch_bounds__(kThreads) void k_bi_rresid(int n, const int32_t* __
   double a1 = 0.0, a2 = 0.0;
   GRID_LOOP(i, n) {
     double s = t[i];
-    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
+    s = merge_row(mg, i, s);
     const double ri = b[i] - s;
     r[i] = ri;
     if (z) z[i] = ri * m[i];
@@ -334,7 +330,7 @@ int bicgstab_run(Context* c, bool bdiag, bool fp32_operator, double tol, int max
   // true-residual replacement (reliable mode), gated on sc->upd
   auto replace = [&](bool with_z, double* rr_next) {
     launch_spmv(c, false, x, t, c->epart64, &sc->skip64);
-    hipLaunchKernelGGL(k_bi_rresid, G, TB, 0, s, n, c->inc_ptr, c->inc_idx, c->epart64, t, b, r,
+    hipLaunchKernelGGL(k_bi_rresid, G, TB, 0, s, n, merge_of(c, c->epart64), t, b, r,
                        (with_z && jac) ? z : nullptr, m, r0, P[4], P[5], sc);
     if (with_z && !jac) hipLaunchKernelGGL(k_apply_m, G, TB, 0, s, n, r, m, bm, z, &sc->upd, sc);
     hipLaunchKernelGGL(k_bi_rfinish, dim3(1), TB, 0, s, P[4], P[5], rr_next, sc);
@@ -348,14 +344,14 @@ int bicgstab_run(Context* c, bool bdiag, bool fp32_operator, double tol, int max
       double* rr_cur = rr + (it & 1);
       double* rr_next = rr + ((it + 1) & 1);
       launch_spmv(c, fp32_operator, z, v, c->epart64, &sc->done);
-      hipLaunchKernelGGL(k_merge_dot2, G, TB, 0, s, n, c->inc_ptr, c->inc_idx, c->epart64, v, r0,
+      hipLaunchKernelGGL(k_merge_dot2, G, TB, 0, s, n, merge_of(c, c->epart64), v, r0,
                          P[0], (const double*)nullptr, P[1], sc);
       hipLaunchKernelGGL(k_bi_half1, G, TB, 0, s, n, P[0], rr_cur, x, z, r, v, m, jac, P[1], sc);
       if (!jac) hipLaunchKernelGGL(k_apply_m, G, TB, 0, s, n, r, m, bm, z, nullptr, sc);
       hipLaunchKernelGGL(k_bi_check, dim3(1), TB, 0, s, P[1], P[1], 1, rr_next, sc);
       if (reliable) replace(true, nullptr);
       launch_spmv(c, fp32_operator, z, t, c->epart64, &sc->done);
-      hipLaunchKernelGGL(k_merge_dot2, G, TB, 0, s, n, c->inc_ptr, c->inc_idx, c->epart64, t, r,
+      hipLaunchKernelGGL(k_merge_dot2, G, TB, 0, s, n, merge_of(c, c->epart64), t, r,
                          P[2], (const double*)t, P[3], sc);
       hipLaunchKernelGGL(k_bi_half2, G, TB, 0, s, n, P[2], P[3], x, z, r, t, r0, P[4], P[5], sc);
       hipLaunchKernelGGL(k_bi_check, dim3(1), TB, 0, s, P[4], P[5], 2, rr_next, sc);
@@ -413,26 +409,22 @@ __global__ __launch_bounds__(kThreads) void k_gm_applym_dot(int n, const double*
 }
 
 // u = b - (t + merged partials)
-__global__ __launch_bounds__(kThreads) void k_gm_resid(int n, const int32_t* __restrict__ inc_ptr,
-                                                        const int32_t* __restrict__ inc_idx,
-                                                        const double* __restrict__ epart,
+__global__ __launch_bounds__(kThreads) void k_gm_resid(int n, MergeArgs mg,
                                                         const double* __restrict__ t,
                                                         const double* __restrict__ b, double* u) {
   GRID_LOOP(i, n) {
     double s = t[i];
-    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
+    s = merge_row(mg, i, s);
     u[i] = b[i] - s;
   }
 }
 
 // u = w + merged partials (the raw product; M is applied by k_gm_mdot)
-__global__ __launch_bounds__(kThreads) void k_gm_merge(int n, const int32_t* __restrict__ inc_ptr,
-                                                        const int32_t* __restrict__ inc_idx,
-                                                        const double* __restrict__ epart,
+__global__ __launch_bounds__(kThreads) void k_gm_merge(int n, MergeArgs mg,
                                                         const double* __restrict__ w, double* u) {
   GRID_LOOP(i, n) {
     double s = w[i];
-    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
+    s = merge_row(mg, i, s);
     u[i] = s;
   }
 }
@@ -589,7 +581,7 @@ int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit,
   for (int l = 1; l <= maxit && !conv; ++l) {
     // g1 = M (b - A x), FP64 operator (the true residual)
     launch_spmv(c, false, x, t, c->epart64);
-    hipLaunchKernelGGL(k_gm_resid, G, TB, 0, s, n, c->inc_ptr, c->inc_idx, c->epart64, t, b, u);
+    hipLaunchKernelGGL(k_gm_resid, G, TB, 0, s, n, merge_of(c, c->epart64), t, b, u);
     hipLaunchKernelGGL(k_gm_applym_dot, G, TB, 0, s, n, u, m, bm, g(1), P[0]);
     hipLaunchKernelGGL(k_gm_norm0, dim3(1), TB, 0, s, P[0], a, sc);
     if (fp32_operator && l > 1) {  // mixed: decide on the FP64 residual
@@ -607,7 +599,7 @@ int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit,
       ++nit;
       ++ni;
       launch_spmv(c, fp32_operator, g(i), t, c->epart64);
-      hipLaunchKernelGGL(k_gm_merge, G, TB, 0, s, n, c->inc_ptr, c->inc_idx, c->epart64, t, u);
+      hipLaunchKernelGGL(k_gm_merge, G, TB, 0, s, n, merge_of(c, c->epart64), t, u);
       double* w = g(i + 1);
       hipLaunchKernelGGL(k_gm_mdot, G, TB, 0, s, n, u, m, bm, w, g(1), P[0]);
       for (int j = 1; j <= i; ++j)

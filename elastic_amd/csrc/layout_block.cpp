@@ -176,16 +176,15 @@ void build_block_tiles(int neq, int nb, const int32_t* ia1, const int32_t* ja1, 
       if (r < L.nrows[t]) acc += nbr[L.row0[t] + r];
     }
   }
-  // merge lists in scalar rows: row nb*J+c receives epart[nb*q + c] for every
-  // external node entry q = (tile, e) with ext = J, in tile order
-  L.inc_ptr.assign((size_t)neq + 1, 0);
-  for (int64_t q = 0; q < next; ++q)
-    for (int c = 0; c < nb; ++c) L.inc_ptr[(size_t)nb * L.ext[q] + c + 1]++;
-  for (int i = 0; i < neq; ++i) L.inc_ptr[i + 1] += L.inc_ptr[i];
-  L.inc_idx.resize((size_t)next * nb);
+  // destination order of the external partials: those of node J are
+  // contiguous (inc_ptr[J] .. inc_ptr[J+1]-1), in tile order
+  L.inc_ptr.assign((size_t)nn + 1, 0);
+  for (int64_t q = 0; q < next; ++q) L.inc_ptr[(size_t)L.ext[q] + 1]++;
+  for (int J = 0; J < nn; ++J) L.inc_ptr[J + 1] += L.inc_ptr[J];
+  L.dpos.resize(next);
   std::vector<int32_t> fill(L.inc_ptr.begin(), L.inc_ptr.end() - 1);
-  for (int64_t q = 0; q < next; ++q)
-    for (int c = 0; c < nb; ++c) L.inc_idx[fill[(size_t)nb * L.ext[q] + c]++] = (int32_t)(nb * q + c);
+  for (int64_t q = 0; q < next; ++q) L.dpos[q] = fill[L.ext[q]]++;
+  L.inc_idx.clear();
 }
 
 }  // namespace elastic

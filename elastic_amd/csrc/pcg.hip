@@ -67,9 +67,7 @@ __global__ void k_pcg_init2(const double* __restrict__ part, double tol, double 
 }
 
 // q += merged external partials; partial p.q
-__global__ __launch_bounds__(kThreads) void k_merge_dot(int n, const int32_t* __restrict__ inc_ptr,
-                                                         const int32_t* __restrict__ inc_idx,
-                                                         const double* __restrict__ epart,
+__global__ __launch_bounds__(kThreads) void k_merge_dot(int n, MergeArgs mg,
                                                          double* q, const double* __restrict__ p,
                                                          double* part, const SolveScalars* sc) {
   __shared__ double sh[8];
@@ -77,7 +75,7 @@ __global__ __launch_bounds__(kThreads) void k_merge_dot(int n, const int32_t* __
   double acc = 0.0;
   GRID_LOOP(i, n) {
     double s = q[i];
-    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
+    s = merge_row(mg, i, s);
     q[i] = s;
     acc += p[i] * s;
   }
@@ -165,9 +163,7 @@ __global__ void k_rel_force(SolveScalars* sc) {
 }
 
 // r = b - (t + merged partials); with Jacobi also partial r.(m r)
-__global__ __launch_bounds__(kThreads) void k_rel_resid(int n, const int32_t* __restrict__ inc_ptr,
-                                                         const int32_t* __restrict__ inc_idx,
-                                                         const double* __restrict__ epart,
+__global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
                                                          const double* __restrict__ t,
                                                          const double* __restrict__ b,
                                                          const double* __restrict__ m, double* r,
@@ -178,7 +174,7 @@ __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, const int32_t* __
   double acc = 0.0;
   GRID_LOOP(i, n) {
     double s = t[i];
-    for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
+    s = merge_row(mg, i, s);
     const double ri = b[i] - s;
     r[i] = ri;
     acc += ri * (ri * m[i]);
@@ -225,7 +221,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   auto replace = [&](int iters_so_far) {
     double* dnext = dslots + (iters_so_far & 1);
     launch_spmv(c, false, V.x, V.t, V.epart, &sc->skip64);
-    hipLaunchKernelGGL(k_rel_resid, G, TB, 0, s, n, c->inc_ptr, c->inc_idx, V.epart,
+    hipLaunchKernelGGL(k_rel_resid, G, TB, 0, s, n, merge_of(c, V.epart),
                        V.t, V.b, V.m, V.r, jac, part_b, sc);
     if (!jac) hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, &sc->upd, sc);
     hipLaunchKernelGGL(k_rel_finish, dim3(1), TB, 0, s, part_b, dnext, sc);
@@ -239,7 +235,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
       double* dcur = dslots + (it & 1);
       double* dnxt = dslots + ((it + 1) & 1);
       launch_spmv(c, fp32_operator, V.p, V.q, V.epart, &sc->done);
-      hipLaunchKernelGGL(k_merge_dot, G, TB, 0, s, n, c->inc_ptr, c->inc_idx, V.epart,
+      hipLaunchKernelGGL(k_merge_dot, G, TB, 0, s, n, merge_of(c, V.epart),
                          V.q, V.p, part_a, sc);
       hipLaunchKernelGGL(k_pcg_update, G, TB, 0, s, n, part_a, dcur, V.x, V.p, V.r, V.q, V.m, jac,
                          part_b, sc);

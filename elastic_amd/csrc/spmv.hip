@@ -109,15 +109,8 @@ __global__ __launch_bounds__(256) void k_spmv_tiles(TileArgs a, const TV* __rest
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_merge(int n, const int32_t* __restrict__ inc_ptr,
-                                                const int32_t* __restrict__ inc_idx,
-                                                const T* __restrict__ epart, T* __restrict__ y) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    T s = y[i];
-    for (int q = inc_ptr[i]; q < inc_ptr[i + 1]; ++q) s += epart[inc_idx[q]];
-    y[i] = s;
-  }
+__global__ __launch_bounds__(256) void k_merge(int n, MergeArgs mg, double* __restrict__ y) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) y[i] = merge_row(mg, i, y[i]);
 }
 
 template <typename TV, typename T>
@@ -147,8 +140,7 @@ void launch_spmv_tiles(const DevTiles& t, const TV* ad, const TV* al, const T* x
 void launch_merge(Context* c, const double* epart, double* y) {
   const int neq = c->neq;
   int blocks = std::max(1, std::min((neq + 255) / 256, 4096));
-  hipLaunchKernelGGL(k_merge<double>, dim3(blocks), dim3(256), 0, c->stream, neq, c->inc_ptr,
-                     c->inc_idx, epart, y);
+  hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(256), 0, c->stream, neq, merge_of(c, epart), y);
 }
 
 template void launch_spmv_tiles<double, double>(const DevTiles&, const double*, const double*,

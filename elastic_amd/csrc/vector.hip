@@ -29,11 +29,9 @@ __global__ __launch_bounds__(kT) void k_recip(int n, const T* __restrict__ ad, T
   for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) m[i] = T(1) / ad[i];
 }
 
-// y += merged partials (when inc_ptr != null); then 6 partial sums:
+// y += merged partials (when mg.ptr != null); then 6 partial sums:
 // x.y, x.x, (b-y).m(b-y), (b-y).(b-y), b.b, b.(m b)
-__global__ __launch_bounds__(kT) void k_post(int n, const int32_t* __restrict__ inc_ptr,
-                                              const int32_t* __restrict__ inc_idx,
-                                              const double* __restrict__ epart,
+__global__ __launch_bounds__(kT) void k_post(int n, MergeArgs mg,
                                               const double* __restrict__ x, double* y,
                                               const double* __restrict__ b,
                                               const double* __restrict__ m, double* part) {
@@ -41,8 +39,7 @@ __global__ __launch_bounds__(kT) void k_post(int n, const int32_t* __restrict__ 
   double a[6] = {0, 0, 0, 0, 0, 0};
   for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
     double s = y[i];
-    if (inc_ptr)
-      for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) s += epart[inc_idx[k]];
+    s = merge_row(mg, i, s);
     y[i] = s;
     const double xi = x[i], bi = b[i], ri = bi - s;
     a[0] += xi * s;
@@ -92,11 +89,9 @@ void launch_recip(int n, const T* ad, T* m, hipStream_t s) {
 template void launch_recip<double>(int, const double*, double*, hipStream_t);
 template void launch_recip<float>(int, const float*, float*, hipStream_t);
 
-void launch_post(const int32_t* inc_ptr, const int32_t* inc_idx, int n, const double* epart,
-                 const double* x, double* y, const double* b, const double* m, double* part,
-                 double* out6, hipStream_t s) {
-  hipLaunchKernelGGL(k_post, dim3(kRedBlocks), dim3(kT), 0, s, n, inc_ptr, inc_idx, epart, x, y, b,
-                     m, part);
+void launch_post(const MergeArgs& mg, int n, const double* x, double* y, const double* b,
+                 const double* m, double* part, double* out6, hipStream_t s) {
+  hipLaunchKernelGGL(k_post, dim3(kRedBlocks), dim3(kT), 0, s, n, mg, x, y, b, m, part);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, s, part, 6, out6);
 }
 
