@@ -13,8 +13,8 @@
 // in chunk order) are functions of its own size, so a result is bit for bit
 // the same alone and in any mix.  The method per instance is the PCG of
 // pcg.hip (Iters.f:252-524, Jacobi, x0 = 0, M-norm stopping rule), in FP64
-// or with the FP32 operator (FP32 values, staging and sums) and FP64
-// true-residual replacement.
+// or with the FP32 operator (FP32 values; staging and sums in FP64, or in
+// FP32 when the context asks for it) and FP64 true-residual replacement.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -42,7 +42,7 @@ struct BInst {
   double* part;   // 3 * nch partials: p.q, r.Mr, true-residual r.Mr
   double* dslot;  // 2 ping-pong slots of r.Mr
   SolveScalars* sc;
-  int n, nb, nch, fp32;
+  int n, nb, nch, fp32, acc32;
 };
 
 // ---------------------------------------------------------------- kernels
@@ -84,10 +84,12 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
     const bool f32 = mode == 0 && I.fp32;
     if (mode == 0) {
       if (I.nb == 3) {
-        if (f32) block_tile<float, float, 3, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
+        if (f32 && I.acc32) block_tile<float, float, 3, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
+        else if (f32) block_tile<float, double, 3, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
         else block_tile<double, double, 3, kBatchPer>(I.a, t, I.v64, I.d64, I.p, I.q, I.epart, smem);
       } else {
-        if (f32) block_tile<float, float, 2, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
+        if (f32 && I.acc32) block_tile<float, float, 2, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
+        else if (f32) block_tile<float, double, 2, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
         else block_tile<double, double, 2, kBatchPer>(I.a, t, I.v64, I.d64, I.p, I.q, I.epart, smem);
       }
     } else {
@@ -553,6 +555,7 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   I.nb = c->layout_nb;
   I.nch = nch;
   I.fp32 = precision == ELASTIC_MIXED;
+  I.acc32 = c->accum32 ? 1 : 0;
   b->h_insts[k] = I;
   BATCH_HIP(b, hipMemcpyAsync(b->d_insts + k, &b->h_insts[k], sizeof(BInst), hipMemcpyHostToDevice,
                               b->stream));

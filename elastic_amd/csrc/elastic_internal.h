@@ -141,7 +141,7 @@ struct Context {
   int tile_nnz = 2048;
   int tile_blocks = 512;
   int layout_nb = 1;           // 1: scalar tiles; 2 or 3: node-blocked tiles
-  bool accum32 = true;         // FP32 operator: FP32 staging and sums (else FP64)
+  bool accum32 = false;        // FP32 operator: FP32 staging and sums (else FP64)
   bool allow_blocks = true;
   TileLayout layout;
   DevTiles tiles;

@@ -98,8 +98,8 @@ int elastic_set_values(elastic_ctx* ctx, const double* ad, const double* al);
 int elastic_get_values(elastic_ctx* ctx, double* ad, double* al);
 /* y = (D + L + L^T) x, Matvec.f:528-621.  precision ELASTIC_FP64 or
  * ELASTIC_MIXED (the FP32 operator of the mixed method: values rounded to
- * FP32, x staged and products summed in FP32 unless "accum32" is 0; y and
- * the partial sums between tiles in FP64). */
+ * FP32, products summed in FP64, or in FP32 with "accum32"; y and the
+ * partial sums between tiles in FP64). */
 int elastic_spmv(elastic_ctx* ctx, int32_t precision, const double* x, double* y);
 
 /* ------------------------------------------------------- GPU assembly */
@@ -209,8 +209,9 @@ int elastic_set_tile_nnz(elastic_ctx* ctx, int32_t tile_nnz);
 int elastic_set_tile_blocks(elastic_ctx* ctx, int32_t tile_blocks);
 /* Tuning knobs (no counterpart in the reference): "tile_blocks" /
  * "tile_nnz" (tile size; rebuilds the layout at once when a structure is
- * set), "accum32" (1, the default: the FP32 operator stages x and sums in
- * FP32; 0: in FP64), "check_every" (iterations between convergence polls). */
+ * set), "accum32" (1: the FP32 operator stages x and sums in FP32, faster
+ * but less accurate, which ill-conditioned systems may not tolerate; 0, the
+ * default: in FP64), "check_every" (iterations between convergence polls). */
 int elastic_set_option(elastic_ctx* ctx, const char* name, int64_t value);
 /* Autotuner (no counterpart in the reference): with values resident, times
  * the product of the given precision for each candidate tile size (HIP

@@ -178,3 +178,20 @@ def test_server_small_stream(mode):
         assert rel(r.x, x64) <= 1e-8
         s.ctx.close()
     assert res["latency_ms_p99"] >= res["latency_ms_p50"] > 0
+
+
+def test_batch_fp32_accumulation_instances(golden):
+    """Instances whose context asks for FP32 sums keep them in the batch and
+    still meet the mixed bar; alone and in the mix stay bit-identical."""
+    gs = [golden(n) for n in ["quad4_8", "hex8_5j", "tria3_8"]]
+    ctxs = [make_ctx(g) for g in gs]
+    for c in ctxs[:2]:
+        c.set_option("accum32", 1)
+    mix = run_batch(ctxs, MIXED, 1e-11)
+    for i, g in enumerate(gs):
+        x, st = mix[i]
+        assert st["status"] == 0 and rel(x, g["x"]) <= 1e-8
+        xa, sa = run_batch([ctxs[i]], MIXED, 1e-11)[0]
+        assert np.array_equal(xa, x) and sa["iters"] == st["iters"]
+    for c in ctxs:
+        c.close()

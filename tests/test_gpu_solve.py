@@ -228,3 +228,21 @@ def test_autotune_keeps_results(golden, name):
         x, st = c.solve(g["b"], tol=float(g["tol"]))
         assert abs(st["iters"] - int(g["pcg_iters"])) <= 2
         assert rel(x, g["x"]) <= 1e-10
+
+
+@pytest.mark.parametrize("name", ["hex8_5j", "quad4_8", "tria3_8", "tet4_3j"])
+def test_fp32_accumulation_option(golden, name):
+    """The FP32 operator with FP32 staging and sums ("accum32"): the product
+    within 2e-5 relative L2 of the reference's FP64 product (FP32 sums over
+    up to 81 terms per row), and the mixed PCG still within 1e-8 of the
+    reference's x (the FP64 true-residual replacement corrects it)."""
+    from elastic_amd.solver import MIXED, Context
+
+    g = golden(name)
+    with Context(0) as c:
+        c.set_option("accum32", 1)
+        load_system(c, g)
+        assert rel(c.spmv(g["spmv_v"], precision=MIXED), g["spmv_y"]) <= 2e-5
+        x, st = c.solve(g["b"], precision=MIXED, tol=float(g["tol"]))
+        assert st["norm_m_r"] < st["conv"]
+        assert rel(x, g["x"]) <= 1e-8
