@@ -73,10 +73,14 @@ def main():
     def sync():
         torch.cuda.synchronize()
 
+    def phase(msg):
+        print("%s (%.1f s)" % (msg, time.time() - t0), file=sys.stderr, flush=True)
+
     # reference answer on the GPU: FP64 PCG tightened to 1e-12
     st_ref = ctx.solve_resident(precision=FP64, tol=1e-12, maxit=20000)
     x_ref = ctx.download_x()
 
+    phase("reference solve done")
     # plain FP64 GPU PCG (the drop-in), timed
     lat64 = []
     for i in range(args.warmup + args.steps):
@@ -88,6 +92,7 @@ def main():
             lat64.append(time.perf_counter() - a)
     x64 = ctx.download_x()
 
+    phase("fp64 solves timed")
     # measured method: FP32 operator inside the FP64 reliable-update PCG.  The
     # FP32 operator sums in FP64 or in FP32: both are timed once here
     # (outside the timed region) and the faster one that converges is kept
@@ -115,6 +120,7 @@ def main():
     sync()
     t_total = time.perf_counter() - t_start
     xm = ctx.download_x()
+    phase("mixed solves timed")
     err_m = float(np.linalg.norm(xm - x_ref) / np.linalg.norm(x_ref))
     err_64 = float(np.linalg.norm(x64 - x_ref) / np.linalg.norm(x_ref))
 
@@ -128,10 +134,12 @@ def main():
     nb = ctx.layout_info()["nb"]
     copy = copy_probe(torch)
 
+    phase("product, assembly and copy timed")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sysm, st64["iters"], args.cpu_seconds)
 
+    phase("cpu baseline done")
     serving = None
     if args.serving > 0:
         ctx.close()
@@ -196,7 +204,7 @@ def serving_bench(nreq: int, ncompare: int):
 
     workers = min(16, os.cpu_count() or 1)
     srv = Server(max_slots=64, iters_per_step=16, prep_workers=workers, precision=MIXED,
-                 tol=TOL, hbm_budget=64e9)
+                 tol=TOL, hbm_budget=64e9, progress=sys.stderr)
     a = time.perf_counter()
     res = srv.run(request_stream(nreq, rate=200.0), realtime=True)
     print("serving stream: %d requests in %.1f s" % (res["requests"], time.perf_counter() - a),
@@ -208,6 +216,7 @@ def serving_bench(nreq: int, ncompare: int):
     # solve-only comparison on resident systems
     from elastic_amd.serving import MeshRequest
 
+    print("solve-only comparison: preparing %d systems" % ncompare, file=sys.stderr, flush=True)
     systems = []
     for k in range(ncompare):
         s_ = System(MeshRequest(k)())
@@ -221,6 +230,7 @@ def serving_bench(nreq: int, ncompare: int):
         st = s_.ctx.solve_resident(precision=FP64, tol=TOL, maxit=50000)
         it64 += st["iters"]
     t_seq = time.perf_counter() - a
+    print("sequential fp64: %.1f s" % t_seq, file=sys.stderr, flush=True)
     x64 = [s_.ctx.download_x() for s_ in systems]
     a = time.perf_counter()
     done, itb = {}, 0

@@ -134,7 +134,7 @@ class Server:
     def __init__(self, device: int = 0, max_slots: int = 64, iters_per_step: int = 16,
                  hbm_budget: float = 64e9, prep_workers: int = 8, precision: int = MIXED,
                  tol: float = 1e-10, maxit: int = 50000, mode: str = "batched",
-                 keep_x: bool = False):
+                 keep_x: bool = False, progress=None):
         self.device = device
         self.max_slots = max_slots
         self.iters_per_step = iters_per_step
@@ -145,6 +145,7 @@ class Server:
         self.maxit = maxit
         self.mode = mode
         self.keep_x = keep_x
+        self.progress = progress  # a file (e.g. sys.stderr) for a status line every 5 s
 
     def run(self, requests: list[Request], realtime: bool = True) -> dict:
         """Serves the stream; realtime=False makes every request arrive at 0."""
@@ -173,9 +174,16 @@ class Server:
             sysm.ctx.close()
             done.append(req)
 
+        last = 0.0
         try:
             while len(done) < len(reqs):
                 t = now()
+                if self.progress is not None and t - last >= 5.0:
+                    last = t
+                    print("serving %.1f s: done %d/%d, running %d, ready %d, preparing %d, "
+                          "waiting %d" % (t, len(done), len(reqs), len(running), len(ready),
+                                          len(preparing), len(pending)),
+                          file=self.progress, flush=True)
                 # admission by HBM footprint, in arrival order
                 while pending and pending[0].arrival <= t:
                     r = pending[0]
