@@ -38,7 +38,8 @@ class Stats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("outer", C.c_int32), ("status", C.c_int32),
                 ("spmv_count", C.c_int32), ("xkx", C.c_double), ("norm_x", C.c_double),
                 ("norm_r", C.c_double), ("norm_m_r", C.c_double), ("conv", C.c_double),
-                ("rel_res", C.c_double), ("t_device_ms", C.c_double), ("t_total_ms", C.c_double)]
+                ("rel_res", C.c_double), ("t_device_ms", C.c_double), ("t_total_ms", C.c_double),
+                ("op64_from", C.c_int32), ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -39,7 +39,7 @@ struct BInst {
   double* epart;
   double *x, *r, *p, *q, *b, *t;
   const double* m;
-  double* part;   // 3 * nch partials: p.q, r.Mr, true-residual r.Mr
+  double* part;   // 5 * nch partials: p.q, r.Mr, true-residual r.Mr, p.r, (Mr).q
   double* dslot;  // 2 ping-pong slots of r.Mr
   SolveScalars* sc;
   int n, nb, nch, fp32, acc32;
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
                                                  const uint32_t* __restrict__ work,
                                                  const int32_t* __restrict__ count, int mode) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int flag;
+  __shared__ int flag, op32;
   const int nw = *count;
   for (int w = blockIdx.x; w < nw; w += gridDim.x) {
     const uint32_t e = work[w];
@@ -78,10 +78,11 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
     if (threadIdx.x == 0) {
       const SolveScalars* sc = I.sc;
       flag = mode == 0 ? !sc->done : (sc->upd && !sc->done);
+      op32 = I.fp32 && !sc->op64;  // the progress guard may hand over to FP64
     }
     __syncthreads();
     if (!flag) continue;
-    const bool f32 = mode == 0 && I.fp32;
+    const bool f32 = mode == 0 && op32;
     if (mode == 0) {
       if (I.nb == 3) {
         if (f32 && I.acc32) block_tile<float, float, 3, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
@@ -123,19 +124,25 @@ __global__ __launch_bounds__(kThreads) void k_bt_merge_dot(const BInst* __restri
     const int c = wl_index(e);
     if (uniform_done(I, sh)) continue;
     const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
-    double acc = 0.0;
+    double acc = 0.0, apr = 0.0;
     for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
       double s = I.q[i];
       s = merge_row(I.mg, i, s);
       I.q[i] = s;
       acc += I.p[i] * s;
+      if (I.fp32) apr += I.p[i] * I.r[i];
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) I.part[c] = s;
+    if (I.fp32) {  // flexible recurrences of the mixed method: p.r
+      const double t = block_sum(apr, sh);
+      if (threadIdx.x == 0) I.part[3 * I.nch + c] = t;
+    }
   }
 }
 
-// alpha = d / p.q ; x += alpha p ; r -= alpha q ; partial r.(m r)
+// alpha = d / p.q (mixed: p.r / p.q) ; x += alpha p ; r -= alpha q ;
+// partials r.(m r) and (mixed) (m r).q
 __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict__ insts,
                                                         const uint32_t* __restrict__ work,
                                                         const int32_t* __restrict__ count,
@@ -148,17 +155,24 @@ __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict_
     const int c = wl_index(e);
     if (uniform_done(I, sh)) continue;
     const double pq = inst_sum(I.part, I.nch, sh);
-    const double alpha = I.dslot[cur] / pq;
+    const double num = I.fp32 ? inst_sum(I.part + 3 * I.nch, I.nch, sh) : I.dslot[cur];
+    const double alpha = num / pq;
     const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
-    double acc = 0.0;
+    double acc = 0.0, azq = 0.0;
     for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
       I.x[i] = I.x[i] + alpha * I.p[i];
       const double ri = I.r[i] - alpha * I.q[i];
       I.r[i] = ri;
-      acc += ri * (ri * I.m[i]);
+      const double zi = ri * I.m[i];
+      acc += ri * zi;
+      azq += zi * I.q[i];
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) I.part[I.nch + c] = s;
+    if (I.fp32) {
+      const double t = block_sum(azq, sh);
+      if (threadIdx.x == 0) I.part[4 * I.nch + c] = t;
+    }
     if (c == 0 && threadIdx.x == 0) {
       I.sc->alpha = alpha;
       I.sc->pq = pq;
@@ -166,7 +180,8 @@ __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict_
   }
 }
 
-// beta = r.Mr / d ; p = M r + beta p ; chunk 0 keeps the books
+// beta = r.Mr / d (mixed: -(Mr).q / p.q) ; p = M r + beta p ; chunk 0 keeps
+// the books
 __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict__ insts,
                                                          const uint32_t* __restrict__ work,
                                                          const int32_t* __restrict__ count,
@@ -179,7 +194,8 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
     const int c = wl_index(e);
     if (uniform_done(I, sh)) continue;
     const double di = inst_sum(I.part + I.nch, I.nch, sh);
-    const double beta = di / I.dslot[cur];
+    const double beta =
+        I.fp32 ? -inst_sum(I.part + 4 * I.nch, I.nch, sh) / I.sc->pq : di / I.dslot[cur];
     const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
     for (int i = i0 + threadIdx.x; i < i1; i += kThreads) I.p[i] = I.r[i] * I.m[i] + beta * I.p[i];
     if (c == 0 && threadIdx.x == 0) {
@@ -191,7 +207,8 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
       const double rn = sqrt(fabs(di));
       if (!sc->reliable) {
         if (rn < sc->conv) sc->done = 1;
-      } else if (rn < sc->delta * sc->rmax || rn < sc->conv) {
+      } else if (rn < sc->delta * sc->rmax || rn < sc->conv ||
+                 sc->iter - sc->last_upd >= kForceReplace) {
         sc->upd = 1;
         sc->skip64 = 0;
       }
@@ -244,6 +261,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_finish(const BInst* __restr
       SolveScalars* sc = I.sc;
       sc->true_rmr = rz;
       sc->nupd += 1;
+      progress_guard(sc, sqrt(fabs(rz)));
       if (sqrt(fabs(rz)) < sc->conv) {
         sc->done = 1;
         sc->status = 0;
@@ -298,20 +316,21 @@ __global__ void k_bt_init2(const BInst* __restrict__ insts, int slot, double tol
     sc->upd = 0;
     sc->skip64 = 1;
     sc->nupd = 0;
+    progress_init(sc, sqrt(fabs(d)));
     sc->done = (d == 0.0) ? 1 : 0;
   }
 }
 
 // done flags / iterations of the active slots for the host
 struct BStatus {
-  int32_t done, status, iter, nupd;
+  int32_t done, status, iter, nupd, op64_from;
 };
 __global__ void k_bt_status(const BInst* __restrict__ insts, const int32_t* __restrict__ slots,
                             const int32_t* __restrict__ count, BStatus* out) {
   const int ns = *count;
   for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < ns; w += gridDim.x * blockDim.x) {
     const SolveScalars* sc = insts[slots[w]].sc;
-    out[w] = BStatus{sc->done, sc->status, sc->iter, sc->nupd};
+    out[w] = BStatus{sc->done, sc->status, sc->iter, sc->nupd, sc->op64_from};
   }
 }
 
@@ -520,7 +539,7 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   if (maxit <= 0) return ELASTIC_BAD_ARG;
   if (tol <= 0.0) tol = 2.220446049250313e-16;
   const int nch = (c->neq + kChunk - 1) / kChunk;
-  if (3 * nch > 8 * kRedBlocks) return berr(b, ELASTIC_BAD_ARG, "system too large for a batch slot");
+  if (5 * nch > 8 * kRedBlocks) return berr(b, ELASTIC_BAD_ARG, "system too large for a batch slot");
   if (c->blk.ntiles >= (1 << 24) || nch >= (1 << 24)) return ELASTIC_BAD_ARG;
   int k = -1;
   for (int i = 0; i < b->max_slots; ++i)
@@ -632,6 +651,7 @@ int elastic_batch_poll(elastic_batch* b, int64_t* tags, elastic_stats* stats, in
       o.outer = st.nupd;
       o.status = st.status;
       o.spmv_count = st.iter + st.nupd;
+      o.op64_from = st.op64_from;
       o.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
                                                               S.t0).count();
       // post-solve figures on the instance's own context (FP64 operator)

@@ -570,6 +570,7 @@ static int solve_pcg(Context* c, int precond, int precision, double tol, int max
   st->t_device_ms = ms;
   st->iters = h.iter;
   st->outer = precision == ELASTIC_MIXED ? h.nupd : 1;
+  st->op64_from = precision == ELASTIC_MIXED ? h.op64_from : -1;
   st->spmv_count = h.iter + (precision == ELASTIC_MIXED ? h.nupd : 0);
   st->conv = h.conv;
   if (!h.done) st->status = ELASTIC_MAXIT;
@@ -610,6 +611,7 @@ static int solve_other(Context* c, int solver, int precond, int precision, doubl
   st->t_device_ms = ms;
   st->iters = kr.iters;
   st->outer = kr.outer;
+  st->op64_from = mixed ? kr.op64_from : -1;
   st->status = kr.status;
   st->conv = kr.conv;
   st->spmv_count = solver == ELASTIC_SOLVER_BICGSTAB ? 2 * kr.iters + kr.nupd : kr.iters + kr.outer;

@@ -59,6 +59,31 @@ __device__ inline double apply_m(int i, int n, const double* __restrict__ r,
   return a[4] * x1 + a[5] * x2 + a[2] * x3;
 }
 
+// Progress guard of the mixed methods, run at every FP64 true-residual
+// replacement with the new true residual norm tr.  When it has not dropped
+// below 0.9 x the previous one at three replacements in a row, the FP32
+// operator is not accurate enough for this system (its perturbation times
+// the condition number approaches 1), and the solve continues with the FP64
+// operator from the current x.  A replacement is also forced after
+// kForceReplace iterations without one, so stagnation is always seen.
+constexpr int kForceReplace = 500;
+__device__ inline void progress_guard(SolveScalars* sc, double tr) {
+  sc->stall = tr > 0.9 * sc->last_true ? sc->stall + 1 : 0;
+  sc->last_true = tr;
+  if (sc->stall >= 3 && !sc->op64) {
+    sc->op64 = 1;
+    sc->op64_from = sc->iter;
+  }
+  sc->last_upd = sc->iter;
+}
+__device__ inline void progress_init(SolveScalars* sc, double r0) {
+  sc->last_true = r0;
+  sc->stall = 0;
+  sc->op64 = 0;
+  sc->op64_from = -1;
+  sc->last_upd = 0;
+}
+
 #define GRID_LOOP(i, n) \
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < (n); i += gridDim.x * kThreads)
 

@@ -128,6 +128,9 @@ struct SolveScalars {
   int32_t reliable, upd, skip64, nupd;
   // BiCGSTAB omega; GMRES: |e(i)| of the last inner step and its norm h(i+1,i)
   double omega, gm_e, gm_h;
+  // progress guard of the mixed methods (device_util.h: progress_guard)
+  double last_true;
+  int32_t stall, op64, op64_from, last_upd;
 };
 
 constexpr int kRedBlocks = 1024;  // fixed reduction grid (deterministic sums)
@@ -212,7 +215,7 @@ int post_solve_stats(Context* c, elastic_stats* st);
 // then recomputes the true residual (BiCGSTAB: reliable replacement as in
 // PCG; GMRES: every restart) and convergence is decided on it.
 struct KrylovResult {
-  int iters = 0, outer = 0, nupd = 0, status = 0;
+  int iters = 0, outer = 0, nupd = 0, status = 0, op64_from = -1;
   bool done = false;
   double conv = 0.0;
 };

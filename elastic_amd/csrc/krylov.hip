@@ -114,6 +114,7 @@ __global__ void k_bi_init2(const double* __restrict__ part, double tol, double d
     sc->upd = 0;
     sc->skip64 = 1;
     sc->nupd = 0;
+    progress_init(sc, sqrt(fabs(d)));
     sc->done = (d == 0.0) ? 1 : 0;
   }
 }
@@ -236,7 +237,8 @@ __global__ void k_bi_check(const double* __restrict__ part_rr, const double* __r
     const double rn = sqrt(fabs(d));
     if (!sc->reliable) {
       if (rn < sc->conv) sc->done = 1;
-    } else if (rn < sc->conv || rn < sc->delta * sc->rmax) {
+    } else if (rn < sc->conv || rn < sc->delta * sc->rmax ||
+               sc->iter - sc->last_upd >= kForceReplace) {
       sc->upd = 1;
       sc->skip64 = 0;
     }
@@ -279,6 +281,7 @@ __global__ void k_bi_rfinish(const double* __restrict__ part_rr, const double* _
   if (threadIdx.x == 0) {
     sc->nupd += 1;
     sc->true_rmr = d;
+    progress_guard(sc, sqrt(fabs(d)));
     if (sqrt(fabs(d)) < sc->conv) {
       sc->done = 1;
     } else {
@@ -338,19 +341,20 @@ int bicgstab_run(Context* c, bool bdiag, bool fp32_operator, double tol, int max
   int it = 0;
   const int every = std::max(1, c->check_every);
   bool done = false;
+  bool op32 = fp32_operator;  // the progress guard may hand over to the FP64 operator
   while (it < maxit && !done) {
     const int chunk = std::min(every, maxit - it);
     for (int j = 0; j < chunk; ++j, ++it) {
       double* rr_cur = rr + (it & 1);
       double* rr_next = rr + ((it + 1) & 1);
-      launch_spmv(c, fp32_operator, z, v, c->epart64, &sc->done);
+      launch_spmv(c, op32, z, v, c->epart64, &sc->done);
       hipLaunchKernelGGL(k_merge_dot2, G, TB, 0, s, n, merge_of(c, c->epart64), v, r0,
                          P[0], (const double*)nullptr, P[1], sc);
       hipLaunchKernelGGL(k_bi_half1, G, TB, 0, s, n, P[0], rr_cur, x, z, r, v, m, jac, P[1], sc);
       if (!jac) hipLaunchKernelGGL(k_apply_m, G, TB, 0, s, n, r, m, bm, z, nullptr, sc);
       hipLaunchKernelGGL(k_bi_check, dim3(1), TB, 0, s, P[1], P[1], 1, rr_next, sc);
       if (reliable) replace(true, nullptr);
-      launch_spmv(c, fp32_operator, z, t, c->epart64, &sc->done);
+      launch_spmv(c, op32, z, t, c->epart64, &sc->done);
       hipLaunchKernelGGL(k_merge_dot2, G, TB, 0, s, n, merge_of(c, c->epart64), t, r,
                          P[2], (const double*)t, P[3], sc);
       hipLaunchKernelGGL(k_bi_half2, G, TB, 0, s, n, P[2], P[3], x, z, r, t, r0, P[4], P[5], sc);
@@ -363,6 +367,7 @@ int bicgstab_run(Context* c, bool bdiag, bool fp32_operator, double tol, int max
                                         hipMemcpyDeviceToHost, s));
     ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
     done = c->scal_host->done != 0;
+    if (c->scal_host->op64) op32 = false;
   }
   ELASTIC_HIP_CHECK(c, hipGetLastError());
   const SolveScalars& h = *c->scal_host;
@@ -372,6 +377,7 @@ int bicgstab_run(Context* c, bool bdiag, bool fp32_operator, double tol, int max
   out->done = done && h.status == 0;
   out->status = h.status ? h.status : (done ? ELASTIC_OK : ELASTIC_MAXIT);
   out->conv = h.conv;
+  out->op64_from = h.op64_from;
   return ELASTIC_OK;
 }
 
@@ -578,6 +584,10 @@ int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit,
   out->conv = econv;
   int nit = 0, cycles = 0;
   bool conv = hs->gm_h == 0.0;
+  bool op32 = fp32_operator;  // progress guard on the FP64 restart residuals
+  double last_true = hs->gm_h;
+  int stall = 0;
+  out->op64_from = -1;
   for (int l = 1; l <= maxit && !conv; ++l) {
     // g1 = M (b - A x), FP64 operator (the true residual)
     launch_spmv(c, false, x, t, c->epart64);
@@ -591,6 +601,12 @@ int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit,
         conv = true;
         break;
       }
+      stall = hs->gm_h > 0.9 * last_true ? stall + 1 : 0;
+      last_true = hs->gm_h;
+      if (op32 && stall >= 2) {
+        op32 = false;
+        out->op64_from = nit;
+      }
     }
     hipLaunchKernelGGL(k_gm_scale, G, TB, 0, s, n, g(1), (const double*)a.e);
     ++cycles;
@@ -598,7 +614,7 @@ int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit,
     for (int i = 1; i <= k; ++i) {
       ++nit;
       ++ni;
-      launch_spmv(c, fp32_operator, g(i), t, c->epart64);
+      launch_spmv(c, op32, g(i), t, c->epart64);
       hipLaunchKernelGGL(k_gm_merge, G, TB, 0, s, n, merge_of(c, c->epart64), t, u);
       double* w = g(i + 1);
       hipLaunchKernelGGL(k_gm_mdot, G, TB, 0, s, n, u, m, bm, w, g(1), P[0]);

@@ -62,49 +62,66 @@ __global__ void k_pcg_init2(const double* __restrict__ part, double tol, double 
     sc->upd = 0;
     sc->skip64 = 1;
     sc->nupd = 0;
+    progress_init(sc, sqrt(fabs(d)));
     sc->done = (d == 0.0) ? 1 : 0;  // b = 0: x = 0 is the solution
   }
 }
 
-// q += merged external partials; partial p.q
+// q += merged external partials; partial p.q, and p.r for the flexible
+// recurrences (when part_pr != null)
 __global__ __launch_bounds__(kThreads) void k_merge_dot(int n, MergeArgs mg,
                                                          double* q, const double* __restrict__ p,
-                                                         double* part, const SolveScalars* sc) {
+                                                         double* part, const double* __restrict__ r,
+                                                         double* part_pr, const SolveScalars* sc) {
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
-  double acc = 0.0;
+  double acc = 0.0, apr = 0.0;
   GRID_LOOP(i, n) {
     double s = q[i];
     s = merge_row(mg, i, s);
     q[i] = s;
     acc += p[i] * s;
+    if (part_pr) apr += p[i] * r[i];
   }
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (part_pr) {
+    const double t = block_sum(apr, sh);
+    if (threadIdx.x == 0) part_pr[blockIdx.x] = t;
+  }
 }
 
-// alpha = d / p.q ; x += alpha p ; r -= alpha q ; with Jacobi also partial r.(m r)
+// alpha = d / p.q (or p.r / p.q) ; x += alpha p ; r -= alpha q ; with Jacobi
+// also partial r.(m r) and, for the flexible recurrences, (m r).q
 __global__ __launch_bounds__(kThreads) void k_pcg_update(int n, const double* __restrict__ part_pq,
+                                                          const double* __restrict__ part_pr,
                                                           const double* __restrict__ dslot,
                                                           double* x, const double* __restrict__ p,
                                                           double* r, const double* __restrict__ q,
                                                           const double* __restrict__ m,
                                                           int jacobi, double* part_rz,
-                                                          SolveScalars* sc) {
+                                                          double* part_zq, SolveScalars* sc) {
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
   const double pq = grid_sum(part_pq, sh);
-  const double alpha = *dslot / pq;
-  double acc = 0.0;
+  const double num = part_pr ? grid_sum(part_pr, sh) : *dslot;
+  const double alpha = num / pq;
+  double acc = 0.0, azq = 0.0;
   GRID_LOOP(i, n) {
     x[i] = x[i] + alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
-    acc += ri * (ri * m[i]);
+    const double zi = ri * m[i];
+    acc += ri * zi;
+    azq += zi * q[i];
   }
   if (jacobi) {
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) part_rz[blockIdx.x] = s;
+    if (part_zq) {
+      const double t = block_sum(azq, sh);
+      if (threadIdx.x == 0) part_zq[blockIdx.x] = t;
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->alpha = alpha;
@@ -112,22 +129,34 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int n, const double* __
   }
 }
 
-// partial r.(M^-1 r) for the block preconditioner (after r is complete)
+// partial r.(M^-1 r) for the block preconditioner (after r is complete), and
+// (M^-1 r).q for the flexible recurrences
 __global__ __launch_bounds__(kThreads) void k_rz(int n, const double* __restrict__ r,
                                                   const double* __restrict__ m,
                                                   const double* __restrict__ bm, double* part,
+                                                  const double* __restrict__ q, double* part_zq,
                                                   const int32_t* gate, const SolveScalars* sc) {
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
   if (gate && !block_flag(gate, sh)) return;
-  double acc = 0.0;
-  GRID_LOOP(i, n) acc += r[i] * apply_m(i, n, r, m, bm);
+  double acc = 0.0, azq = 0.0;
+  GRID_LOOP(i, n) {
+    const double zi = apply_m(i, n, r, m, bm);
+    acc += r[i] * zi;
+    if (part_zq) azq += zi * q[i];
+  }
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (part_zq) {
+    const double t = block_sum(azq, sh);
+    if (threadIdx.x == 0) part_zq[blockIdx.x] = t;
+  }
 }
 
-// di = r.z ; beta = di / d ; p = z + beta p ; bookkeeping in block 0
+// di = r.z ; beta = di / d (or -z.q / p.q) ; p = z + beta p ; bookkeeping in
+// block 0
 __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* __restrict__ part_rz,
+                                                           const double* __restrict__ part_zq,
                                                            const double* __restrict__ dold,
                                                            double* dnew, double* p,
                                                            const double* __restrict__ r,
@@ -137,7 +166,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
   const double di = grid_sum(part_rz, sh);
-  const double beta = di / *dold;
+  const double beta = part_zq ? -grid_sum(part_zq, sh) / sc->pq : di / *dold;
   GRID_LOOP(i, n) p[i] = apply_m(i, n, r, m, bm) + beta * p[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *dnew = di;
@@ -147,7 +176,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
     const double rn = sqrt(fabs(di));
     if (!sc->reliable) {
       if (rn < sc->conv) sc->done = 1;
-    } else if (rn < sc->delta * sc->rmax || rn < sc->conv) {
+    } else if (rn < sc->delta * sc->rmax || rn < sc->conv ||
+               sc->iter - sc->last_upd >= kForceReplace) {
       sc->upd = 1;  // true-residual replacement at the end of this chunk
       sc->skip64 = 0;
     }
@@ -194,6 +224,7 @@ __global__ void k_rel_finish(const double* __restrict__ part, double* dslot, Sol
   if (threadIdx.x == 0) {
     sc->true_rmr = rz;
     sc->nupd += 1;
+    progress_guard(sc, sqrt(fabs(rz)));
     if (sqrt(fabs(rz)) < sc->conv) {
       sc->done = 1;
     } else {
@@ -213,6 +244,9 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   const dim3 G(kRedBlocks), TB(kThreads);
   double* part_a = c->red64;
   double* part_b = c->red64 + kRedBlocks;
+  // flexible recurrences with the FP32 operator and residual replacement
+  double* part_pr = reliable ? c->red64 + 2 * kRedBlocks : nullptr;
+  double* part_zq = reliable ? c->red64 + 3 * kRedBlocks : nullptr;
   const int jac = V.bm == nullptr;
   SolveScalars* sc = c->scal;
   hipLaunchKernelGGL(k_pcg_init, G, TB, 0, s, n, V.b, V.m, V.bm, V.x, V.r, V.p, part_a);
@@ -223,25 +257,30 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
     launch_spmv(c, false, V.x, V.t, V.epart, &sc->skip64);
     hipLaunchKernelGGL(k_rel_resid, G, TB, 0, s, n, merge_of(c, V.epart),
                        V.t, V.b, V.m, V.r, jac, part_b, sc);
-    if (!jac) hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, &sc->upd, sc);
+    if (!jac)
+      hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)nullptr,
+                         (double*)nullptr, &sc->upd, sc);
     hipLaunchKernelGGL(k_rel_finish, dim3(1), TB, 0, s, part_b, dnext, sc);
   };
   int it = 0;
   const int every = std::max(2, c->check_every & ~1);
   bool done = false;
+  bool op32 = fp32_operator;  // the progress guard may hand over to the FP64 operator
   while (it < maxit) {
     const int chunk = std::min(every, maxit - it);
     for (int j = 0; j < chunk; ++j, ++it) {
       double* dcur = dslots + (it & 1);
       double* dnxt = dslots + ((it + 1) & 1);
-      launch_spmv(c, fp32_operator, V.p, V.q, V.epart, &sc->done);
-      hipLaunchKernelGGL(k_merge_dot, G, TB, 0, s, n, merge_of(c, V.epart),
-                         V.q, V.p, part_a, sc);
-      hipLaunchKernelGGL(k_pcg_update, G, TB, 0, s, n, part_a, dcur, V.x, V.p, V.r, V.q, V.m, jac,
-                         part_b, sc);
-      if (!jac) hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, nullptr, sc);
-      hipLaunchKernelGGL(k_pcg_pupdate, G, TB, 0, s, n, part_b, dcur, dnxt, V.p, V.r, V.m, V.bm,
-                         sc);
+      launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done);
+      hipLaunchKernelGGL(k_merge_dot, G, TB, 0, s, n, merge_of(c, V.epart), V.q, V.p, part_a,
+                         (const double*)V.r, part_pr, sc);
+      hipLaunchKernelGGL(k_pcg_update, G, TB, 0, s, n, part_a, (const double*)part_pr, dcur, V.x,
+                         V.p, V.r, V.q, V.m, jac, part_b, part_zq, sc);
+      if (!jac)
+        hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)V.q,
+                           part_zq, (const int32_t*)nullptr, sc);
+      hipLaunchKernelGGL(k_pcg_pupdate, G, TB, 0, s, n, part_b, (const double*)part_zq, dcur, dnxt,
+                         V.p, V.r, V.m, V.bm, sc);
       // the replacement kernels gate themselves on the flag set by k_pcg_pupdate
       if (reliable && ((it + 1) & 1) == 0) replace(it + 1);
     }
@@ -252,6 +291,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
       done = true;
       break;
     }
+    if (c->scal_host->op64) op32 = false;
   }
   if (!done && reliable) {  // maxit: leave the true residual of the final x
     hipLaunchKernelGGL(k_rel_force, dim3(1), dim3(64), 0, s, sc);

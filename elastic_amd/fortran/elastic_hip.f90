@@ -22,6 +22,7 @@ module elastic_hip
   type, bind(C) :: elastic_stats
     integer(c_int32_t) :: iters, outer, status, spmv_count
     real(c_double) :: xkx, norm_x, norm_r, norm_m_r, conv, rel_res, t_device_ms, t_total_ms
+    integer(c_int32_t) :: op64_from, reserved
   end type elastic_stats
 
   interface

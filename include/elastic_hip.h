@@ -62,6 +62,9 @@ typedef struct {
   double rel_res;       /* norm_m_r / ||b||_M */
   double t_device_ms;   /* device time of the iterations (HIP events, inputs resident) */
   double t_total_ms;    /* wall time of the call incl. b upload and x download */
+  int32_t op64_from;    /* MIXED: iteration from which the FP64 operator took over
+                           because the FP32 one stopped making progress; -1: never */
+  int32_t reserved;
 } elastic_stats;
 
 /* ------------------------------------------------------------ context */
