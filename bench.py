@@ -192,29 +192,29 @@ def main():
 
 def serving_bench(nreq: int, ncompare: int):
     """The plane-strain request stream (half quad4, half tria3, side U{32..512},
-    Poisson arrivals at 200 requests/s) served by the batched mixed-precision
-    PCG with continuous admission: throughput and p50/p99 latency from arrival
-    to x on the host (host set-up and GPU assembly included).  Then the
-    solve-only comparison on the first `ncompare` requests with every system
-    resident: batched mixed-precision PCG against plain FP64 PCG solved one
-    after the other (the drop-in path), both to the same tolerance."""
-    from elastic_amd.serving import Batch, Server, request_stream
+    Poisson arrivals at 200 requests/s) served by the batch with continuous
+    admission, once with FP64 and once with mixed-precision instances:
+    throughput and p50/p99 latency from arrival to x on the host (host set-up
+    and GPU assembly included).  Then the solve-only comparison on the first
+    `ncompare` requests with every system resident: plain FP64 PCG solved one
+    after the other (the drop-in path) against the batch, FP64 and mixed."""
+    from elastic_amd.serving import Batch, MeshRequest, Server, request_stream
     from elastic_amd.solver import FP64, MIXED
     from elastic_amd.system import System
 
     workers = min(16, os.cpu_count() or 1)
-    srv = Server(max_slots=64, iters_per_step=16, prep_workers=workers, precision=MIXED,
-                 tol=TOL, hbm_budget=64e9, progress=sys.stderr)
-    a = time.perf_counter()
-    res = srv.run(request_stream(nreq, rate=200.0), realtime=True)
-    print("serving stream: %d requests in %.1f s" % (res["requests"], time.perf_counter() - a),
-          file=sys.stderr, flush=True)
-    stream = {k: v for k, v in res.items() if k != "done"}
-    stream["workload"] = ("%d plane-strain requests (even quad4, odd tria3, side U{32..512}), "
-                          "Poisson arrivals 200/s, mixed precision to %g" % (nreq, TOL))
-
-    # solve-only comparison on resident systems
-    from elastic_amd.serving import MeshRequest
+    out = {"workload": ("%d plane-strain requests (even quad4, odd tria3, side U{32..512}), "
+                        "Poisson arrivals 200/s, to %g" % (nreq, TOL))}
+    for name, prec in (("stream_batched_fp64", FP64), ("stream_batched_mixed", MIXED)):
+        srv = Server(max_slots=64, iters_per_step=16, prep_workers=workers, precision=prec,
+                     tol=TOL, hbm_budget=64e9, progress=sys.stderr)
+        a = time.perf_counter()
+        res = srv.run(request_stream(nreq, rate=200.0), realtime=True)
+        print("%s: %d requests in %.1f s" % (name, res["requests"], time.perf_counter() - a),
+              file=sys.stderr, flush=True)
+        out[name] = {k: v for k, v in res.items() if k != "done"}
+        out[name]["fp64_operator_handovers"] = int(sum(r.stats.get("op64_from", -1) >= 0
+                                                       for r in res["done"]))
 
     print("solve-only comparison: preparing %d systems" % ncompare, file=sys.stderr, flush=True)
     systems = []
@@ -232,28 +232,31 @@ def serving_bench(nreq: int, ncompare: int):
     t_seq = time.perf_counter() - a
     print("sequential fp64: %.1f s" % t_seq, file=sys.stderr, flush=True)
     x64 = [s_.ctx.download_x() for s_ in systems]
-    a = time.perf_counter()
-    done, itb = {}, 0
-    with Batch(0, max_slots=64, iters_per_step=16) as b:
-        queue = list(range(len(systems)))
-        while len(done) < len(systems):
-            while queue and b.submit(systems[queue[0]].ctx, MIXED, TOL, 50000, queue[0]):
-                queue.pop(0)
-            b.step()
-            for tag, st in b.poll():
-                done[tag] = st
-                itb += st["iters"]
-    t_bat = time.perf_counter() - a
-    err = max(float(np.linalg.norm(s_.ctx.download_x() - x64[i]) / np.linalg.norm(x64[i]))
-              for i, s_ in enumerate(systems))
+    compare = {"requests": ncompare, "dof": dof,
+               "sequential_fp64": {"s": t_seq, "dof_per_s": dof / t_seq, "iters": it64}}
+    for name, prec in (("batched_fp64", FP64), ("batched_mixed", MIXED)):
+        a = time.perf_counter()
+        done, itb = {}, 0
+        with Batch(0, max_slots=64, iters_per_step=16) as b:
+            queue = list(range(len(systems)))
+            while len(done) < len(systems):
+                while queue and b.submit(systems[queue[0]].ctx, prec, TOL, 50000, queue[0]):
+                    queue.pop(0)
+                b.step()
+                for tag, st in b.poll():
+                    done[tag] = st
+                    itb += st["iters"]
+        t_bat = time.perf_counter() - a
+        err = max(float(np.linalg.norm(s_.ctx.download_x() - x64[i]) / np.linalg.norm(x64[i]))
+                  for i, s_ in enumerate(systems))
+        print("%s: %.1f s" % (name, t_bat), file=sys.stderr, flush=True)
+        compare[name] = {"s": t_bat, "dof_per_s": dof / t_bat, "iters": itb,
+                         "speedup_vs_sequential_fp64": t_seq / t_bat,
+                         "max_rel_err_vs_fp64": err}
     for s_ in systems:
         s_.ctx.close()
-    compare = {"requests": ncompare, "dof": dof,
-               "sequential_fp64": {"s": t_seq, "dof_per_s": dof / t_seq, "iters": it64},
-               "batched_mixed": {"s": t_bat, "dof_per_s": dof / t_bat, "iters": itb},
-               "speedup": t_seq / t_bat,
-               "max_rel_err_vs_fp64": err}
-    return {"stream": stream, "solve_only": compare}
+    out["solve_only"] = compare
+    return out
 
 
 def torch_sync():
