@@ -52,7 +52,7 @@ static void free_tiles(DevTiles& t) {
 static void free_blocks(DevBlocks& b) {
   void* ptrs[] = {b.row0,   b.nrows,  b.nblk, b.ncols,   b.ext0,    b.seg0, b.rp0,
                   b.blk0,   b.word,   b.rowptr, b.segptr, b.ext, b.inc_ptr, b.inc_idx,
-                  b.dpos,   b.ia,     b.v64,  b.d64,     b.v32, b.d32};
+                  b.ia,     b.v64,  b.d64,     b.v32, b.d32};
   for (void* p : ptrs)
     if (p) hipFree(p);
   b = DevBlocks();
@@ -151,7 +151,7 @@ static int build_layout(elastic_ctx* x) {
     rc = rc ? rc : upload(c, &b.segptr, L.segptr.data(), L.segptr.size());
     rc = rc ? rc : upload(c, &b.ext, L.ext.data(), L.ext.size());
     rc = rc ? rc : upload(c, &b.inc_ptr, L.inc_ptr.data(), L.inc_ptr.size());
-    rc = rc ? rc : upload(c, &b.dpos, L.dpos.data(), L.dpos.size());
+    rc = rc ? rc : upload(c, &b.inc_idx, L.inc_idx.data(), L.inc_idx.size());
     rc = rc ? rc : upload(c, &b.ia, ia, (size_t)neq + 1);
     const size_t nvals = (size_t)L.nblocks * nb * nb, ndiag = (size_t)L.nn * (nb * (nb + 1) / 2);
     rc = rc ? rc : dalloc(c, &b.v64, nvals);
@@ -166,7 +166,7 @@ static int build_layout(elastic_ctx* x) {
     b.nblocks = L.nblocks;
     b.next_nodes = L.next_nodes;
     c->inc_ptr = b.inc_ptr;
-    c->inc_idx = nullptr;  // destination order: per-node pointers only
+    c->inc_idx = b.inc_idx;  // node-granular lists
     nepart = (int64_t)nb * L.next_nodes;
   }
   if (nb == 1) {  // FP32 operator of the scalar tiles (the blocked layout has its planes)

@@ -73,7 +73,6 @@ struct BlockLayout {
   std::vector<uint32_t> word;
   std::vector<uint16_t> rowptr, segptr;
   std::vector<int32_t> ext, inc_ptr, inc_idx;
-  std::vector<int32_t> dpos;  // destination slot of each (tile, external node) partial
 };
 
 // 3 (or 2) when the CSRC pattern is a pattern of node blocks, else 1
@@ -89,7 +88,7 @@ struct DevBlocks {
   int64_t* blk0 = nullptr;
   uint32_t* word = nullptr;
   uint16_t *rowptr = nullptr, *segptr = nullptr;
-  int32_t *ext = nullptr, *inc_ptr = nullptr, *inc_idx = nullptr, *dpos = nullptr;
+  int32_t *ext = nullptr, *inc_ptr = nullptr, *inc_idx = nullptr;
   int32_t* ia = nullptr;  // device copy of ia (value pack)
   // values: plane p of block q of tile t at blk0[t]*nb*nb + p*nblk[t] + q;
   // node diagonal blocks: entry c of node I at c*nn + I (nb diagonal entries,
@@ -176,8 +175,9 @@ struct Context {
 
 // How the external partials of a product reach their rows.
 //   nb == 1 (scalar tiles): row i adds epart[idx[k]], k in ptr[i]..ptr[i+1]-1
-//   nb > 1 (node blocks): the partials are stored in destination order, so
-//     row i = nb*J + r adds epart[nb*k + r], k in ptr[J]..ptr[J+1]-1
+//   nb > 1 (node blocks): the tiles write nb values per (tile, external
+//     node) in tile order; row i = nb*J + r adds epart[nb*idx[k] + r],
+//     k in ptr[J]..ptr[J+1]-1 (one index per partial node)
 // Either way the partials of a row are added in tile order.
 struct MergeArgs {
   const int32_t* ptr;
@@ -192,7 +192,7 @@ __device__ inline double merge_row(const MergeArgs& m, int i, double s) {
     return s;
   }
   const int J = i / m.nb, r = i - J * m.nb;
-  for (int k = m.ptr[J]; k < m.ptr[J + 1]; ++k) s += m.epart[(int64_t)m.nb * k + r];
+  for (int k = m.ptr[J]; k < m.ptr[J + 1]; ++k) s += m.epart[(int64_t)m.nb * m.idx[k] + r];
   return s;
 }
 

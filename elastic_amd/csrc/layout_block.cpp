@@ -176,15 +176,14 @@ void build_block_tiles(int neq, int nb, const int32_t* ia1, const int32_t* ja1, 
       if (r < L.nrows[t]) acc += nbr[L.row0[t] + r];
     }
   }
-  // destination order of the external partials: those of node J are
-  // contiguous (inc_ptr[J] .. inc_ptr[J+1]-1), in tile order
+  // merge lists at node granularity: node J receives the partials q (tile
+  // order) whose external node is J, listed at inc_ptr[J] .. inc_ptr[J+1]-1
   L.inc_ptr.assign((size_t)nn + 1, 0);
   for (int64_t q = 0; q < next; ++q) L.inc_ptr[(size_t)L.ext[q] + 1]++;
   for (int J = 0; J < nn; ++J) L.inc_ptr[J + 1] += L.inc_ptr[J];
-  L.dpos.resize(next);
+  L.inc_idx.resize(next);
   std::vector<int32_t> fill(L.inc_ptr.begin(), L.inc_ptr.end() - 1);
-  for (int64_t q = 0; q < next; ++q) L.dpos[q] = fill[L.ext[q]]++;
-  L.inc_idx.clear();
+  for (int64_t q = 0; q < next; ++q) L.inc_idx[fill[L.ext[q]]++] = (int32_t)q;
 }
 
 }  // namespace elastic

@@ -25,13 +25,12 @@ struct BlkArgs {
   const uint32_t* word;
   const uint16_t *rowptr, *segptr;
   const int32_t* ext;
-  const int32_t* dpos;
   int nn, nblk_max, ncols_max;
 };
 
 inline BlkArgs blk_args(const DevBlocks& b) {
-  return BlkArgs{b.row0,   b.nrows,  b.nblk, b.ncols, b.ext0, b.seg0,     b.rp0,        b.blk0,
-                 b.word,   b.rowptr, b.segptr, b.ext, b.dpos, b.nn, b.nblk_max, b.ncols_max};
+  return BlkArgs{b.row0, b.nrows,  b.nblk,   b.ncols, b.ext0, b.seg0,    b.rp0,        b.blk0,
+                 b.word, b.rowptr, b.segptr, b.ext,   b.nn,   b.nblk_max, b.ncols_max};
 }
 
 // Value planes of a tile: FP64 values are stored plane by plane (a lane's load
@@ -247,10 +246,8 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
 #pragma unroll
       for (int r = 0; r < NB; ++r) y[(int64_t)NB * (I0 + c) + r] = (double)s[r];
     } else {
-      // the partial of external node ext[c - nr] goes to its destination slot
-      const int64_t d = (int64_t)NB * a.dpos[ext0 + c - nr];
 #pragma unroll
-      for (int r = 0; r < NB; ++r) epart[d + r] = (double)s[r];
+      for (int r = 0; r < NB; ++r) epart[(int64_t)NB * (ext0 + c - nr) + r] = (double)s[r];
     }
   }
 }
