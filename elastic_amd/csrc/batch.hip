@@ -43,6 +43,7 @@ struct BInst {
   double* dslot;  // 2 ping-pong slots of r.Mr
   SolveScalars* sc;
   int n, nb, nch, fp32, acc32;
+  int live, ntiles;  // live: the slot holds an admitted instance (checked first)
 };
 
 // ---------------------------------------------------------------- kernels
@@ -61,14 +62,20 @@ __device__ inline double inst_sum(const double* __restrict__ part, int nch, doub
   return r;
 }
 
-// mode 0: q = A p with the instance's operator (FP32 values when fp32), skipped
-// once done; mode 1: t = A64 x, only for instances that asked for a true
-// residual (reliable replacement)
+// The product of one operator variant (value type TV, sum type TA, node block
+// NB) over its work list, so each variant compiles with the registers of the
+// single-system kernel.  mode 0: q = A p; the FP64 list holds every instance
+// and serves FP64 instances and mixed ones the progress guard handed over to
+// FP64, the FP32 lists hold the mixed instances that are still on FP32.
+// mode 1 (FP64 lists): t = A64 x for mixed instances that asked for a true
+// residual (reliable replacement).
+template <typename TV, typename TA, int NB, int MODE>
 __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts,
                                                  const uint32_t* __restrict__ work,
-                                                 const int32_t* __restrict__ count, int mode) {
+                                                 const int32_t* __restrict__ count) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int flag, op32;
+  __shared__ int flag;
+  constexpr bool f64 = sizeof(TV) == 8;
   const int nw = *count;
   for (int w = blockIdx.x; w < nw; w += gridDim.x) {
     const uint32_t e = work[w];
@@ -76,35 +83,66 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
     const int t = wl_index(e);
     __syncthreads();  // LDS of the previous tile is free, flag is free
     if (threadIdx.x == 0) {
-      const SolveScalars* sc = I.sc;
-      flag = mode == 0 ? !sc->done : (sc->upd && !sc->done);
-      op32 = I.fp32 && !sc->op64;  // the progress guard may hand over to FP64
+      if (!I.live || t >= I.ntiles) {
+        flag = 0;  // not a work item of an admitted instance
+      } else {
+        const SolveScalars* sc = I.sc;
+        if (MODE == 1) flag = I.fp32 && sc->upd && !sc->done;
+        else if (f64) flag = !sc->done && (!I.fp32 || sc->op64);
+        else flag = !sc->done && !sc->op64;
+      }
     }
     __syncthreads();
     if (!flag) continue;
-    const bool f32 = mode == 0 && op32;
-    if (mode == 0) {
-      if (I.nb == 3) {
-        if (f32 && I.acc32) block_tile<float, float, 3, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
-        else if (f32) block_tile<float, double, 3, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
-        else block_tile<double, double, 3, kBatchPer>(I.a, t, I.v64, I.d64, I.p, I.q, I.epart, smem);
-      } else {
-        if (f32 && I.acc32) block_tile<float, float, 2, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
-        else if (f32) block_tile<float, double, 2, kBatchPer>(I.a, t, I.v32, I.d32, I.p, I.q, I.epart, smem);
-        else block_tile<double, double, 2, kBatchPer>(I.a, t, I.v64, I.d64, I.p, I.q, I.epart, smem);
-      }
-    } else {
-      if (I.nb == 3) block_tile<double, double, 3, kBatchPer>(I.a, t, I.v64, I.d64, I.x, I.t, I.epart, smem);
-      else block_tile<double, double, 2, kBatchPer>(I.a, t, I.v64, I.d64, I.x, I.t, I.epart, smem);
-    }
+    const TV* vals = f64 ? (const TV*)I.v64 : (const TV*)I.v32;
+    const TV* diag = f64 ? (const TV*)I.d64 : (const TV*)I.d32;
+    if (MODE == 0) block_tile<TV, TA, NB, kBatchPer>(I.a, t, vals, diag, I.p, I.q, I.epart, smem);
+    else block_tile<TV, TA, NB, kBatchPer>(I.a, t, vals, diag, I.x, I.t, I.epart, smem);
   }
 }
 
-// reads an instance flag into a block-uniform value
-__device__ inline bool uniform_done(const BInst& I, double* sh, bool want_upd = false) {
+// operator variants: v = 3 * (nb == 3) + {0: FP64, 1: FP32 values FP64 sums,
+// 2: FP32 values and sums}
+constexpr int kVariants = 6;
+template <int MODE>
+static const void* variant_kernel(int v) {
+  switch (v) {
+    case 0: return (const void*)k_bt_spmv<double, double, 2, MODE>;
+    case 1: return (const void*)k_bt_spmv<float, double, 2, MODE>;
+    case 2: return (const void*)k_bt_spmv<float, float, 2, MODE>;
+    case 3: return (const void*)k_bt_spmv<double, double, 3, MODE>;
+    case 4: return (const void*)k_bt_spmv<float, double, 3, MODE>;
+    default: return (const void*)k_bt_spmv<float, float, 3, MODE>;
+  }
+}
+static void launch_variant(int v, dim3 G, dim3 TB, size_t lds, hipStream_t s, const BInst* insts,
+                           const uint32_t* work, const int32_t* count, int mode) {
+  if (mode == 1) {  // only the FP64 lists serve the true residuals
+    if (v == 0) hipLaunchKernelGGL((k_bt_spmv<double, double, 2, 1>), G, TB, lds, s, insts, work, count);
+    else hipLaunchKernelGGL((k_bt_spmv<double, double, 3, 1>), G, TB, lds, s, insts, work, count);
+    return;
+  }
+  switch (v) {
+    case 0: hipLaunchKernelGGL((k_bt_spmv<double, double, 2, 0>), G, TB, lds, s, insts, work, count); break;
+    case 1: hipLaunchKernelGGL((k_bt_spmv<float, double, 2, 0>), G, TB, lds, s, insts, work, count); break;
+    case 2: hipLaunchKernelGGL((k_bt_spmv<float, float, 2, 0>), G, TB, lds, s, insts, work, count); break;
+    case 3: hipLaunchKernelGGL((k_bt_spmv<double, double, 3, 0>), G, TB, lds, s, insts, work, count); break;
+    case 4: hipLaunchKernelGGL((k_bt_spmv<float, double, 3, 0>), G, TB, lds, s, insts, work, count); break;
+    default: hipLaunchKernelGGL((k_bt_spmv<float, float, 3, 0>), G, TB, lds, s, insts, work, count); break;
+  }
+}
+
+// reads an instance flag into a block-uniform value: true when the work item
+// (chunk c) is to be skipped; items of slots that hold no admitted instance
+// and chunks beyond the instance's size are always skipped
+__device__ inline bool uniform_done(const BInst& I, int c, double* sh, bool want_upd = false) {
   if (threadIdx.x == 0) {
-    const SolveScalars* sc = I.sc;
-    sh[5] = want_upd ? (double)(!sc->upd || sc->done) : (double)sc->done;
+    if (!I.live || c >= I.nch) {
+      sh[5] = 1.0;
+    } else {
+      const SolveScalars* sc = I.sc;
+      sh[5] = want_upd ? (double)(!sc->upd || sc->done) : (double)sc->done;
+    }
   }
   __syncthreads();
   const bool d = sh[5] != 0.0;
@@ -122,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_merge_dot(const BInst* __restri
     const uint32_t e = work[w];
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
-    if (uniform_done(I, sh)) continue;
+    if (uniform_done(I, c, sh)) continue;
     const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
     double acc = 0.0, apr = 0.0;
     for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
@@ -153,7 +191,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict_
     const uint32_t e = work[w];
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
-    if (uniform_done(I, sh)) continue;
+    if (uniform_done(I, c, sh)) continue;
     const double pq = inst_sum(I.part, I.nch, sh);
     const double num = I.fp32 ? inst_sum(I.part + 3 * I.nch, I.nch, sh) : I.dslot[cur];
     const double alpha = num / pq;
@@ -192,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
     const uint32_t e = work[w];
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
-    if (uniform_done(I, sh)) continue;
+    if (uniform_done(I, c, sh)) continue;
     const double di = inst_sum(I.part + I.nch, I.nch, sh);
     const double beta =
         I.fp32 ? -inst_sum(I.part + 4 * I.nch, I.nch, sh) / I.sc->pq : di / I.dslot[cur];
@@ -231,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_resid(const BInst* __restri
     const uint32_t e = work[w];
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
-    if (uniform_done(I, sh, true)) continue;
+    if (uniform_done(I, c, sh, true)) continue;
     const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
     double acc = 0.0;
     for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
@@ -255,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_finish(const BInst* __restr
   const int ns = *count;
   for (int w = blockIdx.x; w < ns; w += gridDim.x) {
     const BInst& I = insts[slots[w]];
-    if (uniform_done(I, sh, true)) continue;
+    if (uniform_done(I, 0, sh, true)) continue;
     const double rz = inst_sum(I.part + 2 * I.nch, I.nch, sh);
     if (threadIdx.x == 0) {
       SolveScalars* sc = I.sc;
@@ -329,7 +367,12 @@ __global__ void k_bt_status(const BInst* __restrict__ insts, const int32_t* __re
                             const int32_t* __restrict__ count, BStatus* out) {
   const int ns = *count;
   for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < ns; w += gridDim.x * blockDim.x) {
-    const SolveScalars* sc = insts[slots[w]].sc;
+    const BInst& I = insts[slots[w]];
+    if (!I.live) {
+      out[w] = BStatus{0, 0, 0, 0, -1};
+      continue;
+    }
+    const SolveScalars* sc = I.sc;
     out[w] = BStatus{sc->done, sc->status, sc->iter, sc->nupd, sc->op64_from};
   }
 }
@@ -357,11 +400,17 @@ struct elastic_batch {
   std::vector<Slot> slots;
   std::vector<BInst> h_insts;
   BInst* d_insts = nullptr;
-  uint32_t *d_swork = nullptr, *d_vwork = nullptr;
-  int32_t *d_slist = nullptr, *d_counts = nullptr;  // counts: spmv, vec, slots
-  size_t swork_cap = 0, vwork_cap = 0;
-  std::vector<uint32_t> h_swork, h_vwork;
+  // one device buffer per product variant: a captured graph holds these
+  // pointers, so a list may only change in place (a reallocation re-captures)
+  uint32_t* d_sw[kVariants] = {nullptr};
+  size_t sw_cap[kVariants] = {0};
+  uint32_t* d_vwork = nullptr;
+  // counts: the kVariants product lists, then vector items, then slots
+  int32_t *d_slist = nullptr, *d_counts = nullptr;
+  size_t vwork_cap = 0;
+  std::vector<uint32_t> h_sw[kVariants], h_vwork;
   std::vector<int32_t> h_slist, h_counts;
+  unsigned mask = 0, graph_mask = ~0u;  // non-empty variant lists (+ bit 8: mixed present)
   BStatus *d_status = nullptr, *h_status = nullptr;  // h_status pinned
   bool dirty = true, stepped = false;
   size_t lds = 0, graph_lds = 0;
@@ -390,38 +439,59 @@ static void enqueue_iteration(elastic_batch* b, int it) {
   hipStream_t s = b->stream;
   const dim3 G(kBatchGrid), TB(kThreads);
   const int cur = it & 1;
-  hipLaunchKernelGGL(k_bt_spmv, G, TB, b->lds, s, b->d_insts, b->d_swork, b->d_counts, 0);
-  hipLaunchKernelGGL(k_bt_merge_dot, G, TB, 0, s, b->d_insts, b->d_vwork, b->d_counts + 1);
-  hipLaunchKernelGGL(k_bt_update, G, TB, 0, s, b->d_insts, b->d_vwork, b->d_counts + 1, cur);
-  hipLaunchKernelGGL(k_bt_pupdate, G, TB, 0, s, b->d_insts, b->d_vwork, b->d_counts + 1, cur);
-  if (cur == 1) {  // every second iteration: true-residual replacement where asked
-    hipLaunchKernelGGL(k_bt_spmv, G, TB, b->lds, s, b->d_insts, b->d_swork, b->d_counts, 1);
-    hipLaunchKernelGGL(k_bt_rel_resid, G, TB, 0, s, b->d_insts, b->d_vwork, b->d_counts + 1);
+  const int32_t* vcount = b->d_counts + kVariants;
+  for (int v = 0; v < kVariants; ++v)
+    if (b->mask & (1u << v))
+      launch_variant(v, G, TB, b->lds, s, b->d_insts, b->d_sw[v], b->d_counts + v, 0);
+  hipLaunchKernelGGL(k_bt_merge_dot, G, TB, 0, s, b->d_insts, b->d_vwork, vcount);
+  hipLaunchKernelGGL(k_bt_update, G, TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
+  hipLaunchKernelGGL(k_bt_pupdate, G, TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
+  if (cur == 1 && (b->mask & 0x100u)) {  // every second iteration: true residuals where asked
+    for (int v = 0; v < kVariants; v += 3)  // the FP64 lists
+      if (b->mask & (1u << v))
+        launch_variant(v, G, TB, b->lds, s, b->d_insts, b->d_sw[v], b->d_counts + v, 1);
+    hipLaunchKernelGGL(k_bt_rel_resid, G, TB, 0, s, b->d_insts, b->d_vwork, vcount);
     hipLaunchKernelGGL(k_bt_rel_finish, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist,
-                       b->d_counts + 2, 0);
+                       vcount + 1, 0);
   }
 }
 
 static int set_lds_limit(elastic_batch* b) {
   if (b->lds > 64 * 1024)
-    BATCH_HIP(b, hipFuncSetAttribute((const void*)k_bt_spmv,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    for (int v = 0; v < kVariants; ++v) {
+      BATCH_HIP(b, hipFuncSetAttribute(variant_kernel<0>(v),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+      if (v % 3 == 0)
+        BATCH_HIP(b, hipFuncSetAttribute(variant_kernel<1>(v),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    }
   return ELASTIC_OK;
 }
 
 static int upload_lists(elastic_batch* b) {
-  b->h_swork.clear();
+  for (auto& l : b->h_sw) l.clear();
   b->h_vwork.clear();
   b->h_slist.clear();
   size_t lds = 0;
+  bool mixed = false;
   for (int k = 0; k < b->max_slots; ++k) {
     const auto& S = b->slots[k];
     if (!S.used) continue;
+    const BInst& I = b->h_insts[k];
     b->h_slist.push_back(k);
-    for (int t = 0; t < S.ntiles; ++t) b->h_swork.push_back(((uint32_t)k << 24) | (uint32_t)t);
+    const int v0 = I.nb == 3 ? 3 : 0;
+    for (int t = 0; t < S.ntiles; ++t) b->h_sw[v0].push_back(((uint32_t)k << 24) | (uint32_t)t);
+    if (I.fp32) {
+      mixed = true;
+      const int v = v0 + (I.acc32 ? 2 : 1);
+      for (int t = 0; t < S.ntiles; ++t) b->h_sw[v].push_back(((uint32_t)k << 24) | (uint32_t)t);
+    }
     for (int c = 0; c < S.nch; ++c) b->h_vwork.push_back(((uint32_t)k << 24) | (uint32_t)c);
     lds = std::max(lds, S.lds);
   }
+  unsigned mask = mixed ? 0x100u : 0u;
+  for (int v = 0; v < kVariants; ++v)
+    if (!b->h_sw[v].empty()) mask |= 1u << v;
   auto grow = [&](uint32_t*& p, size_t& cap, size_t need) -> int {
     if (need <= cap) return ELASTIC_OK;
     size_t nc = std::max(need, 2 * cap + 4096);
@@ -434,26 +504,32 @@ static int upload_lists(elastic_batch* b) {
     b->graph_lds = 0;  // pointers changed: capture again
     return ELASTIC_OK;
   };
-  int rc = grow(b->d_swork, b->swork_cap, b->h_swork.size());
+  int rc = 0;
+  for (int v = 0; v < kVariants && !rc; ++v) rc = grow(b->d_sw[v], b->sw_cap[v], b->h_sw[v].size());
   if (rc) return rc;
   rc = grow(b->d_vwork, b->vwork_cap, b->h_vwork.size());
   if (rc) return rc;
-  b->h_counts = {(int32_t)b->h_swork.size(), (int32_t)b->h_vwork.size(),
-                 (int32_t)b->h_slist.size()};
+  b->h_counts.assign(kVariants + 2, 0);
+  for (int v = 0; v < kVariants; ++v) b->h_counts[v] = (int32_t)b->h_sw[v].size();
+  b->h_counts[kVariants] = (int32_t)b->h_vwork.size();
+  b->h_counts[kVariants + 1] = (int32_t)b->h_slist.size();
   hipStream_t s = b->stream;
-  if (!b->h_swork.empty())
-    BATCH_HIP(b, hipMemcpyAsync(b->d_swork, b->h_swork.data(), b->h_swork.size() * 4,
-                                hipMemcpyHostToDevice, s));
+  for (int v = 0; v < kVariants; ++v)
+    if (!b->h_sw[v].empty())
+      BATCH_HIP(b, hipMemcpyAsync(b->d_sw[v], b->h_sw[v].data(), b->h_sw[v].size() * 4,
+                                  hipMemcpyHostToDevice, s));
   if (!b->h_vwork.empty())
     BATCH_HIP(b, hipMemcpyAsync(b->d_vwork, b->h_vwork.data(), b->h_vwork.size() * 4,
                                 hipMemcpyHostToDevice, s));
   if (!b->h_slist.empty())
     BATCH_HIP(b, hipMemcpyAsync(b->d_slist, b->h_slist.data(), b->h_slist.size() * 4,
                                 hipMemcpyHostToDevice, s));
-  BATCH_HIP(b, hipMemcpyAsync(b->d_counts, b->h_counts.data(), 3 * 4, hipMemcpyHostToDevice, s));
+  BATCH_HIP(b, hipMemcpyAsync(b->d_counts, b->h_counts.data(), b->h_counts.size() * 4,
+                              hipMemcpyHostToDevice, s));
   // the host vectors must outlive the copies: wait here (admission time only)
   BATCH_HIP(b, hipStreamSynchronize(s));
   if (lds > b->lds) b->lds = lds;
+  b->mask = mask;
   b->dirty = false;
   return ELASTIC_OK;
 }
@@ -470,6 +546,7 @@ static int capture(elastic_batch* b) {
   BATCH_HIP(b, hipStreamEndCapture(b->stream, &b->graph));
   BATCH_HIP(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
   b->graph_lds = b->lds;
+  b->graph_mask = b->mask;
   return ELASTIC_OK;
 }
 
@@ -492,7 +569,7 @@ int elastic_batch_create(elastic_batch** out, int32_t device, int32_t max_slots,
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&b->d_insts, sizeof(BInst) * max_slots) != hipSuccess ||
       hipMalloc((void**)&b->d_slist, sizeof(int32_t) * max_slots) != hipSuccess ||
-      hipMalloc((void**)&b->d_counts, sizeof(int32_t) * 4) != hipSuccess ||
+      hipMalloc((void**)&b->d_counts, sizeof(int32_t) * (kVariants + 2)) != hipSuccess ||
       hipMalloc((void**)&b->d_status, sizeof(BStatus) * max_slots) != hipSuccess ||
       hipHostMalloc((void**)&b->h_status, sizeof(BStatus) * max_slots) != hipSuccess) {
     elastic_batch_destroy(b);
@@ -513,8 +590,10 @@ int elastic_batch_destroy(elastic_batch* b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   if (b->gexec) hipGraphExecDestroy(b->gexec);
   if (b->graph) hipGraphDestroy(b->graph);
-  for (void* p : {(void*)b->d_insts, (void*)b->d_swork, (void*)b->d_vwork, (void*)b->d_slist,
-                  (void*)b->d_counts, (void*)b->d_status})
+  for (void* p : {(void*)b->d_insts, (void*)b->d_vwork, (void*)b->d_slist, (void*)b->d_counts,
+                  (void*)b->d_status})
+    if (p) hipFree(p);
+  for (uint32_t* p : b->d_sw)
     if (p) hipFree(p);
   if (b->h_status) hipHostFree(b->h_status);
   if (b->stream) hipStreamDestroy(b->stream);
@@ -575,6 +654,8 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   I.nch = nch;
   I.fp32 = precision == ELASTIC_MIXED;
   I.acc32 = c->accum32 ? 1 : 0;
+  I.live = 1;
+  I.ntiles = c->blk.ntiles;
   b->h_insts[k] = I;
   BATCH_HIP(b, hipMemcpyAsync(b->d_insts + k, &b->h_insts[k], sizeof(BInst), hipMemcpyHostToDevice,
                               b->stream));
@@ -610,7 +691,7 @@ int elastic_batch_step(elastic_batch* b) {
   }
   if (b->h_slist.empty()) return ELASTIC_OK;
   if (b->use_graph) {
-    if (!b->gexec || b->graph_lds < b->lds) {
+    if (!b->gexec || b->graph_lds < b->lds || b->graph_mask != b->mask) {
       int rc = capture(b);
       if (rc) return rc;
     }
@@ -621,7 +702,7 @@ int elastic_batch_step(elastic_batch* b) {
     for (int it = 0; it < b->iters; ++it) enqueue_iteration(b, it);
   }
   hipLaunchKernelGGL(k_bt_status, dim3(1), dim3(256), 0, b->stream, b->d_insts, b->d_slist,
-                     b->d_counts + 2, b->d_status);
+                     b->d_counts + kVariants + 1, b->d_status);
   BATCH_HIP(b, hipMemcpyAsync(b->h_status, b->d_status, sizeof(BStatus) * b->h_slist.size(),
                               hipMemcpyDeviceToHost, b->stream));
   b->stepped = true;
@@ -661,6 +742,13 @@ int elastic_batch_poll(elastic_batch* b, int64_t* tags, elastic_stats* stats, in
       int rc = post_solve_stats(c, &o);
       if (rc) return berr(b, rc, c->err);
     }
+    // the slot holds no instance any more: kernels skip it even if a list
+    // still named it (the lists are rebuilt before the next step)
+    // (synchronous: the batch stream is idle here, and a later submit into
+    // this slot rewrites the host copy)
+    b->h_insts[k].live = 0;
+    BATCH_HIP(b, hipMemcpy(&b->d_insts[k].live, &b->h_insts[k].live, sizeof(int),
+                           hipMemcpyHostToDevice));
     S.used = false;
     S.ctx = nullptr;
     ++nd;
