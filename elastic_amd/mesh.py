@@ -170,8 +170,25 @@ def tet4_cube(N: int, jitter: float = 0.0, seed: int = 20261203, logE=(-1.0, 1.0
     tets = hexes[:, _TET_SPLIT].reshape(-1, 4)
     numel = tets.shape[0]
     det = tet_jacobian_det(x[tets - 1])
-    if np.any(det <= 1e-14):
-        raise ValueError("jittered tetrahedron with non-positive Jacobian; change seed/jitter")
+    # resample the jitter of the nodes of tetrahedra whose jtetra4 determinant
+    # is not above 1e-14 (Elmlib.f:2685), round by round, from counters no
+    # earlier draw used; meshes without such tetrahedra are left unchanged
+    n1, h = N + 1, 1.0 / N
+    rnd = 0
+    while np.any(det <= 1e-14):
+        rnd += 1
+        if rnd > 64 or jitter <= 0.0:
+            raise ValueError("tetrahedron with non-positive Jacobian after resampling")
+        nodes = np.unique(tets[det <= 1e-14].ravel() - 1)
+        i, j, k = nodes % n1, (nodes // n1) % n1, nodes // (n1 * n1)
+        inner = (i > 0) & (i < N) & (j > 0) & (j < N) & (k > 0) & (k < N)
+        nodes, i, j, k = nodes[inner], i[inner], j[inner], k[inner]
+        base = np.stack([i * h, j * h, k * h], axis=1)
+        for d in range(3):
+            ctr = (np.uint64(rnd) * np.uint64(nnode) + nodes.astype(np.uint64)) * np.uint64(3) + np.uint64(d)
+            u = uniform01(seed + 7919, ctr)
+            x[nodes, d] = base[:, d] + (2.0 * u - 1.0) * jitter * h
+        det = tet_jacobian_det(x[tets - 1])
     ix = np.empty((numel, 5), np.int32)
     ix[:, :4] = tets
     top = x[:, 2] == 1.0
