@@ -165,7 +165,9 @@ __global__ __launch_bounds__(kThreads) void k_bi_half1(int n, const double* __re
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
   const double vr0 = grid_sum(part_vr0, sh);
-  if (vr0 == 0.0) {  // breakdown: every block sees the same sum
+  // breakdown (every block sees the same sums): a zero or non-finite
+  // denominator, or a non-finite r.r0 carried from the previous iteration
+  if (vr0 == 0.0 || !isfinite(vr0) || !isfinite(*rr_cur)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->status = ELASTIC_BREAKDOWN;
       sc->done = 1;
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(kThreads) void k_bi_half2(int n, const double* __re
   if (block_flag(&sc->done, sh)) return;
   const double tr = grid_sum(part_tr, sh);
   const double tt = grid_sum(part_tt, sh);
-  if (tt == 0.0) {
+  if (tt == 0.0 || !isfinite(tt) || !isfinite(tr)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->status = ELASTIC_BREAKDOWN;
       sc->done = 1;
@@ -234,6 +236,11 @@ __global__ void k_bi_check(const double* __restrict__ part_rr, const double* __r
   if (threadIdx.x == 0) {
     if (half == 1) sc->iter += 1;
     if (half == 2) *rr_next = rr0;
+    if (!isfinite(d) || (half == 2 && !isfinite(rr0))) {
+      sc->status = ELASTIC_BREAKDOWN;
+      sc->done = 1;
+      return;
+    }
     const double rn = sqrt(fabs(d));
     if (!sc->reliable) {
       if (rn < sc->conv) sc->done = 1;
