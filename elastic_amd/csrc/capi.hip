@@ -623,7 +623,7 @@ static int solve_other(Context* c, int solver, int precond, int precision, doubl
   else
     st->rel_res = bnm > 0 ? st->norm_m_r / bnm : 0.0;
   if (st->status == ELASTIC_BREAKDOWN)
-    return set_error(c, ELASTIC_BREAKDOWN, "zero denominator in the Krylov recurrence");
+    return set_error(c, ELASTIC_BREAKDOWN, "zero or non-finite denominator in the Krylov recurrence");
   return ELASTIC_OK;
 }
 
