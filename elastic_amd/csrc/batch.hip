@@ -411,6 +411,12 @@ struct elastic_batch {
   std::vector<uint32_t> h_sw[kVariants], h_vwork;
   std::vector<int32_t> h_slist, h_counts;
   unsigned mask = 0, graph_mask = ~0u;  // non-empty variant lists (+ bit 8: mixed present)
+  // grid of each product variant: the workgroups the device keeps resident
+  // at the current LDS size (a workgroup walks its share of the tiles, so a
+  // larger grid would leave late starters running after the rest)
+  int vgrid[2][kVariants] = {{0}};
+  size_t vgrid_lds = 0;
+  int ncu = 0;
   BStatus *d_status = nullptr, *h_status = nullptr;  // h_status pinned
   bool dirty = true, stepped = false;
   size_t lds = 0, graph_lds = 0;
@@ -442,14 +448,16 @@ static void enqueue_iteration(elastic_batch* b, int it) {
   const int32_t* vcount = b->d_counts + kVariants;
   for (int v = 0; v < kVariants; ++v)
     if (b->mask & (1u << v))
-      launch_variant(v, G, TB, b->lds, s, b->d_insts, b->d_sw[v], b->d_counts + v, 0);
+      launch_variant(v, dim3(b->vgrid[0][v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
+                     b->d_counts + v, 0);
   hipLaunchKernelGGL(k_bt_merge_dot, G, TB, 0, s, b->d_insts, b->d_vwork, vcount);
   hipLaunchKernelGGL(k_bt_update, G, TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
   hipLaunchKernelGGL(k_bt_pupdate, G, TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
   if (cur == 1 && (b->mask & 0x100u)) {  // every second iteration: true residuals where asked
     for (int v = 0; v < kVariants; v += 3)  // the FP64 lists
       if (b->mask & (1u << v))
-        launch_variant(v, G, TB, b->lds, s, b->d_insts, b->d_sw[v], b->d_counts + v, 1);
+        launch_variant(v, dim3(b->vgrid[1][v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
+                       b->d_counts + v, 1);
     hipLaunchKernelGGL(k_bt_rel_resid, G, TB, 0, s, b->d_insts, b->d_vwork, vcount);
     hipLaunchKernelGGL(k_bt_rel_finish, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist,
                        vcount + 1, 0);
@@ -457,6 +465,20 @@ static void enqueue_iteration(elastic_batch* b, int it) {
 }
 
 static int set_lds_limit(elastic_batch* b) {
+  if (b->vgrid_lds != b->lds) {
+    for (int mode = 0; mode < 2; ++mode)
+      for (int v = 0; v < kVariants; ++v) {
+        const void* k = mode == 0 ? variant_kernel<0>(v) : variant_kernel<1>(v);
+        int per_cu = 0;
+        if (b->lds > 64 * 1024)
+          (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kThreads, b->lds) != hipSuccess ||
+            per_cu < 1)
+          per_cu = 1;
+        b->vgrid[mode][v] = std::max(1, std::min(per_cu * b->ncu, kBatchGrid * 4));
+      }
+    b->vgrid_lds = b->lds;
+  }
   if (b->lds > 64 * 1024)
     for (int v = 0; v < kVariants; ++v) {
       BATCH_HIP(b, hipFuncSetAttribute(variant_kernel<0>(v),
@@ -565,6 +587,9 @@ int elastic_batch_create(elastic_batch** out, int32_t device, int32_t max_slots,
   if (const char* e = getenv("ELASTIC_RELIABLE_DELTA")) b->delta = atof(e);
   b->slots.resize(max_slots);
   b->h_insts.resize(max_slots);
+  if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      b->ncu < 1)
+    b->ncu = 256;
   std::memset(b->h_insts.data(), 0, sizeof(BInst) * max_slots);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&b->d_insts, sizeof(BInst) * max_slots) != hipSuccess ||
