@@ -415,8 +415,9 @@ struct elastic_batch {
   // at the current LDS size (a workgroup walks its share of the tiles, so a
   // larger grid would leave late starters running after the rest)
   int vgrid[2][kVariants] = {{0}};
-  size_t vgrid_lds = 0;
+  size_t vgrid_lds = ~(size_t)0;  // LDS size the grids were computed for
   int ncu = 0;
+  int ggrid[4] = {0, 0, 0, 0};  // merge_dot, update, pupdate, rel_resid
   BStatus *d_status = nullptr, *h_status = nullptr;  // h_status pinned
   bool dirty = true, stepped = false;
   size_t lds = 0, graph_lds = 0;
@@ -443,22 +444,22 @@ static size_t tile_lds(int nb, int ncols_max, int nblk_max) {
 // the step (its parity selects the r.Mr slot)
 static void enqueue_iteration(elastic_batch* b, int it) {
   hipStream_t s = b->stream;
-  const dim3 G(kBatchGrid), TB(kThreads);
+  const dim3 TB(kThreads);
   const int cur = it & 1;
   const int32_t* vcount = b->d_counts + kVariants;
   for (int v = 0; v < kVariants; ++v)
     if (b->mask & (1u << v))
       launch_variant(v, dim3(b->vgrid[0][v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
                      b->d_counts + v, 0);
-  hipLaunchKernelGGL(k_bt_merge_dot, G, TB, 0, s, b->d_insts, b->d_vwork, vcount);
-  hipLaunchKernelGGL(k_bt_update, G, TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
-  hipLaunchKernelGGL(k_bt_pupdate, G, TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
+  hipLaunchKernelGGL(k_bt_merge_dot, dim3(b->ggrid[0]), TB, 0, s, b->d_insts, b->d_vwork, vcount);
+  hipLaunchKernelGGL(k_bt_update, dim3(b->ggrid[1]), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
+  hipLaunchKernelGGL(k_bt_pupdate, dim3(b->ggrid[2]), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
   if (cur == 1 && (b->mask & 0x100u)) {  // every second iteration: true residuals where asked
     for (int v = 0; v < kVariants; v += 3)  // the FP64 lists
       if (b->mask & (1u << v))
         launch_variant(v, dim3(b->vgrid[1][v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
                        b->d_counts + v, 1);
-    hipLaunchKernelGGL(k_bt_rel_resid, G, TB, 0, s, b->d_insts, b->d_vwork, vcount);
+    hipLaunchKernelGGL(k_bt_rel_resid, dim3(b->ggrid[3]), TB, 0, s, b->d_insts, b->d_vwork, vcount);
     hipLaunchKernelGGL(k_bt_rel_finish, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist,
                        vcount + 1, 0);
   }
@@ -477,6 +478,15 @@ static int set_lds_limit(elastic_batch* b) {
           per_cu = 1;
         b->vgrid[mode][v] = std::max(1, std::min(per_cu * b->ncu, kBatchGrid * 4));
       }
+    const void* vk[4] = {(const void*)k_bt_merge_dot, (const void*)k_bt_update,
+                         (const void*)k_bt_pupdate, (const void*)k_bt_rel_resid};
+    for (int q = 0; q < 4; ++q) {
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, vk[q], kThreads, 0) != hipSuccess ||
+          per_cu < 1)
+        per_cu = 1;
+      b->ggrid[q] = std::max(1, std::min(per_cu * b->ncu, kBatchGrid * 4));
+    }
     b->vgrid_lds = b->lds;
   }
   if (b->lds > 64 * 1024)
