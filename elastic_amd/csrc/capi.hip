@@ -63,11 +63,12 @@ static void free_layout(Context* c) {
   free_tiles(c->tiles);
   free_blocks(c->blk);
   c->inc_ptr = c->inc_idx = nullptr;
-  void* ptrs[] = {c->A32.ad, c->A32.al, c->epart64};
+  void* ptrs[] = {c->A32.ad, c->A32.al, c->epart64, c->tdot};
   for (void* p : ptrs)
     if (p) hipFree(p);
   c->A32 = DevMatrix<float>();
   c->epart64 = nullptr;
+  c->tdot = nullptr;
 }
 
 static void free_system(Context* c) {
@@ -168,6 +169,7 @@ static int build_layout(elastic_ctx* x) {
     c->inc_ptr = b.inc_ptr;
     c->inc_idx = b.inc_idx;  // node-granular lists
     nepart = (int64_t)nb * L.next_nodes;
+    rc = rc ? rc : dalloc(c, &c->tdot, (size_t)L.ntiles);
   }
   if (nb == 1) {  // FP32 operator of the scalar tiles (the blocked layout has its planes)
     rc = rc ? rc : dalloc(c, &c->A32.ad, neq);

@@ -130,6 +130,8 @@ struct SolveScalars {
   // progress guard of the mixed methods (device_util.h: progress_guard)
   double last_true;
   int32_t stall, op64, op64_from, last_upd;
+  // p.(A p) formed by the product's tiles (node-blocked layouts)
+  double pq_tiles;
 };
 
 constexpr int kRedBlocks = 1024;  // fixed reduction grid (deterministic sums)
@@ -156,6 +158,7 @@ struct Context {
   bool values_set = false, values32 = false, bdiag_set = false;
   // workspace
   double *epart64 = nullptr, *red64 = nullptr;
+  double* tdot = nullptr;  // per-tile shares of p.(A p) (node-blocked layouts)
   double *v64[8] = {nullptr};
   SolveScalars* scal = nullptr;
   SolveScalars* scal_host = nullptr;  // pinned
@@ -243,8 +246,15 @@ template <typename S, typename D>
 void launch_cast(int64_t n, const S* a, D* b, hipStream_t s);
 // y = A x with the active layout; fp32 selects the FP32-value operator.  The
 // external partials are left in epart for the merge lists c->inc_ptr/inc_idx.
+// Optional output of a node-blocked product: x.(A x), formed from per-tile
+// shares (part, one per tile) that a one-workgroup kernel adds in tile order
+// into *result.
+struct DotOut {
+  double* part;
+  double* result;
+};
 void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epart,
-                 const int32_t* skip = nullptr);
+                 const int32_t* skip = nullptr, const DotOut* dot = nullptr);
 // gathers the CSRC values into the node-blocked planes (after every assembly)
 void launch_block_pack(Context* c);
 template <typename T>

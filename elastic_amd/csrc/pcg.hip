@@ -2,8 +2,10 @@
 //
 // PCG follows Iters.f:252-524 step for step (x0 = 0, Jacobi or 3x3
 // block-Jacobi M, stop when sqrt|r.M r| < tol sqrt|b.M b|).  One iteration is
-// four kernels and no host synchronisation:
-//   spmv tiles (q = A p)  ->  merge + p.q  ->  x, r update + r.z  ->  p update
+// four kernels and no host synchronisation on the node-blocked layout:
+//   spmv tiles (q = A p, and each tile's share of p.q)  ->  sum of the shares
+//     ->  merge of q's external partials + x, r update + r.z  ->  p update
+// (the row-tile layout merges q and forms p.q in a kernel of its own).
 // Dot products are reduced in two fixed-order stages (per-block partials over a
 // fixed grid of kRedBlocks blocks, then every consumer block sums the partials
 // in the same order), so every block sees the same alpha and beta and the
@@ -30,7 +32,8 @@ namespace elastic {
 __global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const double* __restrict__ b,
                                                         const double* __restrict__ m,
                                                         const double* __restrict__ bm, double* x,
-                                                        double* r, double* p, double* part) {
+                                                        double* r, double* p, double* part,
+                                                        double* part_pr) {
   __shared__ double sh[8];
   double acc = 0.0;
   GRID_LOOP(i, n) {
@@ -42,6 +45,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const double* __re
   }
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (part_pr && threadIdx.x == 0) part_pr[blockIdx.x] = s;  // p.r = z.b
 }
 
 __global__ void k_pcg_init2(const double* __restrict__ part, double tol, double delta,
@@ -63,6 +67,7 @@ __global__ void k_pcg_init2(const double* __restrict__ part, double tol, double 
     sc->skip64 = 1;
     sc->nupd = 0;
     progress_init(sc, sqrt(fabs(d)));
+    sc->pq_tiles = 0.0;
     sc->done = (d == 0.0) ? 1 : 0;  // b = 0: x = 0 is the solution
   }
 }
@@ -108,12 +113,75 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int n, const double* __
   const double alpha = num / pq;
   double acc = 0.0, azq = 0.0;
   GRID_LOOP(i, n) {
+    const double qi = q[i];
     x[i] = x[i] + alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
+    const double ri = r[i] - alpha * qi;
     r[i] = ri;
     const double zi = ri * m[i];
     acc += ri * zi;
-    azq += zi * q[i];
+    azq += zi * qi;
+  }
+  if (jacobi) {
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part_rz[blockIdx.x] = s;
+    if (part_zq) {
+      const double t = block_sum(azq, sh);
+      if (threadIdx.x == 0) part_zq[blockIdx.x] = t;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->alpha = alpha;
+    sc->pq = pq;
+  }
+}
+
+// k_pcg_update on the node-blocked layout (n = NB * nn), one lane per node:
+// the node's merge list is walked once for its NB rows (contiguous partials),
+// and the vector loads are issued before that gather.  p.q comes from the
+// product's tiles (sc->pq_tiles).
+template <int NB>
+__global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs mg,
+                                                                const double* __restrict__ part_pr,
+                                                                const double* __restrict__ dslot,
+                                                                double* x,
+                                                                const double* __restrict__ p,
+                                                                double* r, double* q, int keep_q,
+                                                                const double* __restrict__ m,
+                                                                int jacobi, double* part_rz,
+                                                                double* part_zq, SolveScalars* sc) {
+  __shared__ double sh[8];
+  if (block_flag(&sc->done, sh)) return;
+  const double pq = sc->pq_tiles;
+  const double num = part_pr ? grid_sum(part_pr, sh) : *dslot;
+  const double alpha = num / pq;
+  double acc = 0.0, azq = 0.0;
+  GRID_LOOP(J, nn) {
+    const int64_t i0 = (int64_t)NB * J;
+    const int k0 = mg.ptr[J], k1 = mg.ptr[J + 1];
+    double qv[NB], xv[NB], pv[NB], rv[NB], mv[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      qv[c] = q[i0 + c];
+      xv[c] = x[i0 + c];
+      pv[c] = p[i0 + c];
+      rv[c] = r[i0 + c];
+      mv[c] = m[i0 + c];
+    }
+    for (int k = k0; k < k1; ++k) {
+      const double* e = mg.epart + (int64_t)NB * mg.idx[k];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) qv[c] += e[c];
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      if (keep_q) q[i0 + c] = qv[c];
+      x[i0 + c] = xv[c] + alpha * pv[c];
+      const double ri = rv[c] - alpha * qv[c];
+      r[i0 + c] = ri;
+      const double zi = ri * mv[c];
+      acc += ri * zi;
+      azq += zi * qv[c];
+    }
   }
   if (jacobi) {
     const double s = block_sum(acc, sh);
@@ -153,8 +221,8 @@ __global__ __launch_bounds__(kThreads) void k_rz(int n, const double* __restrict
   }
 }
 
-// di = r.z ; beta = di / d (or -z.q / p.q) ; p = z + beta p ; bookkeeping in
-// block 0
+// di = r.z ; beta = di / d (or -z.q / p.q) ; p = z + beta p ; partial p.r
+// when part_pr != null ; bookkeeping in block 0
 __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* __restrict__ part_rz,
                                                            const double* __restrict__ part_zq,
                                                            const double* __restrict__ dold,
@@ -162,12 +230,21 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
                                                            const double* __restrict__ r,
                                                            const double* __restrict__ m,
                                                            const double* __restrict__ bm,
-                                                           SolveScalars* sc) {
+                                                           double* part_pr, SolveScalars* sc) {
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
   const double di = grid_sum(part_rz, sh);
   const double beta = part_zq ? -grid_sum(part_zq, sh) / sc->pq : di / *dold;
-  GRID_LOOP(i, n) p[i] = apply_m(i, n, r, m, bm) + beta * p[i];
+  double apr = 0.0;
+  GRID_LOOP(i, n) {
+    const double pi = apply_m(i, n, r, m, bm) + beta * p[i];
+    p[i] = pi;
+    apr += pi * r[i];
+  }
+  if (part_pr) {  // p.r of the next step (flexible recurrences, tile dots)
+    const double t = block_sum(apr, sh);
+    if (threadIdx.x == 0) part_pr[blockIdx.x] = t;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *dnew = di;
     sc->beta = beta;
@@ -192,26 +269,33 @@ __global__ void k_rel_force(SolveScalars* sc) {
   }
 }
 
-// r = b - (t + merged partials); with Jacobi also partial r.(m r)
+// r = b - (t + merged partials); with Jacobi also partial r.(m r); with
+// part_pr also the partial p.r of the replaced residual
 __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
                                                          const double* __restrict__ t,
                                                          const double* __restrict__ b,
                                                          const double* __restrict__ m, double* r,
                                                          int jacobi, double* part,
-                                                         const SolveScalars* sc) {
+                                                         const double* __restrict__ p,
+                                                         double* part_pr, const SolveScalars* sc) {
   __shared__ double sh[8];
   if (!block_flag(&sc->upd, sh) || block_flag(&sc->done, sh)) return;
-  double acc = 0.0;
+  double acc = 0.0, apr = 0.0;
   GRID_LOOP(i, n) {
     double s = t[i];
     s = merge_row(mg, i, s);
     const double ri = b[i] - s;
     r[i] = ri;
     acc += ri * (ri * m[i]);
+    if (part_pr) apr += p[i] * ri;
   }
   if (jacobi) {
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
+  }
+  if (part_pr) {
+    const double s = block_sum(apr, sh);
+    if (threadIdx.x == 0) part_pr[blockIdx.x] = s;
   }
 }
 
@@ -249,14 +333,19 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   double* part_zq = reliable ? c->red64 + 3 * kRedBlocks : nullptr;
   const int jac = V.bm == nullptr;
   SolveScalars* sc = c->scal;
-  hipLaunchKernelGGL(k_pcg_init, G, TB, 0, s, n, V.b, V.m, V.bm, V.x, V.r, V.p, part_a);
+  // node-blocked layout: the product's tiles form p.q themselves
+  const bool tdots = c->tdot != nullptr && c->blk.ntiles > 0;
+  const DotOut dot{c->tdot, &sc->pq_tiles};
+  // p.r after the p update (and after a replacement), read by the next update
+  double* pr_next = tdots ? part_pr : nullptr;
+  hipLaunchKernelGGL(k_pcg_init, G, TB, 0, s, n, V.b, V.m, V.bm, V.x, V.r, V.p, part_a, pr_next);
   hipLaunchKernelGGL(k_pcg_init2, dim3(1), TB, 0, s, part_a, tol, delta, reliable ? 1 : 0, dslots,
                      sc);
   auto replace = [&](int iters_so_far) {
     double* dnext = dslots + (iters_so_far & 1);
     launch_spmv(c, false, V.x, V.t, V.epart, &sc->skip64);
     hipLaunchKernelGGL(k_rel_resid, G, TB, 0, s, n, merge_of(c, V.epart),
-                       V.t, V.b, V.m, V.r, jac, part_b, sc);
+                       V.t, V.b, V.m, V.r, jac, part_b, (const double*)V.p, pr_next, sc);
     if (!jac)
       hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)nullptr,
                          (double*)nullptr, &sc->upd, sc);
@@ -271,16 +360,31 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
     for (int j = 0; j < chunk; ++j, ++it) {
       double* dcur = dslots + (it & 1);
       double* dnxt = dslots + ((it + 1) & 1);
-      launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done);
-      hipLaunchKernelGGL(k_merge_dot, G, TB, 0, s, n, merge_of(c, V.epart), V.q, V.p, part_a,
-                         (const double*)V.r, part_pr, sc);
-      hipLaunchKernelGGL(k_pcg_update, G, TB, 0, s, n, part_a, (const double*)part_pr, dcur, V.x,
-                         V.p, V.r, V.q, V.m, jac, part_b, part_zq, sc);
+      if (tdots) {
+        launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done, &dot);
+        // q is read again only by the block preconditioner's (M^-1 r).q
+        const int keep_q = (!jac && reliable) ? 1 : 0;
+        if (c->layout_nb == 3)
+          hipLaunchKernelGGL(k_pcg_update_nodes<3>, G, TB, 0, s, n / 3, merge_of(c, V.epart),
+                             (const double*)part_pr, dcur, V.x, V.p, V.r, V.q, keep_q, V.m, jac,
+                             part_b, part_zq, sc);
+        else
+          hipLaunchKernelGGL(k_pcg_update_nodes<2>, G, TB, 0, s, n / 2, merge_of(c, V.epart),
+                             (const double*)part_pr, dcur, V.x, V.p, V.r, V.q, keep_q, V.m, jac,
+                             part_b, part_zq, sc);
+      } else {
+        launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done);
+        hipLaunchKernelGGL(k_merge_dot, G, TB, 0, s, n, merge_of(c, V.epart), V.q, V.p, part_a,
+                           (const double*)V.r, part_pr, sc);
+        hipLaunchKernelGGL(k_pcg_update, G, TB, 0, s, n, (const double*)part_a,
+                           (const double*)part_pr, dcur, V.x, V.p, V.r, (const double*)V.q, V.m,
+                           jac, part_b, part_zq, sc);
+      }
       if (!jac)
         hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)V.q,
                            part_zq, (const int32_t*)nullptr, sc);
       hipLaunchKernelGGL(k_pcg_pupdate, G, TB, 0, s, n, part_b, (const double*)part_zq, dcur, dnxt,
-                         V.p, V.r, V.m, V.bm, sc);
+                         V.p, V.r, V.m, V.bm, pr_next, sc);
       // the replacement kernels gate themselves on the flag set by k_pcg_pupdate
       if (reliable && ((it + 1) & 1) == 0) replace(it + 1);
     }

@@ -114,11 +114,29 @@ struct NoHook {
   __device__ void operator()() const {}
 };
 
-template <typename TV, typename T, int NB, int PER, typename Hook = NoHook>
+// Block sum of the lanes' shares of x.(A x) into part[t] (fixed order).
+__device__ __forceinline__ void tile_dot_store(double* part, int t, double v) {
+  __shared__ double red[4];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();  // red of a previous tile in this workgroup is free
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) part[t] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// DOT: also the tile's share of x.(A x) into tdot[t]: x.y over its own rows
+// plus x_J.partial_J over its external columns, with x as staged in LDS (exact
+// for T = double; rounded to FP32 like the operator's own input otherwise),
+// products and sums in FP64.  The shares
+// are added in tile order by a kernel of their own (k_sum_shares), so x.(A x)
+// needs no further pass over y and no floating-point atomics.
+template <typename TV, typename T, int NB, int PER, bool DOT = false, typename Hook = NoHook>
 __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
                                              const TileRegs<TV, NB, PER>& R,
                                              const double* __restrict__ x, double* __restrict__ y,
                                              double* __restrict__ epart, unsigned char* smem,
+                                             double* __restrict__ tdot = nullptr,
                                              Hook hook = Hook()) {
   constexpr int NB2 = NB * NB;
   T* xs = reinterpret_cast<T*>(smem);       // ncols_max * NB
@@ -211,6 +229,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   }
   __syncthreads();
   const uint16_t* sp = a.segptr + a.seg0[t];
+  double dacc = 0.0;  // this lane's share of x.(Ax) (DOT)
   for (int c = tid; c < ncols; c += 256) {
     T s[NB];
 #pragma unroll
@@ -249,18 +268,24 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
 #pragma unroll
       for (int r = 0; r < NB; ++r) epart[(int64_t)NB * (ext0 + c - nr) + r] = (double)s[r];
     }
+    if constexpr (DOT) {
+#pragma unroll
+      for (int r = 0; r < NB; ++r) dacc += (double)s[r] * (double)xs[c * NB + r];
+    }
   }
+  if constexpr (DOT) tile_dot_store(tdot, t, dacc);
 }
 
 // One tile: its loads, then its product.
-template <typename TV, typename T, int NB, int PER>
+template <typename TV, typename T, int NB, int PER, bool DOT = false>
 __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
                                            const TV* __restrict__ diag,
                                            const double* __restrict__ x, double* __restrict__ y,
-                                           double* __restrict__ epart, unsigned char* smem) {
+                                           double* __restrict__ epart, unsigned char* smem,
+                                           double* __restrict__ tdot = nullptr) {
   TileRegs<TV, NB, PER> R;
   load_tile<TV, NB, PER>(a, t, vals, diag, R);
-  compute_tile<TV, T, NB, PER>(a, t, R, x, y, epart, smem);
+  compute_tile<TV, T, NB, PER, DOT>(a, t, R, x, y, epart, smem, tdot);
 }
 
 }  // namespace elastic

@@ -5,16 +5,47 @@
 
 namespace elastic {
 
-template <typename TV, typename T, int NB, int PER>
+// DOT: also the tiles' shares of x.(A x) into tdot
+template <typename TV, typename T, int NB, int PER, bool DOT>
 __global__ __launch_bounds__(256) void k_spmv_blocks(BlkArgs a, const TV* __restrict__ vals,
                                                       const TV* __restrict__ diag,
                                                       const double* __restrict__ x,
                                                       double* __restrict__ y,
                                                       double* __restrict__ epart,
-                                                      const int32_t* __restrict__ skip) {
+                                                      const int32_t* __restrict__ skip,
+                                                      double* __restrict__ tdot) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (skip != nullptr && *skip != 0) return;
-  block_tile<TV, T, NB, PER>(a, blockIdx.x, vals, diag, x, y, epart, smem);
+  block_tile<TV, T, NB, PER, DOT>(a, blockIdx.x, vals, diag, x, y, epart, smem, tdot);
+}
+
+// *out = the n shares added in a fixed order (one workgroup of 1024 lanes):
+// lane l sums shares l, l + 1024, ... (eight loads in flight at a time, added
+// in index order); then a fixed tree over the lanes.
+__global__ __launch_bounds__(1024) void k_sum_shares(const double* __restrict__ part, int n,
+                                                      double* out,
+                                                      const int32_t* __restrict__ skip) {
+  __shared__ double red[16];
+  if (skip != nullptr && *skip != 0) return;
+  double v = 0.0;
+  for (int k0 = threadIdx.x; k0 < n; k0 += 8 * 1024) {
+    double w[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = k0 + 1024 * j;
+      w[j] = k < n ? part[k] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v += w[j];
+  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < 16; ++w) s += red[w];
+    *out = s;
+  }
 }
 
 // Gathers the CSRC al/ad into the block planes (one workgroup per tile) and
@@ -67,37 +98,49 @@ __global__ __launch_bounds__(256) void k_pack_diag(int nn, const int32_t* __rest
 template <typename TV, typename TA, int NB>
 static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* diag,
                                const double* x, double* y, double* epart, hipStream_t s,
-                               const int32_t* skip) {
+                               const int32_t* skip, const DotOut* dot) {
   const size_t lds = sizeof(TA) * NB * ((size_t)b.ncols_max + 2 * kBlkRowsMax + (size_t)b.nblk_max);
   const int per = (b.nblk_max + 255) / 256;
-#define ELASTIC_BLK_LAUNCH(K, P, G, ...)                                                        \
+#define ELASTIC_BLK_LAUNCH(K, P, D, G, ...)                                                     \
   do {                                                                                          \
     static bool raised = false;                                                                 \
     if (lds > 65536 && !raised) {                                                               \
-      (void)hipFuncSetAttribute((const void*)K<TV, TA, NB, P>,                                  \
+      (void)hipFuncSetAttribute((const void*)K<TV, TA, NB, P, D>,                               \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);        \
       raised = true;                                                                            \
     }                                                                                           \
-    hipLaunchKernelGGL((K<TV, TA, NB, P>), dim3(G), dim3(256), lds, s, __VA_ARGS__);            \
+    hipLaunchKernelGGL((K<TV, TA, NB, P, D>), dim3(G), dim3(256), lds, s, __VA_ARGS__);         \
   } while (0)
-  if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 1, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip);
-  else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip);
-  else ELASTIC_BLK_LAUNCH(k_spmv_blocks, 4, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip);
+  double* td = dot ? dot->part : nullptr;
+#define ELASTIC_BLK_PER(D)                                                                               \
+  do {                                                                                                   \
+    if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 1, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td); \
+    else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td); \
+    else ELASTIC_BLK_LAUNCH(k_spmv_blocks, 4, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td); \
+  } while (0)
+  if (dot) {
+    ELASTIC_BLK_PER(true);
+    hipLaunchKernelGGL(k_sum_shares, dim3(1), dim3(1024), 0, s, (const double*)dot->part,
+                       b.ntiles, dot->result, skip);
+  } else {
+    ELASTIC_BLK_PER(false);
+  }
+#undef ELASTIC_BLK_PER
 #undef ELASTIC_BLK_LAUNCH
 }
 
 void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epart,
-                 const int32_t* skip) {
+                 const int32_t* skip, const DotOut* dot) {
   hipStream_t s = c->stream;
   const DevBlocks& b = c->blk;
   if (c->layout_nb == 3) {
-    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 3>(b, b.v32, b.d32, x, y, epart, s, skip);
-    else if (fp32) spmv_blocks_launch<float, double, 3>(b, b.v32, b.d32, x, y, epart, s, skip);
-    else spmv_blocks_launch<double, double, 3>(b, b.v64, b.d64, x, y, epart, s, skip);
+    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot);
+    else if (fp32) spmv_blocks_launch<float, double, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot);
+    else spmv_blocks_launch<double, double, 3>(b, b.v64, b.d64, x, y, epart, s, skip, dot);
   } else if (c->layout_nb == 2) {
-    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 2>(b, b.v32, b.d32, x, y, epart, s, skip);
-    else if (fp32) spmv_blocks_launch<float, double, 2>(b, b.v32, b.d32, x, y, epart, s, skip);
-    else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip);
+    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot);
+    else if (fp32) spmv_blocks_launch<float, double, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot);
+    else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip, dot);
   } else {
     if (fp32)
       launch_spmv_tiles<float, double>(c->tiles, c->A32.ad, c->A32.al, x, y, epart, s, skip);
