@@ -3,7 +3,9 @@
 // together.  Every phase of a PCG iteration is ONE launch over all active
 // instances with a fixed grid that walks a device work list:
 //   SpMV          work item = (instance, tile)          block_tile of spmv_block.h
-//   vector phases work item = (instance, chunk of 2048 rows)
+//                 (q = A p and each tile's share of p.q)
+//   shares sum    one workgroup per instance: p.q in tile order
+//   vector phases work item = (instance, chunk of whole nodes, <= 2048 rows)
 // so a chunk of iterations is captured once as a HIP graph and replayed while
 // instances are admitted and retired: only the work lists and the instance
 // table (device memory) change between replays.
@@ -26,7 +28,7 @@
 
 namespace elastic {
 
-constexpr int kChunk = 2048;         // rows per vector work item (256 lanes x 8)
+constexpr int kChunk = 2048;         // rows per vector work item at most (256 lanes x 8)
 constexpr int kBatchGrid = 2048;     // fixed grid of every batched phase
 constexpr int kMaxSlots = 255;       // slot index lives in the top 8 bits of a work item
 constexpr int kBatchPer = 2;         // node blocks per lane (tiles of <= 512 blocks)
@@ -35,14 +37,15 @@ struct BInst {
   BlkArgs a;
   const double *v64, *d64;
   const float *v32, *d32;
-  MergeArgs mg;   // external partials of the instance's product -> rows
+  MergeArgs mg;   // external partials of the instance's product -> node rows
   double* epart;
+  double* tdot;   // ntiles shares of p.q written by the product's tiles
   double *x, *r, *p, *q, *b, *t;
   const double* m;
-  double* part;   // 5 * nch partials: p.q, r.Mr, true-residual r.Mr, p.r, (Mr).q
+  double* part;   // 5 * nch partials: (unused), r.Mr, true-residual r.Mr, p.r, (Mr).q
   double* dslot;  // 2 ping-pong slots of r.Mr
   SolveScalars* sc;
-  int n, nb, nch, fp32, acc32;
+  int n, nb, nch, cr, fp32, acc32;  // cr: rows per chunk (nb * (kChunk / nb))
   int live, ntiles;  // live: the slot holds an admitted instance (checked first)
 };
 
@@ -64,7 +67,8 @@ __device__ inline double inst_sum(const double* __restrict__ part, int nch, doub
 
 // The product of one operator variant (value type TV, sum type TA, node block
 // NB) over its work list, so each variant compiles with the registers of the
-// single-system kernel.  mode 0: q = A p; the FP64 list holds every instance
+// single-system kernel.  mode 0: q = A p and the tiles' shares of p.q; the
+// FP64 list holds every instance
 // and serves FP64 instances and mixed ones the progress guard handed over to
 // FP64, the FP32 lists hold the mixed instances that are still on FP32.
 // mode 1 (FP64 lists): t = A64 x for mixed instances that asked for a true
@@ -96,8 +100,10 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
     if (!flag) continue;
     const TV* vals = f64 ? (const TV*)I.v64 : (const TV*)I.v32;
     const TV* diag = f64 ? (const TV*)I.d64 : (const TV*)I.d32;
-    if (MODE == 0) block_tile<TV, TA, NB, kBatchPer>(I.a, t, vals, diag, I.p, I.q, I.epart, smem);
-    else block_tile<TV, TA, NB, kBatchPer>(I.a, t, vals, diag, I.x, I.t, I.epart, smem);
+    if (MODE == 0)
+      block_tile<TV, TA, NB, kBatchPer, true>(I.a, t, vals, diag, I.p, I.q, I.epart, smem, I.tdot);
+    else
+      block_tile<TV, TA, NB, kBatchPer>(I.a, t, vals, diag, I.x, I.t, I.epart, smem);
   }
 }
 
@@ -150,37 +156,64 @@ __device__ inline bool uniform_done(const BInst& I, int c, double* sh, bool want
   return d;
 }
 
-// q += merged partials; partial p.q
-__global__ __launch_bounds__(kThreads) void k_bt_merge_dot(const BInst* __restrict__ insts,
-                                                           const uint32_t* __restrict__ work,
-                                                           const int32_t* __restrict__ count) {
+// per instance (one workgroup each): p.q = the tiles' shares in tile order
+__global__ __launch_bounds__(kThreads) void k_bt_sum_shares(const BInst* __restrict__ insts,
+                                                            const int32_t* __restrict__ slots,
+                                                            const int32_t* __restrict__ count) {
   __shared__ double sh[8];
-  const int nw = *count;
-  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
-    const uint32_t e = work[w];
-    const BInst& I = insts[wl_slot(e)];
-    const int c = wl_index(e);
-    if (uniform_done(I, c, sh)) continue;
-    const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
-    double acc = 0.0, apr = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
-      double s = I.q[i];
-      s = merge_row(I.mg, i, s);
-      I.q[i] = s;
-      acc += I.p[i] * s;
-      if (I.fp32) apr += I.p[i] * I.r[i];
-    }
-    const double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) I.part[c] = s;
-    if (I.fp32) {  // flexible recurrences of the mixed method: p.r
-      const double t = block_sum(apr, sh);
-      if (threadIdx.x == 0) I.part[3 * I.nch + c] = t;
-    }
+  const int ns = *count;
+  for (int w = blockIdx.x; w < ns; w += gridDim.x) {
+    const BInst& I = insts[slots[w]];
+    if (uniform_done(I, 0, sh)) continue;
+    double v = 0.0;
+    for (int k = threadIdx.x; k < I.ntiles; k += kThreads) v += I.tdot[k];
+    const double pq = block_sum(v, sh);
+    if (threadIdx.x == 0) I.sc->pq_tiles = pq;
   }
 }
 
-// alpha = d / p.q (mixed: p.r / p.q) ; x += alpha p ; r -= alpha q ;
+// alpha = d / p.q (mixed: p.r / p.q) ; q = own rows + merged external
+// partials (node by node, not stored) ; x += alpha p ; r -= alpha q ;
 // partials r.(m r) and (mixed) (m r).q
+template <int NB>
+__device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, double* sh) {
+  const int J0 = c * (I.cr / NB), J1 = min(I.n / NB, J0 + I.cr / NB);
+  double acc = 0.0, azq = 0.0;
+  for (int J = J0 + threadIdx.x; J < J1; J += kThreads) {
+    const int64_t i0 = (int64_t)NB * J;
+    const int k0 = I.mg.ptr[J], k1 = I.mg.ptr[J + 1];
+    double qv[NB], xv[NB], pv[NB], rv[NB], mv[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      qv[r] = I.q[i0 + r];
+      xv[r] = I.x[i0 + r];
+      pv[r] = I.p[i0 + r];
+      rv[r] = I.r[i0 + r];
+      mv[r] = I.m[i0 + r];
+    }
+    for (int k = k0; k < k1; ++k) {
+      const double* e = I.epart + (int64_t)NB * I.mg.idx[k];
+#pragma unroll
+      for (int r = 0; r < NB; ++r) qv[r] += e[r];
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      I.x[i0 + r] = xv[r] + alpha * pv[r];
+      const double ri = rv[r] - alpha * qv[r];
+      I.r[i0 + r] = ri;
+      const double zi = ri * mv[r];
+      acc += ri * zi;
+      azq += zi * qv[r];
+    }
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) I.part[I.nch + c] = s;
+  if (I.fp32) {
+    const double t = block_sum(azq, sh);
+    if (threadIdx.x == 0) I.part[4 * I.nch + c] = t;
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict__ insts,
                                                         const uint32_t* __restrict__ work,
                                                         const int32_t* __restrict__ count,
@@ -192,25 +225,11 @@ __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict_
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
     if (uniform_done(I, c, sh)) continue;
-    const double pq = inst_sum(I.part, I.nch, sh);
+    const double pq = I.sc->pq_tiles;
     const double num = I.fp32 ? inst_sum(I.part + 3 * I.nch, I.nch, sh) : I.dslot[cur];
     const double alpha = num / pq;
-    const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
-    double acc = 0.0, azq = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
-      I.x[i] = I.x[i] + alpha * I.p[i];
-      const double ri = I.r[i] - alpha * I.q[i];
-      I.r[i] = ri;
-      const double zi = ri * I.m[i];
-      acc += ri * zi;
-      azq += zi * I.q[i];
-    }
-    const double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) I.part[I.nch + c] = s;
-    if (I.fp32) {
-      const double t = block_sum(azq, sh);
-      if (threadIdx.x == 0) I.part[4 * I.nch + c] = t;
-    }
+    if (I.nb == 3) bt_update_chunk<3>(I, c, alpha, sh);
+    else bt_update_chunk<2>(I, c, alpha, sh);
     if (c == 0 && threadIdx.x == 0) {
       I.sc->alpha = alpha;
       I.sc->pq = pq;
@@ -218,8 +237,8 @@ __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict_
   }
 }
 
-// beta = r.Mr / d (mixed: -(Mr).q / p.q) ; p = M r + beta p ; chunk 0 keeps
-// the books
+// beta = r.Mr / d (mixed: -(Mr).q / p.q) ; p = M r + beta p ; (mixed) partial
+// p.r of the new direction ; chunk 0 keeps the books
 __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict__ insts,
                                                          const uint32_t* __restrict__ work,
                                                          const int32_t* __restrict__ count,
@@ -234,8 +253,18 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
     const double di = inst_sum(I.part + I.nch, I.nch, sh);
     const double beta =
         I.fp32 ? -inst_sum(I.part + 4 * I.nch, I.nch, sh) / I.sc->pq : di / I.dslot[cur];
-    const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
-    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) I.p[i] = I.r[i] * I.m[i] + beta * I.p[i];
+    const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
+    double apr = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
+      const double ri = I.r[i];
+      const double pi = ri * I.m[i] + beta * I.p[i];
+      I.p[i] = pi;
+      apr += pi * ri;
+    }
+    if (I.fp32) {  // flexible recurrences of the mixed method
+      const double t = block_sum(apr, sh);
+      if (threadIdx.x == 0) I.part[3 * I.nch + c] = t;
+    }
     if (c == 0 && threadIdx.x == 0) {
       SolveScalars* sc = I.sc;
       I.dslot[cur ^ 1] = di;
@@ -259,7 +288,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
 }
 
 // true residual r = b - (t + merged partials) of instances that asked for it;
-// partial r.(m r)
+// partials r.(m r) and p.r
 __global__ __launch_bounds__(kThreads) void k_bt_rel_resid(const BInst* __restrict__ insts,
                                                            const uint32_t* __restrict__ work,
                                                            const int32_t* __restrict__ count) {
@@ -270,17 +299,20 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_resid(const BInst* __restri
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
     if (uniform_done(I, c, sh, true)) continue;
-    const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
-    double acc = 0.0;
+    const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
+    double acc = 0.0, apr = 0.0;
     for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
       double s = I.t[i];
       s = merge_row(I.mg, i, s);
       const double ri = I.b[i] - s;
       I.r[i] = ri;
       acc += ri * (ri * I.m[i]);
+      apr += I.p[i] * ri;
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) I.part[2 * I.nch + c] = s;
+    const double t = block_sum(apr, sh);  // only mixed instances ask for true residuals
+    if (threadIdx.x == 0) I.part[3 * I.nch + c] = t;
   }
 }
 
@@ -314,12 +346,12 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_finish(const BInst* __restr
 }
 
 // admission of one instance (eager, grid = its chunks): x = 0, r = b, p = M b,
-// partial b.Mb
+// partial b.Mb (= p.r, the first numerator of the mixed method)
 __global__ __launch_bounds__(kThreads) void k_bt_init(const BInst* __restrict__ insts, int slot) {
   __shared__ double sh[8];
   const BInst& I = insts[slot];
   const int c = blockIdx.x;
-  const int i0 = c * kChunk, i1 = min(I.n, i0 + kChunk);
+  const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
   double acc = 0.0;
   for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
     const double bi = I.b[i], zi = bi * I.m[i];
@@ -329,7 +361,10 @@ __global__ __launch_bounds__(kThreads) void k_bt_init(const BInst* __restrict__ 
     acc += bi * zi;
   }
   const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) I.part[I.nch + c] = s;
+  if (threadIdx.x == 0) {
+    I.part[I.nch + c] = s;
+    I.part[3 * I.nch + c] = s;
+  }
 }
 
 __global__ void k_bt_init2(const BInst* __restrict__ insts, int slot, double tol, int maxit,
@@ -354,6 +389,7 @@ __global__ void k_bt_init2(const BInst* __restrict__ insts, int slot, double tol
     sc->upd = 0;
     sc->skip64 = 1;
     sc->nupd = 0;
+    sc->pq_tiles = 0.0;
     progress_init(sc, sqrt(fabs(d)));
     sc->done = (d == 0.0) ? 1 : 0;
   }
@@ -417,7 +453,7 @@ struct elastic_batch {
   int vgrid[2][kVariants] = {{0}};
   size_t vgrid_lds = ~(size_t)0;  // LDS size the grids were computed for
   int ncu = 0;
-  int ggrid[4] = {0, 0, 0, 0};  // merge_dot, update, pupdate, rel_resid
+  int ggrid[3] = {0, 0, 0};  // update, pupdate, rel_resid
   BStatus *d_status = nullptr, *h_status = nullptr;  // h_status pinned
   bool dirty = true, stepped = false;
   size_t lds = 0, graph_lds = 0;
@@ -451,15 +487,16 @@ static void enqueue_iteration(elastic_batch* b, int it) {
     if (b->mask & (1u << v))
       launch_variant(v, dim3(b->vgrid[0][v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
                      b->d_counts + v, 0);
-  hipLaunchKernelGGL(k_bt_merge_dot, dim3(b->ggrid[0]), TB, 0, s, b->d_insts, b->d_vwork, vcount);
-  hipLaunchKernelGGL(k_bt_update, dim3(b->ggrid[1]), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
-  hipLaunchKernelGGL(k_bt_pupdate, dim3(b->ggrid[2]), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
+  hipLaunchKernelGGL(k_bt_sum_shares, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist,
+                     vcount + 1);
+  hipLaunchKernelGGL(k_bt_update, dim3(b->ggrid[0]), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
+  hipLaunchKernelGGL(k_bt_pupdate, dim3(b->ggrid[1]), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
   if (cur == 1 && (b->mask & 0x100u)) {  // every second iteration: true residuals where asked
     for (int v = 0; v < kVariants; v += 3)  // the FP64 lists
       if (b->mask & (1u << v))
         launch_variant(v, dim3(b->vgrid[1][v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
                        b->d_counts + v, 1);
-    hipLaunchKernelGGL(k_bt_rel_resid, dim3(b->ggrid[3]), TB, 0, s, b->d_insts, b->d_vwork, vcount);
+    hipLaunchKernelGGL(k_bt_rel_resid, dim3(b->ggrid[2]), TB, 0, s, b->d_insts, b->d_vwork, vcount);
     hipLaunchKernelGGL(k_bt_rel_finish, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist,
                        vcount + 1, 0);
   }
@@ -478,9 +515,9 @@ static int set_lds_limit(elastic_batch* b) {
           per_cu = 1;
         b->vgrid[mode][v] = std::max(1, std::min(per_cu * b->ncu, kBatchGrid * 4));
       }
-    const void* vk[4] = {(const void*)k_bt_merge_dot, (const void*)k_bt_update,
-                         (const void*)k_bt_pupdate, (const void*)k_bt_rel_resid};
-    for (int q = 0; q < 4; ++q) {
+    const void* vk[3] = {(const void*)k_bt_update, (const void*)k_bt_pupdate,
+                         (const void*)k_bt_rel_resid};
+    for (int q = 0; q < 3; ++q) {
       int per_cu = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, vk[q], kThreads, 0) != hipSuccess ||
           per_cu < 1)
@@ -652,7 +689,9 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   if (precision != ELASTIC_FP64 && precision != ELASTIC_MIXED) return ELASTIC_BAD_ARG;
   if (maxit <= 0) return ELASTIC_BAD_ARG;
   if (tol <= 0.0) tol = 2.220446049250313e-16;
-  const int nch = (c->neq + kChunk - 1) / kChunk;
+  if (!c->tdot) return berr(b, ELASTIC_NOT_READY, "context has no tile shares buffer");
+  const int cr = c->layout_nb * (kChunk / c->layout_nb);  // whole nodes per chunk
+  const int nch = (c->neq + cr - 1) / cr;
   if (5 * nch > 8 * kRedBlocks) return berr(b, ELASTIC_BAD_ARG, "system too large for a batch slot");
   if (c->blk.ntiles >= (1 << 24) || nch >= (1 << 24)) return ELASTIC_BAD_ARG;
   int k = -1;
@@ -674,6 +713,7 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   I.d32 = c->blk.d32;
   I.mg = merge_of(c, c->epart64);
   I.epart = c->epart64;
+  I.tdot = c->tdot;
   I.x = c->v64[VX];
   I.r = c->v64[VR];
   I.p = c->v64[VP];
@@ -687,6 +727,7 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   I.n = c->neq;
   I.nb = c->layout_nb;
   I.nch = nch;
+  I.cr = cr;
   I.fp32 = precision == ELASTIC_MIXED;
   I.acc32 = c->accum32 ? 1 : 0;
   I.live = 1;
