@@ -174,8 +174,9 @@ __global__ __launch_bounds__(kThreads) void k_bt_sum_shares(const BInst* __restr
 
 // alpha = d / p.q (mixed: p.r / p.q) ; q = own rows + merged external
 // partials (node by node, not stored) ; x += alpha p ; r -= alpha q ;
-// partials r.(m r) and (mixed) (m r).q
-template <int NB>
+// partials r.(m r) and (mixed) (m r).q.  EP: type of the stored partials
+// (FP32 after a product with FP32 sums).
+template <int NB, typename EP>
 __device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, double* sh) {
   const int J0 = c * (I.cr / NB), J1 = min(I.n / NB, J0 + I.cr / NB);
   double acc = 0.0, azq = 0.0;
@@ -192,9 +193,9 @@ __device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, doub
       mv[r] = I.m[i0 + r];
     }
     for (int k = k0; k < k1; ++k) {
-      const double* e = I.epart + (int64_t)NB * I.mg.idx[k];
+      const EP* e = reinterpret_cast<const EP*>(I.epart) + (int64_t)NB * I.mg.idx[k];
 #pragma unroll
-      for (int r = 0; r < NB; ++r) qv[r] += e[r];
+      for (int r = 0; r < NB; ++r) qv[r] += (double)e[r];
     }
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
@@ -228,8 +229,15 @@ __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict_
     const double pq = I.sc->pq_tiles;
     const double num = I.fp32 ? inst_sum(I.part + 3 * I.nch, I.nch, sh) : I.dslot[cur];
     const double alpha = num / pq;
-    if (I.nb == 3) bt_update_chunk<3>(I, c, alpha, sh);
-    else bt_update_chunk<2>(I, c, alpha, sh);
+    // the FP32-sum variant served this instance's product (see k_bt_spmv)
+    const bool e32 = I.fp32 && I.acc32 && !I.sc->op64;
+    if (I.nb == 3) {
+      if (e32) bt_update_chunk<3, float>(I, c, alpha, sh);
+      else bt_update_chunk<3, double>(I, c, alpha, sh);
+    } else {
+      if (e32) bt_update_chunk<2, float>(I, c, alpha, sh);
+      else bt_update_chunk<2, double>(I, c, alpha, sh);
+    }
     if (c == 0 && threadIdx.x == 0) {
       I.sc->alpha = alpha;
       I.sc->pq = pq;

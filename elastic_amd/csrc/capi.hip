@@ -231,6 +231,7 @@ int elastic_destroy(elastic_ctx* x) {
   if (c->red64) hipFree(c->red64);
   for (hipEvent_t e : {c->ev0, c->ev1, c->ev2, c->ev3})
     if (e) hipEventDestroy(e);
+  for (hipEvent_t e : c->tev) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete x;
   return ELASTIC_OK;
@@ -369,6 +370,9 @@ int elastic_set_option(elastic_ctx* x, const char* name, int64_t value) {
   } else if (n == "check_every") {
     if (value < 1 || value > 4096) return ELASTIC_BAD_ARG;
     c->check_every = (int)value;
+  } else if (n == "time_products") {
+    if (value < 0 || value > 4096) return ELASTIC_BAD_ARG;
+    c->time_stride = (int)value;
   } else if (n == "tile_blocks" || n == "tile_nnz") {
     const bool blocks = n == "tile_blocks";
     if (blocks && (value < 32 || value > kBlkTileMax)) return ELASTIC_BAD_ARG;
@@ -424,7 +428,7 @@ int elastic_get_values(elastic_ctx* x, double* ad, double* al) {
 
 static int spmv_resident(Context* c, int precision, const double* x64, double* y64) {
   launch_spmv(c, precision == ELASTIC_MIXED, x64, y64, c->epart64);
-  launch_merge(c, c->epart64, y64);
+  launch_merge(c, c->epart64, y64, epart_f32(c, precision == ELASTIC_MIXED));
   return ELASTIC_OK;
 }
 
@@ -448,13 +452,22 @@ int elastic_time_spmv(elastic_ctx* x, int32_t precision, int32_t reps, double* t
   if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
   hipSetDevice(c->device);
   const bool f32 = precision == ELASTIC_MIXED;
-  auto tiles = [&]() { launch_spmv(c, f32, c->v64[VT], c->v64[VU], c->epart64); };
-  auto merge = [&]() { launch_merge(c, c->epart64, c->v64[VU]); };
-  for (int i = 0; i < 3; ++i) tiles(), merge();  // warm-up
+  // the product as PCG launches it: on node-blocked layouts with the tiles'
+  // shares of x.(Ax) (kernel time), then their sum and the merge (total)
+  double* dres = c->red64 + 8 * kRedBlocks + 32;  // clear of the other scalar slots
+  const DotOut shares{c->tdot, nullptr}, full{c->tdot, dres};
+  auto tiles = [&]() {
+    launch_spmv(c, f32, c->v64[VT], c->v64[VU], c->epart64, nullptr, c->tdot ? &shares : nullptr);
+  };
+  auto merge = [&]() {
+    launch_spmv(c, f32, c->v64[VT], c->v64[VU], c->epart64, nullptr, c->tdot ? &full : nullptr);
+    launch_merge(c, c->epart64, c->v64[VU], epart_f32(c, f32));
+  };
+  for (int i = 0; i < 3; ++i) merge();  // warm-up
   ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < reps; ++i) tiles();
   ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev1, c->stream));
-  for (int i = 0; i < reps; ++i) tiles(), merge();
+  for (int i = 0; i < reps; ++i) merge();
   ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev2, c->stream));
   ELASTIC_HIP_CHECK(c, hipEventSynchronize(c->ev2));
   float a = 0, b = 0;
@@ -463,6 +476,18 @@ int elastic_time_spmv(elastic_ctx* x, int32_t precision, int32_t reps, double* t
   if (kernel_ms) *kernel_ms = a / reps;
   if (total_ms) *total_ms = b / reps;
   ELASTIC_HIP_CHECK(c, hipGetLastError());
+  return ELASTIC_OK;
+}
+
+int elastic_product_timing(elastic_ctx* x, double* avg_ms, int64_t* count, int32_t reset) {
+  if (!x || !avg_ms || !count) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  *count = c->tprod_n;
+  *avg_ms = c->tprod_n ? c->tprod_ms / (double)c->tprod_n : 0.0;
+  if (reset) {
+    c->tprod_ms = 0.0;
+    c->tprod_n = 0;
+  }
   return ELASTIC_OK;
 }
 

@@ -173,6 +173,13 @@ struct Context {
   // autotuning/measurement
   double last_spmv_ms = 0.0;
   int check_every = 16;
+  // sampled event timing of PCG's iteration products: every time_stride-th
+  // product of an iteration is bracketed by events (0: off); totals accumulate
+  // over solves until read (elastic_product_timing)
+  int time_stride = 0;
+  std::vector<hipEvent_t> tev;  // 2 per sampled product of a poll chunk
+  double tprod_ms = 0.0;
+  int64_t tprod_n = 0;
   int zero_diag_row = -1;  // first row with ad == 0 (1-based), -1 if none
 };
 
@@ -187,6 +194,7 @@ struct MergeArgs {
   const int32_t* idx;
   const double* epart;
   int nb;
+  int ep32;  // the partials are FP32 (written by a product with FP32 sums)
 };
 __device__ inline double merge_row(const MergeArgs& m, int i, double s) {
   if (!m.ptr) return s;
@@ -195,6 +203,11 @@ __device__ inline double merge_row(const MergeArgs& m, int i, double s) {
     return s;
   }
   const int J = i / m.nb, r = i - J * m.nb;
+  if (m.ep32) {
+    const float* e = reinterpret_cast<const float*>(m.epart);
+    for (int k = m.ptr[J]; k < m.ptr[J + 1]; ++k) s += (double)e[(int64_t)m.nb * m.idx[k] + r];
+    return s;
+  }
   for (int k = m.ptr[J]; k < m.ptr[J + 1]; ++k) s += m.epart[(int64_t)m.nb * m.idx[k] + r];
   return s;
 }
@@ -203,8 +216,15 @@ __device__ inline double merge_row(const MergeArgs& m, int i, double s) {
 enum VecSlot { VX = 0, VR, VP, VQ, VB, VT, VU, VW };
 
 int set_error(Context* c, int code, const std::string& msg);
-inline MergeArgs merge_of(const Context* c, const double* epart) {
-  return MergeArgs{c->inc_ptr, c->inc_idx, epart, c->layout_nb};
+// Partials of a product with FP32 sums (node-blocked layouts) are stored as
+// FP32 (exactly the values summed); every other product stores FP64.
+inline bool epart_f32(const Context* c, bool fp32_operator) {
+  return fp32_operator && c->accum32 && c->layout_nb > 1;
+}
+// merge lists of the context; ep32: the partials were written by a product
+// with FP32 sums (epart_f32)
+inline MergeArgs merge_of(const Context* c, const double* epart, bool ep32 = false) {
+  return MergeArgs{c->inc_ptr, c->inc_idx, epart, c->layout_nb, ep32 ? 1 : 0};
 }
 // arrays derived from new A64 values (asynchronous on c->stream): the
 // Jacobi inverse, the FP32 operator (scalar tiles) or the block planes
@@ -241,14 +261,14 @@ void free_j2(void* j);
 template <typename TV, typename T>
 void launch_spmv_tiles(const DevTiles& t, const TV* ad, const TV* al, const T* x, T* y, T* epart,
                        hipStream_t s, const int32_t* skip = nullptr);
-void launch_merge(Context* c, const double* epart, double* y);
+void launch_merge(Context* c, const double* epart, double* y, bool ep32 = false);
 template <typename S, typename D>
 void launch_cast(int64_t n, const S* a, D* b, hipStream_t s);
 // y = A x with the active layout; fp32 selects the FP32-value operator.  The
 // external partials are left in epart for the merge lists c->inc_ptr/inc_idx.
 // Optional output of a node-blocked product: x.(A x), formed from per-tile
 // shares (part, one per tile) that a one-workgroup kernel adds in tile order
-// into *result.
+// into *result (result == null: the shares only).
 struct DotOut {
   double* part;
   double* result;

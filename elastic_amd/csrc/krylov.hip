@@ -355,15 +355,15 @@ int bicgstab_run(Context* c, bool bdiag, bool fp32_operator, double tol, int max
       double* rr_cur = rr + (it & 1);
       double* rr_next = rr + ((it + 1) & 1);
       launch_spmv(c, op32, z, v, c->epart64, &sc->done);
-      hipLaunchKernelGGL(k_merge_dot2, G, TB, 0, s, n, merge_of(c, c->epart64), v, r0,
-                         P[0], (const double*)nullptr, P[1], sc);
+      hipLaunchKernelGGL(k_merge_dot2, G, TB, 0, s, n, merge_of(c, c->epart64, epart_f32(c, op32)),
+                         v, r0, P[0], (const double*)nullptr, P[1], sc);
       hipLaunchKernelGGL(k_bi_half1, G, TB, 0, s, n, P[0], rr_cur, x, z, r, v, m, jac, P[1], sc);
       if (!jac) hipLaunchKernelGGL(k_apply_m, G, TB, 0, s, n, r, m, bm, z, nullptr, sc);
       hipLaunchKernelGGL(k_bi_check, dim3(1), TB, 0, s, P[1], P[1], 1, rr_next, sc);
       if (reliable) replace(true, nullptr);
       launch_spmv(c, op32, z, t, c->epart64, &sc->done);
-      hipLaunchKernelGGL(k_merge_dot2, G, TB, 0, s, n, merge_of(c, c->epart64), t, r,
-                         P[2], (const double*)t, P[3], sc);
+      hipLaunchKernelGGL(k_merge_dot2, G, TB, 0, s, n, merge_of(c, c->epart64, epart_f32(c, op32)),
+                         t, r, P[2], (const double*)t, P[3], sc);
       hipLaunchKernelGGL(k_bi_half2, G, TB, 0, s, n, P[2], P[3], x, z, r, t, r0, P[4], P[5], sc);
       hipLaunchKernelGGL(k_bi_check, dim3(1), TB, 0, s, P[4], P[5], 2, rr_next, sc);
       if (reliable) replace(false, rr_next);
@@ -622,7 +622,8 @@ int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit,
       ++nit;
       ++ni;
       launch_spmv(c, op32, g(i), t, c->epart64);
-      hipLaunchKernelGGL(k_gm_merge, G, TB, 0, s, n, merge_of(c, c->epart64), t, u);
+      hipLaunchKernelGGL(k_gm_merge, G, TB, 0, s, n, merge_of(c, c->epart64, epart_f32(c, op32)),
+                         t, u);
       double* w = g(i + 1);
       hipLaunchKernelGGL(k_gm_mdot, G, TB, 0, s, n, u, m, bm, w, g(1), P[0]);
       for (int j = 1; j <= i; ++j)

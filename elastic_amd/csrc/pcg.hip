@@ -138,8 +138,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int n, const double* __
 // k_pcg_update on the node-blocked layout (n = NB * nn), one lane per node:
 // the node's merge list is walked once for its NB rows (contiguous partials),
 // and the vector loads are issued before that gather.  p.q comes from the
-// product's tiles (sc->pq_tiles).
-template <int NB>
+// product's tiles (sc->pq_tiles).  EP: type of the stored partials.
+template <int NB, typename EP>
 __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs mg,
                                                                 const double* __restrict__ part_pr,
                                                                 const double* __restrict__ dslot,
@@ -168,9 +168,9 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
       mv[c] = m[i0 + c];
     }
     for (int k = k0; k < k1; ++k) {
-      const double* e = mg.epart + (int64_t)NB * mg.idx[k];
+      const EP* e = reinterpret_cast<const EP*>(mg.epart) + (int64_t)NB * mg.idx[k];
 #pragma unroll
-      for (int c = 0; c < NB; ++c) qv[c] += e[c];
+      for (int c = 0; c < NB; ++c) qv[c] += (double)e[c];
     }
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
@@ -355,25 +355,46 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   const int every = std::max(2, c->check_every & ~1);
   bool done = false;
   bool op32 = fp32_operator;  // the progress guard may hand over to the FP64 operator
+  // sampled event timing of the products (measurement option)
+  const int stride = c->time_stride;
+  if (stride > 0 && c->tev.size() < (size_t)2 * every) {
+    for (hipEvent_t e : c->tev) hipEventDestroy(e);
+    c->tev.assign(2 * every, nullptr);
+    // timing only: no system-scope fence (no cache writeback/invalidation)
+    // when an event completes, which would slow the kernel after it
+    for (auto& e : c->tev)
+      ELASTIC_HIP_CHECK(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+  }
   while (it < maxit) {
     const int chunk = std::min(every, maxit - it);
+    const int it0 = it;
     for (int j = 0; j < chunk; ++j, ++it) {
       double* dcur = dslots + (it & 1);
       double* dnxt = dslots + ((it + 1) & 1);
+      const bool timed = stride > 0 && it % stride == 0;
+      if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j], s));
       if (tdots) {
         launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done, &dot);
+        if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j + 1], s));
         // q is read again only by the block preconditioner's (M^-1 r).q
         const int keep_q = (!jac && reliable) ? 1 : 0;
-        if (c->layout_nb == 3)
-          hipLaunchKernelGGL(k_pcg_update_nodes<3>, G, TB, 0, s, n / 3, merge_of(c, V.epart),
-                             (const double*)part_pr, dcur, V.x, V.p, V.r, V.q, keep_q, V.m, jac,
-                             part_b, part_zq, sc);
-        else
-          hipLaunchKernelGGL(k_pcg_update_nodes<2>, G, TB, 0, s, n / 2, merge_of(c, V.epart),
-                             (const double*)part_pr, dcur, V.x, V.p, V.r, V.q, keep_q, V.m, jac,
-                             part_b, part_zq, sc);
+        const bool e32 = epart_f32(c, op32);
+        const MergeArgs mg = merge_of(c, V.epart, e32);
+#define ELASTIC_UPD(NB, EP)                                                                    \
+  hipLaunchKernelGGL((k_pcg_update_nodes<NB, EP>), G, TB, 0, s, n / NB, mg,                    \
+                     (const double*)part_pr, dcur, V.x, V.p, V.r, V.q, keep_q, V.m, jac, part_b, \
+                     part_zq, sc)
+        if (c->layout_nb == 3) {
+          if (e32) ELASTIC_UPD(3, float);
+          else ELASTIC_UPD(3, double);
+        } else {
+          if (e32) ELASTIC_UPD(2, float);
+          else ELASTIC_UPD(2, double);
+        }
+#undef ELASTIC_UPD
       } else {
         launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done);
+        if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j + 1], s));
         hipLaunchKernelGGL(k_merge_dot, G, TB, 0, s, n, merge_of(c, V.epart), V.q, V.p, part_a,
                            (const double*)V.r, part_pr, sc);
         hipLaunchKernelGGL(k_pcg_update, G, TB, 0, s, n, (const double*)part_a,
@@ -391,6 +412,15 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
     ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->scal_host, sc, sizeof(SolveScalars),
                                         hipMemcpyDeviceToHost, s));
     ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
+    if (stride > 0) {  // products of iterations that ran (later ones return at once)
+      for (int j = 0; j < chunk; ++j)
+        if ((it0 + j) % stride == 0 && it0 + j < c->scal_host->iter) {
+          float ms = 0.f;
+          ELASTIC_HIP_CHECK(c, hipEventElapsedTime(&ms, c->tev[2 * j], c->tev[2 * j + 1]));
+          c->tprod_ms += ms;
+          c->tprod_n += 1;
+        }
+    }
     if (c->scal_host->done) {
       done = true;
       break;

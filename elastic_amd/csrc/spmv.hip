@@ -137,10 +137,10 @@ void launch_spmv_tiles(const DevTiles& t, const TV* ad, const TV* al, const T* x
 #undef ELASTIC_SPMV_LAUNCH
 }
 
-void launch_merge(Context* c, const double* epart, double* y) {
+void launch_merge(Context* c, const double* epart, double* y, bool ep32) {
   const int neq = c->neq;
   int blocks = std::max(1, std::min((neq + 255) / 256, 4096));
-  hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(256), 0, c->stream, neq, merge_of(c, epart), y);
+  hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(256), 0, c->stream, neq, merge_of(c, epart, ep32), y);
 }
 
 template void launch_spmv_tiles<double, double>(const DevTiles&, const double*, const double*,

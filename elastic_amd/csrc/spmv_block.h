@@ -264,6 +264,10 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
     if (c < nr) {
 #pragma unroll
       for (int r = 0; r < NB; ++r) y[(int64_t)NB * (I0 + c) + r] = (double)s[r];
+    } else if constexpr (sizeof(T) == 4) {  // FP32 sums: FP32 partials (exact)
+      float* ef = reinterpret_cast<float*>(epart);
+#pragma unroll
+      for (int r = 0; r < NB; ++r) ef[(int64_t)NB * (ext0 + c - nr) + r] = s[r];
     } else {
 #pragma unroll
       for (int r = 0; r < NB; ++r) epart[(int64_t)NB * (ext0 + c - nr) + r] = (double)s[r];

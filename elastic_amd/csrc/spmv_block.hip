@@ -120,8 +120,9 @@ static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* dia
   } while (0)
   if (dot) {
     ELASTIC_BLK_PER(true);
-    hipLaunchKernelGGL(k_sum_shares, dim3(1), dim3(1024), 0, s, (const double*)dot->part,
-                       b.ntiles, dot->result, skip);
+    if (dot->result)  // (null: the shares only)
+      hipLaunchKernelGGL(k_sum_shares, dim3(1), dim3(1024), 0, s, (const double*)dot->part,
+                         b.ntiles, dot->result, skip);
   } else {
     ELASTIC_BLK_PER(false);
   }

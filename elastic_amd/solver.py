@@ -79,6 +79,13 @@ class Context:
         """Tuning knob of the context (see elastic_set_option)."""
         self._chk(self._L.elastic_set_option(self._h, name.encode(), int(value)))
 
+    def product_timing(self, reset: bool = True):
+        """(average ms, count) of the PCG iteration products sampled under
+        the "time_products" option since the last reset."""
+        ms, n = C.c_double(), C.c_int64()
+        self._chk(self._L.elastic_product_timing(self._h, C.byref(ms), C.byref(n), 1 if reset else 0))
+        return ms.value, n.value
+
     def set_tile_blocks(self, tile_blocks: int):
         self._chk(self._L.elastic_set_tile_blocks(self._h, tile_blocks))
 

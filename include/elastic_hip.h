@@ -214,8 +214,16 @@ int elastic_set_tile_blocks(elastic_ctx* ctx, int32_t tile_blocks);
  * "tile_nnz" (tile size; rebuilds the layout at once when a structure is
  * set), "accum32" (1: the FP32 operator stages x and sums in FP32, faster
  * but less accurate, which ill-conditioned systems may not tolerate; 0, the
- * default: in FP64), "check_every" (iterations between convergence polls). */
+ * default: in FP64), "check_every" (iterations between convergence polls),
+ * "time_products" (S > 0: PCG brackets the product of every S-th iteration
+ * with HIP events on its stream; 0, the default: off). */
 int elastic_set_option(elastic_ctx* ctx, const char* name, int64_t value);
+/* Average device time of the PCG iteration products sampled under the
+ * "time_products" option, and how many were sampled, since the last reset
+ * (reset != 0 clears the totals after reading).  Measurement only; no
+ * counterpart in the reference (its matvectime timer, Iters.f:383-386, is
+ * the closest). */
+int elastic_product_timing(elastic_ctx* ctx, double* avg_ms, int64_t* count, int32_t reset);
 /* Autotuner (no counterpart in the reference): with values resident, times
  * the product of the given precision for each candidate tile size (HIP
  * events, reps launches each), rebuilds the layout with the fastest and
