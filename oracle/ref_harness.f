@@ -13,11 +13,15 @@ c *   mode 2 = + gmres(k=50) called with its own signature (Iters.f:528)*
 c *   mode 4 = + pbicgstab with its own signature (Iters.f:1079)       *
 c *   mode 8 = + block_precond nblock=3 (Precond.f:163)                *
 c *   mode 16 = element loop by colours (omp_elmt, Pform.f:108-184)    *
+c *   mode 32 = + nodal stresses (tform_mec, Pform.f:258) and the      *
+c *             result files of write_mesh_res_mec (Write_vtk.f:742):  *
+c *             legacy ASCII, legacy binary and XML ASCII              *
 c *   (modes add up)                                                   *
 c * Output (raw little-endian stream files in outdir):                 *
 c *   meta.txt, inum.i4, id.i4, ia.i4, ja.i4, ad.f8, al.f8, b.f8,      *
 c *   x.f8, spmv_v.f8, spmv_y.f8, resid.f8, and optionally             *
-c *   x_gmres.f8, x_bicgstab.f8, bdiag.f8                              *
+c *   x_gmres.f8, x_bicgstab.f8, bdiag.f8, tx.f8, u.f8 and the files  *
+c *   res_istep_0.vtk, resb_istep_0.vtk, res_istep_0.vtu               *
 c **********************************************************************
       program ref_harness
       use Malloc
@@ -39,7 +43,10 @@ c **********************************************************************
       integer*8 i_z,i_r,i_s,i_v,i_y,i_g,i_h,i_c,i_t,i_p,i_r0
       integer*8 i_colorg,i_elcolor
       integer*8 i_bd,i_aux,mm
-      integer param(4),nbd
+      integer param(4),nbd,ntn
+      integer*8 i_tx,i_ic
+      logical pflag(10)
+      character*80 prename
       real*8 solvtol,tt,dot_par
       logical fstress0,fmec,pq
       character*8 sia,sja,sau,sal,sad,mc
@@ -259,6 +266,35 @@ c ... 3x3 block-Jacobi set-up (Precond.f:163), dead in the driver
      .       ia(i_bd),neq,ia(i_aux),param)
         nbd = 6*int(mm)
         call wf8(trim(outd)//'/bdiag.f8',ia(i_bd),nbd)
+      endif
+c ... nodal stresses and the result files (Main.f:930-963), after the
+c     displacements of the solve went to u (update_mec above)
+      if (iand(mode,32) .ne. 0) then
+        ntn = 4
+        if (ndm .eq. 3) ntn = 6
+        i_tx = alloc_8('tx      ',ntn,nnode)
+        i_ic = alloc_4('ic      ',1,nnode)
+        call azero(ia(i_tx),nnode*ntn)
+        call mzero(ia(i_ic),nnode)
+        call tform_mec(ia(i_ix),ia(i_x),ia(i_e),ia(i_ie),ia(i_ic),
+     .       ia(i_xl),ia(i_ul),ia(i_txnl),ia(i_u),ia(i_tx0),ia(i_tx),
+     .       nnodev,nnode,numel,nenv,nen,ndm,ndf,nst,ntn,3,.false.,1)
+        call wf8(trim(outd)//'/tx.f8',ia(i_tx),ntn*nnode)
+        call wf8(trim(outd)//'/u.f8',ia(i_u),ndf*nnode)
+        pflag = .false.
+        pflag(2) = .true.
+        pflag(3) = .true.
+        prename = trim(outd)//'/res'
+        call write_mesh_res_mec(ia(i_ix),ia(i_x),ia(i_u),ia(i_tx),
+     .       nnode,numel,nen,ndm,ndf,ntn,prename,0,.false.,.true.,
+     .       pflag,31)
+        call write_mesh_res_mec(ia(i_ix),ia(i_x),ia(i_u),ia(i_tx),
+     .       nnode,numel,nen,ndm,ndf,ntn,prename,0,.false.,.false.,
+     .       pflag,31)
+        prename = trim(outd)//'/resb'
+        call write_mesh_res_mec(ia(i_ix),ia(i_x),ia(i_u),ia(i_tx),
+     .       nnode,numel,nen,ndm,ndf,ntn,prename,0,.true.,.true.,
+     .       pflag,31)
       endif
       close(10)
       close(20)

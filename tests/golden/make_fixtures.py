@@ -4,7 +4,8 @@ The reference (Fortran, /root/reference/mef/src) is compiled by
 oracle/Makefile.ref into oracle/_ref/ref_harness, a test-only driver that runs
 the reference's own mesh/solvm path (RCM, equation numbering, CSRC structure,
 element stiffness + load assembly, PCG through ``solv``) and additionally calls
-``gmres``, ``pbicgstab`` and ``block_precond`` with their own signatures.
+``gmres``, ``pbicgstab`` and ``block_precond`` with their own signatures, and
+``tform_mec`` + ``write_mesh_res_mec`` for the nodal stresses and result files.
 For every case this script writes a deck with elastic_amd.deck, runs the
 harness, and stores its outputs as a small .npz fixture (large cases keep
 SHA-256 digests of the integer arrays and summaries of the floating-point ones).
@@ -50,10 +51,16 @@ def run_harness(deck_path, tol, maxit, mode, workdir, nth=1):
                b=rd("b.f8", np.float64), x=rd("x.f8", np.float64),
                spmv_v=rd("spmv_v.f8", np.float64), spmv_y=rd("spmv_y.f8", np.float64),
                resid=rd("resid.f8", np.float64))
-    for name in ("x_gmres", "x_bicgstab", "bdiag"):
+    for name in ("x_gmres", "x_bicgstab", "bdiag", "tx", "u"):
         p = os.path.join(workdir, name + ".f8")
         if os.path.exists(p):
             out[name] = np.fromfile(p, dtype=np.float64)
+    # result files of write_mesh_res_mec, kept byte for byte
+    for key, fname in (("vtk_ascii", "res_istep_0.vtk"), ("vtk_binary", "resb_istep_0.vtk"),
+                       ("vtu_ascii", "res_istep_0.vtu")):
+        p = os.path.join(workdir, fname)
+        if os.path.exists(p):
+            out[key] = np.frombuffer(open(p, "rb").read(), dtype=np.uint8)
     log = open(os.path.join(workdir, "solver.log")).read()
     stats = {}
     for key, pat in (("pcg", r"PCG:\s+it\s+(\d+) x \* Kx\s+(\S+) \|\|x\|\|\s+(\S+)"),
