@@ -90,6 +90,7 @@ def lib():
                                    I32P, F64P]
     L.elastic_assemble.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, vp]
     L.elastic_time_assembly.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.elastic_stress.argtypes = [vp, vp, C.c_int32, vp, vp]
     L.elastic_j2_setup.argtypes = [vp, C.c_double, C.c_double, vp]
     L.elastic_j2_increment.argtypes = [vp, C.c_double]
     L.elastic_j2_assemble.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]

@@ -566,6 +566,10 @@ struct MeshDev {
   int32_t *ix = nullptr, *id = nullptr, *ie = nullptr, *eload = nullptr, *order = nullptr,
           *slots = nullptr, *ltype = nullptr, *lnparc = nullptr, *err = nullptr;
   double *x = nullptr, *e = nullptr, *lf = nullptr, *u = nullptr, *b = nullptr;
+  // nodal stresses: (element, local node) pairs of every node in ascending
+  // element order, and the buffers of the stress pass (allocated on first use)
+  int32_t *inc_ptr = nullptr, *inc = nullptr;
+  double *pl = nullptr, *tx = nullptr, *tx0 = nullptr;
 };
 
 int color_elements(const int32_t* ix, int nnode, int numel, int nen, int nenv,
@@ -576,7 +580,8 @@ void element_slots(const int32_t* ix, const int32_t* id, int numel, int nen, int
 void free_mesh(MeshDev* m) {
   if (!m) return;
   void* ptrs[] = {m->ix, m->id, m->ie, m->eload, m->order, m->slots, m->ltype,
-                  m->lnparc, m->err, m->x, m->e, m->lf, m->u, m->b};
+                  m->lnparc, m->err, m->x, m->e, m->lf, m->u, m->b,
+                  m->inc_ptr, m->inc, m->pl, m->tx, m->tx0};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete m;
@@ -616,6 +621,178 @@ static void launch_loads(const AsmArgs& a, const MeshDev* m, hipStream_t s) {
   }
 }
 
+// ------------------------------------------------- nodal stresses (isw = 3)
+// tform_mec (Pform.f:258-421) with the elements' isw = 3 branches (label 300
+// of elmt02..07, Elmlib02.f): the element's stresses at its nodes, then per
+// node the sum over its elements in element order, divided by their count,
+// plus the initial stress.  Same operations in the same order as the
+// reference (FP contraction off), so tx is bit-identical to tform_mec's.
+struct StressArgs {
+  const int32_t* ix;
+  const double* x;
+  const int32_t* ie;
+  const double* e;
+  const double* u;   // ndf per node
+  double* pl;        // ntn per (element, local node)
+  int32_t* err;      // an element (1-based) with a non-positive Jacobian at a node
+  int numel;
+};
+
+// deform2d / deform3d (Elmlib.f:2368-2472)
+template <int NEN>
+__device__ inline void deform2d_d(const double* hx, const double* hy, const double* u,
+                                  double* eps) {
+  double ux = 0.0, uy = 0.0, vx = 0.0, vy = 0.0;
+  for (int i = 0; i < NEN; ++i) {
+    ux = ux + hx[i] * u[2 * i];
+    uy = uy + hy[i] * u[2 * i];
+    vx = vx + hx[i] * u[2 * i + 1];
+    vy = vy + hy[i] * u[2 * i + 1];
+  }
+  eps[0] = ux;
+  eps[1] = vy;
+  eps[2] = uy + vx;
+}
+
+template <int NEN>
+__device__ inline void deform3d_d(const double* hx, const double* hy, const double* hz,
+                                  const double* u, double* eps) {
+  double ux = 0.0, uy = 0.0, uz = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+  for (int i = 0; i < NEN; ++i) {
+    ux = ux + hx[i] * u[3 * i];
+    uy = uy + hy[i] * u[3 * i];
+    uz = uz + hz[i] * u[3 * i];
+    vx = vx + hx[i] * u[3 * i + 1];
+    vy = vy + hy[i] * u[3 * i + 1];
+    vz = vz + hz[i] * u[3 * i + 1];
+    wx = wx + hx[i] * u[3 * i + 2];
+    wy = wy + hy[i] * u[3 * i + 2];
+    wz = wz + hz[i] * u[3 * i + 2];
+  }
+  eps[0] = ux;
+  eps[1] = vy;
+  eps[2] = wz;
+  eps[3] = uy + vx;
+  eps[4] = vz + wy;
+  eps[5] = uz + wx;
+}
+
+// stress2d_m (Elmlib.f:2502-2539) and stress3d (Elmlib.f:2540-2573)
+__device__ inline void stress2d_m_d(double d11, double d12, double d22, double d33, double ps,
+                                    bool ept, const double* eps, double* t) {
+  t[0] = d11 * eps[0] + d12 * eps[1];
+  t[1] = d12 * eps[0] + d22 * eps[1];
+  t[2] = ept ? ps * (t[0] + t[1]) : 0.0;
+  t[3] = d33 * eps[2];
+}
+
+__device__ inline void stress3d_d(double a, double b, double c, const double* eps, double* t) {
+  t[0] = (eps[0] + b * eps[1] + b * eps[2]) * a;
+  t[1] = (b * eps[0] + eps[1] + b * eps[2]) * a;
+  t[2] = (b * eps[0] + b * eps[1] + eps[2]) * a;
+  t[3] = eps[3] * c * a;
+  t[4] = eps[4] * c * a;
+  t[5] = eps[5] * c * a;
+}
+
+// One lane per (element, local node): the stresses at that node.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_stress_el(StressArgs a) {
+  using TR = ElemTraits<KIND>;
+  constexpr int NEN = TR::NEN, NDF = TR::NDF, NDM = TR::NDM, NTN = NDM == 3 ? 6 : 4;
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= (int64_t)a.numel * NEN) return;
+  const int el = (int)(q / NEN), node = (int)(q - (int64_t)el * NEN);
+  const int32_t* ixe = a.ix + (size_t)el * (NEN + 1);
+  double xl[NEN * NDM], ul[NEN * NDF];
+  for (int n = 0; n < NEN; ++n) {
+    const size_t g = (size_t)(ixe[n] - 1);
+    for (int d = 0; d < NDM; ++d) xl[n * NDM + d] = a.x[g * NDM + d];
+    for (int d = 0; d < NDF; ++d) ul[n * NDF + d] = a.u[g * NDF + d];
+  }
+  const int ma = ixe[NEN];
+  const int iel = a.ie[ma - 1];
+  const double* em = a.e + (size_t)(ma - 1) * 15;
+  const double ym = em[0], ps = em[1];
+  const bool pstress = (iel == 3 || iel == 5);
+  double ca, cb, cc;
+  if (!pstress) {
+    const double a1 = 1.0 - ps, a2 = 1.0 - 2.0 * ps, a3 = 1.0 + ps;
+    ca = (ym * a1) / (a3 * a2);
+    cb = ps / a1;
+    cc = 0.5 * (a2 / a1);
+  } else {
+    const double a1 = 1.0 - ps, a2 = 1.0 - ps * ps;
+    ca = ym / a2;
+    cb = ps;
+    cc = 0.5 * a1;
+  }
+  double t[NTN], eps[6], det;
+  bool ok = true;
+  if constexpr (KIND == K_TRIA3) {
+    const double xj11 = xl[0] - xl[4], xj12 = xl[1] - xl[5];
+    const double xj21 = xl[2] - xl[4], xj22 = xl[3] - xl[5];
+    det = xj11 * xj22 - xj12 * xj21;
+    ok = det > 0.0;
+    const double ji11 = xj22 / det, ji12 = -xj12 / det, ji21 = -xj21 / det, ji22 = xj11 / det;
+    const double hx[3] = {ji11, ji12, -ji11 - ji12}, hy[3] = {ji21, ji22, -ji21 - ji22};
+    deform2d_d<3>(hx, hy, ul, eps);
+    stress2d_m_d(ca, ca * cb, ca, ca * cc, ps, !pstress, eps, t);
+  } else if constexpr (KIND == K_QUAD4) {
+    const double rn = (node == 0 || node == 3) ? 1.0 : -1.0, sn = node < 2 ? 1.0 : -1.0;
+    double h[4], hx[4], hy[4];
+    sfquad4_d(rn, sn, h, hx, hy);
+    ok = jacob2d_d(hx, hy, xl, 4, 2, &det);
+    deform2d_d<4>(hx, hy, ul, eps);
+    stress2d_m_d(ca, ca * cb, ca, ca * cc, ps, !pstress, eps, t);
+  } else if constexpr (KIND == K_TET4) {
+    double hx[4], hy[4], hz[4];
+    ok = jtetra4_d(xl, hx, hy, hz, &det, true);
+    deform3d_d<4>(hx, hy, hz, ul, eps);
+    stress3d_d(ca, cb, cc, eps, t);
+  } else {
+    const int c4 = node & 3;
+    const double rn = (c4 == 0 || c4 == 3) ? 1.0 : -1.0, sn = c4 < 2 ? 1.0 : -1.0;
+    const double tn = node < 4 ? 1.0 : -1.0;
+    double h[8], hx[8], hy[8], hz[8];
+    sfhexa8_d(rn, sn, tn, h, hx, hy, hz);
+    ok = jacob3d_d(hx, hy, hz, xl, 8, &det);
+    deform3d_d<8>(hx, hy, hz, ul, eps);
+    stress3d_d(ca, cb, cc, eps, t);
+  }
+  if (!ok) atomicCAS(a.err, 0, el + 1);
+  for (int j = 0; j < NTN; ++j) a.pl[q * NTN + j] = t[j];
+}
+
+// One lane per node: the node's element rows added in element order (the
+// incidence list), / count, + initial stress (tx0 may be null).
+__global__ __launch_bounds__(256) void k_stress_nodes(int nnode, int ntn,
+                                                       const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ inc,
+                                                       const double* __restrict__ pl,
+                                                       const double* __restrict__ tx0,
+                                                       double* __restrict__ tx) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= nnode) return;
+  double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int k0 = ptr[n], k1 = ptr[n + 1];
+  for (int k = k0; k < k1; ++k) {
+    const double* p = pl + (size_t)inc[k] * ntn;
+    for (int j = 0; j < ntn; ++j) t[j] = t[j] + p[j];
+  }
+  const double ic = (double)(k1 - k0);
+  for (int j = 0; j < ntn; ++j)
+    tx[(size_t)n * ntn + j] = t[j] / ic + (tx0 ? tx0[(size_t)n * ntn + j] : 0.0);
+}
+
+// update_mec (Predict.f:183-209) on the device: u(j,i) = x(id(j,i)) where the
+// dof is free
+__global__ __launch_bounds__(256) void k_update_u(int n, const int32_t* __restrict__ id,
+                                                   const double* __restrict__ x, double* u) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q < n && id[q] > 0) u[q] = x[id[q] - 1];
+}
+
 }  // namespace elastic
 
 using namespace elastic;
@@ -650,6 +827,15 @@ extern "C" int elastic_set_mesh(elastic_ctx* xc, int32_t nnode, int32_t numel, i
   const int want_nen[] = {3, 4, 4, 8}, want_ndf[] = {2, 2, 3, 3}, want_ndm[] = {2, 2, 3, 3};
   if (nen != want_nen[kind] || ndf != want_ndf[kind] || ndm != want_ndm[kind])
     return set_error(c, ELASTIC_BAD_ARG, "nen/ndf/ndm do not match the element type");
+  // every index the device kernels follow stays in range
+  for (int el = 0; el < numel; ++el)
+    for (int i = 0; i < nen; ++i) {
+      const int no = ix[(size_t)el * (nen + 1) + i];
+      if (no < 1 || no > nnode)
+        return set_error(c, ELASTIC_BAD_ARG, "node number out of range in element " + std::to_string(el + 1));
+    }
+  for (size_t q = 0; q < (size_t)nnode * ndf; ++q)
+    if (id[q] > c->neq) return set_error(c, ELASTIC_BAD_ARG, "equation number above neq");
   std::vector<int32_t> icolor;
   int ncol = color_elements(ix, nnode, numel, nen, nen, icolor);
   if (ncol < 0) return set_error(c, ELASTIC_BAD_ARG, "more than 200 element colours");
@@ -699,6 +885,25 @@ extern "C" int elastic_set_mesh(elastic_ctx* xc, int32_t nnode, int32_t numel, i
   chk(up<double>(&m->u, nullptr, (size_t)nnode * ndf, s));
   chk(up<double>(&m->b, nullptr, (size_t)c->neq, s));
   chk(up<int32_t>(&m->err, nullptr, 1, s));
+  {  // node -> (element, local node) in ascending element order (tform_mec's order)
+    std::vector<int32_t> ptr((size_t)nnode + 1, 0), inc;
+    for (int el = 0; el < numel; ++el)
+      for (int i = 0; i < nen; ++i) {
+        const int no = ix[(size_t)el * (nen + 1) + i];
+        if (no > 0 && no <= nnode) ptr[no] += 1;
+      }
+    for (int n = 0; n < nnode; ++n) ptr[n + 1] += ptr[n];
+    inc.resize(std::max<size_t>(ptr[nnode], 1));
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int el = 0; el < numel; ++el)
+      for (int i = 0; i < nen; ++i) {
+        const int no = ix[(size_t)el * (nen + 1) + i];
+        if (no > 0 && no <= nnode) inc[fill[no - 1]++] = el * nen + i;
+      }
+    chk(up(&m->inc_ptr, ptr.data(), ptr.size(), s));
+    chk(up(&m->inc, inc.data(), inc.size(), s));
+    chk(hipStreamSynchronize(s));  // the host vectors go out of scope
+  }
   chk(hipStreamSynchronize(s));
   if (r != hipSuccess) {
     free_mesh(m);
@@ -773,6 +978,57 @@ extern "C" int elastic_assemble(elastic_ctx* xc, int32_t isw, int32_t lhs, int32
     c->values_set = true;
     c->bdiag_set = false;
   }
+  return ELASTIC_OK;
+}
+
+// tform_mec (Pform.f:258-421, isw = 3): nodal stresses tx(ntn, nnode) of the
+// mesh from u(ndf, nnode) (NULL: zero); from_x: the free dofs of u are first
+// replaced by the context's resident solution x (update_mec).  tx0 (ntn,
+// nnode) may be NULL.
+extern "C" int elastic_stress(elastic_ctx* xc, const double* u, int32_t from_x, const double* tx0,
+                              double* tx) {
+  if (!xc || !tx) return ELASTIC_BAD_ARG;
+  Context* c = &xc->c;
+  MeshDev* m = (MeshDev*)c->mesh;
+  if (!m) return set_error(c, ELASTIC_NOT_READY, "mesh not set");
+  if (from_x && !c->v64[VX]) return set_error(c, ELASTIC_NOT_READY, "no resident solution");
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  const int ntn = m->ndm == 3 ? 6 : 4;
+  const size_t nu = (size_t)m->nnode * m->ndf, nt = (size_t)m->nnode * ntn;
+  const size_t npl = (size_t)m->numel * m->nen * ntn;
+  hipError_t r = hipSuccess;
+  if (!m->pl) r = up<double>(&m->pl, nullptr, npl, s);
+  if (r == hipSuccess && !m->tx) r = up<double>(&m->tx, nullptr, nt, s);
+  if (r == hipSuccess && tx0 && !m->tx0) r = up<double>(&m->tx0, nullptr, nt, s);
+  if (r != hipSuccess)
+    return set_error(c, r == hipErrorOutOfMemory ? ELASTIC_OOM : ELASTIC_HIP_ERROR, "stress buffers");
+  if (u) ELASTIC_HIP_CHECK(c, hipMemcpyAsync(m->u, u, sizeof(double) * nu, hipMemcpyHostToDevice, s));
+  else ELASTIC_HIP_CHECK(c, hipMemsetAsync(m->u, 0, sizeof(double) * nu, s));
+  if (tx0) ELASTIC_HIP_CHECK(c, hipMemcpyAsync(m->tx0, tx0, sizeof(double) * nt, hipMemcpyHostToDevice, s));
+  if (from_x)
+    hipLaunchKernelGGL(k_update_u, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, (int)nu,
+                       (const int32_t*)m->id, (const double*)c->v64[VX], m->u);
+  ELASTIC_HIP_CHECK(c, hipMemsetAsync(m->err, 0, sizeof(int32_t), s));
+  StressArgs a{m->ix, m->x, m->ie, m->e, m->u, m->pl, m->err, m->numel};
+  const dim3 ge((unsigned)(((int64_t)m->numel * m->nen + 255) / 256));
+  switch (m->kind) {
+    case K_TRIA3: hipLaunchKernelGGL(k_stress_el<K_TRIA3>, ge, dim3(256), 0, s, a); break;
+    case K_QUAD4: hipLaunchKernelGGL(k_stress_el<K_QUAD4>, ge, dim3(256), 0, s, a); break;
+    case K_TET4: hipLaunchKernelGGL(k_stress_el<K_TET4>, ge, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_stress_el<K_HEX8>, ge, dim3(256), 0, s, a); break;
+  }
+  hipLaunchKernelGGL(k_stress_nodes, dim3((m->nnode + 255) / 256), dim3(256), 0, s, m->nnode, ntn,
+                     (const int32_t*)m->inc_ptr, (const int32_t*)m->inc, (const double*)m->pl,
+                     (const double*)(tx0 ? m->tx0 : nullptr), m->tx);
+  int32_t err = 0;
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(&err, m->err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(tx, m->tx, sizeof(double) * nt, hipMemcpyDeviceToHost, s));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
+  ELASTIC_HIP_CHECK(c, hipGetLastError());
+  if (err)
+    return set_error(c, ELASTIC_NEG_JACOBIAN,
+                     "non-positive Jacobian at a node of element " + std::to_string(err));
   return ELASTIC_OK;
 }
 

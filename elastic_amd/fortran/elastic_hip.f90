@@ -112,6 +112,16 @@ module elastic_hip
       real(c_double), intent(in) :: u(*)
       real(c_double), intent(inout) :: b(*)
     end function
+    ! tform_mec (Pform.f:258-421) on the GPU: nodal stresses tx(ntn, nnode);
+    ! tx0 is c_loc of the initial stresses or c_null_ptr
+    integer(c_int) function elastic_stress(ctx, u, from_x, tx0, tx) bind(C, name="elastic_stress")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: u(*)
+      integer(c_int32_t), value :: from_x
+      type(c_ptr), value :: tx0
+      real(c_double), intent(out) :: tx(*)
+    end function
     integer(c_int) function elastic_upload_rhs(ctx, b) bind(C, name="elastic_upload_rhs")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -85,6 +85,29 @@ class System:
         self._assemble(2, 0, 1, np.ascontiguousarray(u), b)
         return b
 
+    def displacements(self, x):
+        """u(nnode, ndf) of update_mec (Predict.f:183-209): x on the free dofs,
+        the prescribed values elsewhere."""
+        u = self.u.copy()
+        free = self.idn > 0
+        u[free] = x[self.idn[free] - 1]
+        return u
+
+    def stress(self, u=None, from_x: bool = False, tx0=None):
+        """Nodal stresses tx (nnode, ntn) of tform_mec on the GPU (Pform.f:258-421):
+        ntn = 4 in 2-D (sxx syy szz sxy), 6 in 3-D (sxx syy szz sxy syz sxz).
+        u (nnode, ndf) defaults to the prescribed displacements; from_x puts
+        the context's resident solution on the free dofs first (update_mec)."""
+        m = self.mesh
+        ntn = 6 if m.ndm == 3 else 4
+        uu = np.ascontiguousarray(self.u if u is None else u, np.float64)
+        t0 = None if tx0 is None else np.ascontiguousarray(tx0, np.float64)
+        tx = np.zeros((m.nnode, ntn))
+        self.ctx._chk(self.ctx._L.elastic_stress(
+            self.ctx._h, uu.ctypes.data_as(C.c_void_p), 1 if from_x else 0,
+            None if t0 is None else t0.ctypes.data_as(C.c_void_p), tx.ctypes.data_as(C.c_void_p)))
+        return tx
+
     def values(self):
         return self.ctx.get_values()
 

@@ -125,6 +125,17 @@ int elastic_set_mesh(elastic_ctx* ctx, int32_t nnode, int32_t numel, int32_t nen
  * on the host.  Use elastic_get_values to copy K back. */
 int elastic_assemble(elastic_ctx* ctx, int32_t isw, int32_t lhs, int32_t rhs, const double* u,
                      double* b);
+/* tform_mec (Pform.f:258-421) with the elements' isw = 3 (label 300 of
+ * elmt02..07, Elmlib02.f) on the GPU: the nodal stresses tx(ntn, nnode),
+ * ntn = 4 in 2-D (sxx syy szz sxy) or 6 in 3-D (sxx syy szz sxy syz sxz),
+ * each node's element values averaged in element order plus the initial
+ * stress tx0(ntn, nnode) (NULL: zero).  u(ndf, nnode) on the host (NULL:
+ * zero); from_x != 0 first replaces its free dofs by the context's resident
+ * solution x, as update_mec (Predict.f:183-209) does before tform_mec
+ * (Main.f:946).  Bit-identical to the reference's tform_mec.  A
+ * non-positive Jacobian at an element node returns ELASTIC_NEG_JACOBIAN. */
+int elastic_stress(elastic_ctx* ctx, const double* u, int32_t from_x, const double* tx0,
+                   double* tx);
 /* Average device time (ms) of one stiffness assembly (isw 2, lhs) over reps. */
 int elastic_time_assembly(elastic_ctx* ctx, int32_t reps, double* ms);
 

@@ -73,6 +73,8 @@ def lib():
         L.or_gmres.argtypes = solver_args + [C.c_int, C.c_double, C.c_int, C.POINTER(Stats)]
         L.or_pbicgstab.argtypes = solver_args + [C.c_double, C.c_int, C.POINTER(Stats)]
         L.or_colormsh.argtypes = [I32P, C.c_int, C.c_int, C.c_int, C.c_int, I32P]
+        L.or_tform.argtypes = [I32P, F64P, F64P, I32P, F64P, C.c_void_p, F64P, C.c_int, C.c_int,
+                               C.c_int, C.c_int, C.c_int, C.c_int]
         L.or_colormsh.restype = C.c_int
         _lib = L
     return _lib
@@ -174,6 +176,13 @@ class System:
         if rc:
             raise ValueError("non-positive Jacobian in element %d" % rc)
 
+    def displacements(self, x):
+        """u(ndf, nnode) of update_mec (Predict.f:183-209): x on the free dofs,
+        the prescribed values elsewhere; returned as (nnode, ndf)."""
+        u = self.u.copy()
+        lib().or_update(self.mesh.nnode, self.mesh.ndf, self.idn, u, np.ascontiguousarray(x))
+        return u
+
     def residual(self, x):
         """Second pass of the nonlinear loop (Main.f:789-815): b0 - K u(x)."""
         u = self.u.copy()
@@ -183,6 +192,22 @@ class System:
         self._pform(b, lhs=1, rhs=1, isw=2)
         self.u = saved
         return b
+
+
+def tform(mesh, u, tx0=None):
+    """Nodal stresses of tform_mec (Pform.f:258-421, isw = 3): u (nnode, ndf)
+    -> tx (nnode, ntn), ntn = 4 in 2-D, 6 in 3-D."""
+    ntn = 6 if mesh.ndm == 3 else 4
+    tx = np.zeros((mesh.nnode, ntn))
+    t0 = None if tx0 is None else np.ascontiguousarray(tx0, np.float64)
+    rc = lib().or_tform(np.ascontiguousarray(mesh.ix), np.ascontiguousarray(mesh.x),
+                        np.ascontiguousarray(mesh.e), np.ascontiguousarray(mesh.ie),
+                        np.ascontiguousarray(u, np.float64),
+                        None if t0 is None else t0.ctypes.data_as(C.c_void_p), tx,
+                        mesh.nnode, mesh.numel, mesh.nen, mesh.ndm, mesh.ndf, ntn)
+    if rc:
+        raise ValueError("non-positive Jacobian at a node of element %d" % rc)
+    return tx
 
 
 # ------------------------------------------------------------------- solvers

@@ -928,6 +928,184 @@ int or_pform(const int32_t* ix, const int32_t* iq, const int32_t* ie, const doub
   return 0;
 }
 
+/* ------------------------------------------ nodal stresses (isw = 3) */
+/* deform2d / deform3d (Elmlib.f:2368-2472): strains from nodal derivatives */
+static void deform2d(const double* hx, const double* hy, const double* u, double* eps, int nen) {
+  double ux = 0.0, uy = 0.0, vx = 0.0, vy = 0.0;
+  for (int i = 1; i <= nen; i++) {
+    int j1 = 2 * i - 1, j2 = j1 + 1;
+    ux = ux + hx[i - 1] * u[j1 - 1];
+    uy = uy + hy[i - 1] * u[j1 - 1];
+    vx = vx + hx[i - 1] * u[j2 - 1];
+    vy = vy + hy[i - 1] * u[j2 - 1];
+  }
+  eps[0] = ux;
+  eps[1] = vy;
+  eps[2] = uy + vx;
+}
+
+static void deform3d(const double* hx, const double* hy, const double* hz, const double* u,
+                     double* eps, int nen) {
+  double ux = 0.0, uy = 0.0, uz = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+  for (int i = 1; i <= nen; i++) {
+    int j1 = 3 * i - 2, j2 = j1 + 1, j3 = j2 + 1;
+    ux = ux + hx[i - 1] * u[j1 - 1];
+    uy = uy + hy[i - 1] * u[j1 - 1];
+    uz = uz + hz[i - 1] * u[j1 - 1];
+    vx = vx + hx[i - 1] * u[j2 - 1];
+    vy = vy + hy[i - 1] * u[j2 - 1];
+    vz = vz + hz[i - 1] * u[j2 - 1];
+    wx = wx + hx[i - 1] * u[j3 - 1];
+    wy = wy + hy[i - 1] * u[j3 - 1];
+    wz = wz + hz[i - 1] * u[j3 - 1];
+  }
+  eps[0] = ux;
+  eps[1] = vy;
+  eps[2] = wz;
+  eps[3] = uy + vx;
+  eps[4] = vz + wy;
+  eps[5] = uz + wx;
+}
+
+/* stress2d_m (Elmlib.f:2502-2539): sxx syy szz sxy; szz = ps (sxx + syy) in
+ * plane strain, 0 in plane stress */
+static void stress2d_m(double d11, double d12, double d22, double d33, double ps, int ept,
+                       const double* eps, double* t) {
+  t[0] = d11 * eps[0] + d12 * eps[1];
+  t[1] = d12 * eps[0] + d22 * eps[1];
+  t[2] = ept ? ps * (t[0] + t[1]) : 0.0;
+  t[3] = d33 * eps[2];
+}
+
+/* stress3d (Elmlib.f:2540-2573): sxx syy szz sxy syz sxz */
+static void stress3d(double a, double b, double c, const double* eps, double* t) {
+  t[0] = (eps[0] + b * eps[1] + b * eps[2]) * a;
+  t[1] = (b * eps[0] + eps[1] + b * eps[2]) * a;
+  t[2] = (b * eps[0] + b * eps[1] + eps[2]) * a;
+  t[3] = eps[3] * c * a;
+  t[4] = eps[4] * c * a;
+  t[5] = eps[5] * c * a;
+}
+
+/* the constitutive constants a, b, c of the elements (Elmlib02.f:1693-1702
+ * and the plane-stress form, Elmlib02.f:1033-1048) */
+static void abc(int plane_stress, double ym, double ps, double* a, double* b, double* c) {
+  if (!plane_stress) {
+    double a1 = 1.0 - ps, a2 = 1.0 - 2.0 * ps, a3 = 1.0 + ps;
+    *a = (ym * a1) / (a3 * a2);
+    *b = ps / a1;
+    *c = 0.5 * (a2 / a1);
+  } else {
+    double a1 = 1.0 - ps, a2 = 1.0 - ps * ps;
+    *a = ym / a2;
+    *b = ps;
+    *c = 0.5 * a1;
+  }
+}
+
+/* isw = 3 of the element library: the element's nodal stresses p(ntn, nen).
+ * tria3 (Elmlib02.f:143-190 then 300: one constant stress copied to its 3
+ * nodes), quad4 (Elmlib02.f:716+ and 1018+ label 300: at each node's natural
+ * coordinates), tet4 (Elmlib02.f:1312+ label 300: constant, 4 copies), hex8
+ * (Elmlib02.f:1791-1815: at each node's natural coordinates). */
+int or_element_stress(int iel, const double* e, const double* x, const double* u, double* p,
+                      int ndm) {
+  static const double rq[4] = {1.0, -1.0, -1.0, 1.0}, sq[4] = {1.0, 1.0, -1.0, -1.0};
+  static const double rh[8] = {1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
+  static const double sh[8] = {1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0};
+  static const double th[8] = {1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, -1.0};
+  double a, b, c, eps[6], det;
+  double ym = e[0], ps = e[1];
+  if (iel == 2 || iel == 3) {
+    double xj11 = x[0] - x[2 * ndm], xj12 = x[1] - x[2 * ndm + 1];
+    double xj21 = x[ndm] - x[2 * ndm], xj22 = x[ndm + 1] - x[2 * ndm + 1];
+    det = xj11 * xj22 - xj12 * xj21;
+    if (det <= 0.0) return -1;
+    double ji11 = xj22 / det, ji12 = -xj12 / det, ji21 = -xj21 / det, ji22 = xj11 / det;
+    double hx[3] = {ji11, ji12, -ji11 - ji12}, hy[3] = {ji21, ji22, -ji21 - ji22};
+    abc(iel == 3, ym, ps, &a, &b, &c);
+    deform2d(hx, hy, u, eps, 3);
+    stress2d_m(a, a * b, a, a * c, ps, iel == 2, eps, p);
+    for (int k = 0; k < 4; k++) p[4 + k] = p[8 + k] = p[k];
+    return 0;
+  }
+  if (iel == 4 || iel == 5) {
+    abc(iel == 5, ym, ps, &a, &b, &c);
+    for (int i = 0; i < 4; i++) {
+      double h[4], hx[4], hy[4];
+      sfquad4(h, hx, hy, rq[i], sq[i], 0);
+      if (jacob2d(hx, hy, x, 4, ndm, &det)) return -1;
+      deform2d(hx, hy, u, eps, 4);
+      stress2d_m(a, a * b, a, a * c, ps, iel == 4, eps, p + 4 * i);
+    }
+    return 0;
+  }
+  if (iel == 6) {
+    double hx[4], hy[4], hz[4];
+    abc(0, ym, ps, &a, &b, &c);
+    if (jtetra4(x, hx, hy, hz, &det, 1)) return -1;
+    deform3d(hx, hy, hz, u, eps, 4);
+    stress3d(a, b, c, eps, p);
+    for (int k = 0; k < 6; k++) p[6 + k] = p[12 + k] = p[18 + k] = p[k];
+    return 0;
+  }
+  if (iel == 7) {
+    abc(0, ym, ps, &a, &b, &c);
+    for (int i = 0; i < 8; i++) {
+      double h[8], hx[8], hy[8], hz[8];
+      sfhexa8(h, hx, hy, hz, rh[i], sh[i], th[i], 0);
+      if (jacob3d(hx, hy, hz, x, 8, &det)) return -1;
+      deform3d(hx, hy, hz, u, eps, 8);
+      stress3d(a, b, c, eps, p + 6 * i);
+    }
+    return 0;
+  }
+  return -2;
+}
+
+/* tform_mec (Pform.f:258-421) with isw = 3: element nodal stresses added
+ * to their nodes in element order, divided by the node's element count,
+ * plus the initial stress tx0 (may be NULL).  u(ndf, nnode), tx(ntn,
+ * nnode), ntn = 4 (2-D) or 6 (3-D).  Returns 0, or the 1-based element
+ * whose Jacobian is not positive at a node. */
+int or_tform(const int32_t* ix, const double* x, const double* e, const int32_t* ie,
+             const double* u, const double* tx0, double* tx, int nnode, int numel, int nen,
+             int ndm, int ndf, int ntn) {
+  int32_t* ic = (int32_t*)calloc((size_t)nnode + 1, sizeof(int32_t));
+  for (size_t q = 0; q < (size_t)nnode * ntn; q++) tx[q] = 0.0;
+  double ul[64], pl[64], xl[3 * 8];
+  int rc = 0;
+  for (int nel = 1; nel <= numel && !rc; nel++) {
+    for (int q = 0; q < nen * ntn; q++) pl[q] = 0.0;
+    int kk = 0;
+    for (int i = 1; i <= nen; i++) {
+      int no = A2(ix, nen + 1, i, nel);
+      for (int j = 1; j <= ndf; j++) ul[kk++] = A2(u, ndf, j, no);
+    }
+    for (int i = 1; i <= nen; i++) {
+      int no = A2(ix, nen + 1, i, nel);
+      for (int j = 1; j <= ndm; j++) xl[(i - 1) * ndm + j - 1] = A2(x, ndm, j, no);
+    }
+    int ma = A2(ix, nen + 1, nen + 1, nel);
+    int iel = A1(ie, ma);
+    if (or_element_stress(iel, e + (size_t)(ma - 1) * OR_PROP, xl, ul, pl, ndm)) {
+      rc = nel;
+      break;
+    }
+    for (int i = 1; i <= nen; i++) {
+      int no = A2(ix, nen + 1, i, nel);
+      if (no <= 0) continue;
+      ic[no] += 1;
+      for (int j = 1; j <= ntn; j++) A2(tx, ntn, j, no) = A2(tx, ntn, j, no) + pl[(i - 1) * ntn + j - 1];
+    }
+  }
+  for (int i = 1; i <= nnode && !rc; i++)
+    for (int j = 1; j <= ntn; j++)
+      A2(tx, ntn, j, i) = A2(tx, ntn, j, i) / (double)ic[i] + (tx0 ? A2(tx0, ntn, j, i) : 0.0);
+  free(ic);
+  return rc;
+}
+
 void or_pload(const int32_t* id, double* f, double* u, double* b, const int32_t* nload,
               int nnode, int ndf, const or_loads* ld) {
   for (int i = 1; i <= nnode; i++)

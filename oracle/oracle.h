@@ -68,6 +68,14 @@ int or_pform(const int32_t* ix, const int32_t* iq, const int32_t* ie, const doub
              const or_loads* ld, const double* gravity);
 
 /* Pload.f:2-57 (time-constant loads, t = 0). */
+/* isw = 3 of the element library (Elmlib02.f label 300 of elmt02..07):
+ * nodal stresses p(ntn, nen) of one element */
+int or_element_stress(int iel, const double* e, const double* x, const double* u, double* p,
+                      int ndm);
+/* tform_mec (Pform.f:258-421), isw = 3: averaged nodal stresses tx(ntn, nnode) */
+int or_tform(const int32_t* ix, const double* x, const double* e, const int32_t* ie,
+             const double* u, const double* tx0, double* tx, int nnode, int numel, int nen,
+             int ndm, int ndf, int ntn);
 void or_pload(const int32_t* id, double* f, double* u, double* b, const int32_t* nload,
               int nnode, int ndf, const or_loads* ld);
 
