@@ -236,10 +236,33 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
   const double di = grid_sum(part_rz, sh);
   const double beta = part_zq ? -grid_sum(part_zq, sh) / sc->pq : di / *dold;
   double apr = 0.0;
-  GRID_LOOP(i, n) {
-    const double pi = apply_m(i, n, r, m, bm) + beta * p[i];
-    p[i] = pi;
-    apr += pi * r[i];
+  if (bm == nullptr) {
+    // Jacobi: four rows per lane as two 16-byte loads of r, m and p each, so
+    // that enough bytes are in flight at 4 waves per SIMD; vectors are
+    // 256-byte aligned (hipMalloc), rows 4k..4k+3 are 32-byte aligned
+    const int n4 = n >> 2;
+    for (int k = blockIdx.x * kThreads + threadIdx.x; k < n4; k += gridDim.x * kThreads) {
+      const double2* r2 = reinterpret_cast<const double2*>(r) + 2 * k;
+      const double2* m2 = reinterpret_cast<const double2*>(m) + 2 * k;
+      double2* p2 = reinterpret_cast<double2*>(p) + 2 * k;
+      const double2 ra = r2[0], rb = r2[1], ma = m2[0], mb = m2[1], pa = p2[0], pb = p2[1];
+      const double2 qa = {ra.x * ma.x + beta * pa.x, ra.y * ma.y + beta * pa.y};
+      const double2 qb = {rb.x * mb.x + beta * pb.x, rb.y * mb.y + beta * pb.y};
+      p2[0] = qa;
+      p2[1] = qb;
+      apr += qa.x * ra.x + qa.y * ra.y + qb.x * rb.x + qb.y * rb.y;
+    }
+    for (int i = 4 * n4 + blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+      const double pi = r[i] * m[i] + beta * p[i];
+      p[i] = pi;
+      apr += pi * r[i];
+    }
+  } else {
+    GRID_LOOP(i, n) {
+      const double pi = apply_m(i, n, r, m, bm) + beta * p[i];
+      p[i] = pi;
+      apr += pi * r[i];
+    }
   }
   if (part_pr) {  // p.r of the next step (flexible recurrences, tile dots)
     const double t = block_sum(apr, sh);
