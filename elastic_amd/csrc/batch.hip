@@ -1,14 +1,16 @@
 // Multi-instance PCG for serving: many independent systems (each held by its
 // own context: structure, values, b and vectors resident in HBM) iterate
 // together.  Every phase of a PCG iteration is ONE launch over all active
-// instances with a fixed grid that walks a device work list:
+// instances, one workgroup per item of a device work list:
 //   SpMV          work item = (instance, tile)          block_tile of spmv_block.h
 //                 (q = A p and each tile's share of p.q)
 //   shares sum    one workgroup per instance: p.q in tile order
 //   vector phases work item = (instance, chunk of whole nodes, <= 2048 rows)
-// so a chunk of iterations is captured once as a HIP graph and replayed while
-// instances are admitted and retired: only the work lists and the instance
-// table (device memory) change between replays.
+// The grid of a list is a power-of-two size class of its length (workgroups
+// past the length return at once), so a chunk of iterations is captured once
+// as a HIP graph and replayed while instances are admitted and retired: only
+// the work lists and the instance table (device memory) change between
+// replays, and a new capture is needed only when a size class changes.
 //
 // The arithmetic of an instance never depends on the other instances: its
 // tiles, its chunks and the order of its reductions (per-chunk partials summed
@@ -29,7 +31,6 @@
 namespace elastic {
 
 constexpr int kChunk = 2048;         // rows per vector work item at most (256 lanes x 8)
-constexpr int kBatchGrid = 2048;     // fixed grid of every batched phase
 constexpr int kMaxSlots = 255;       // slot index lives in the top 8 bits of a work item
 constexpr int kBatchPer = 2;         // node blocks per lane (tiles of <= 512 blocks)
 
@@ -66,8 +67,11 @@ __device__ inline double inst_sum(const double* __restrict__ part, int nch, doub
 }
 
 // The product of one operator variant (value type TV, sum type TA, node block
-// NB) over its work list, so each variant compiles with the registers of the
-// single-system kernel.  mode 0: q = A p and the tiles' shares of p.q; the
+// NB) over its work list, one work item per workgroup (a loop over items would
+// keep per-item values live across iterations: 120 instead of 80 VGPRs for
+// the FP64 3-D variant), so each variant compiles with the registers of the
+// single-system kernel; the grid is a size class of the list (workgroups
+// beyond the list's count return at once).  mode 0: q = A p and the tiles' shares of p.q; the
 // FP64 list holds every instance
 // and serves FP64 instances and mixed ones the progress guard handed over to
 // FP64, the FP32 lists hold the mixed instances that are still on FP32.
@@ -80,12 +84,12 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int flag;
   constexpr bool f64 = sizeof(TV) == 8;
-  const int nw = *count;
-  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+  const int w = blockIdx.x;
+  if (w >= *count) return;
+  {
     const uint32_t e = work[w];
     const BInst& I = insts[wl_slot(e)];
     const int t = wl_index(e);
-    __syncthreads();  // LDS of the previous tile is free, flag is free
     if (threadIdx.x == 0) {
       if (!I.live || t >= I.ntiles) {
         flag = 0;  // not a work item of an admitted instance
@@ -97,7 +101,7 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
       }
     }
     __syncthreads();
-    if (!flag) continue;
+    if (!flag) return;
     const TV* vals = f64 ? (const TV*)I.v64 : (const TV*)I.v32;
     const TV* diag = f64 ? (const TV*)I.d64 : (const TV*)I.d32;
     if (MODE == 0)
@@ -220,12 +224,13 @@ __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict_
                                                         const int32_t* __restrict__ count,
                                                         int cur) {
   __shared__ double sh[8];
-  const int nw = *count;
-  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+  const int w = blockIdx.x;  // one work item per workgroup
+  if (w >= *count) return;
+  {
     const uint32_t e = work[w];
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
-    if (uniform_done(I, c, sh)) continue;
+    if (uniform_done(I, c, sh)) return;
     const double pq = I.sc->pq_tiles;
     const double num = I.fp32 ? inst_sum(I.part + 3 * I.nch, I.nch, sh) : I.dslot[cur];
     const double alpha = num / pq;
@@ -252,12 +257,13 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
                                                          const int32_t* __restrict__ count,
                                                          int cur) {
   __shared__ double sh[8];
-  const int nw = *count;
-  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+  const int w = blockIdx.x;  // one work item per workgroup
+  if (w >= *count) return;
+  {
     const uint32_t e = work[w];
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
-    if (uniform_done(I, c, sh)) continue;
+    if (uniform_done(I, c, sh)) return;
     const double di = inst_sum(I.part + I.nch, I.nch, sh);
     const double beta =
         I.fp32 ? -inst_sum(I.part + 4 * I.nch, I.nch, sh) / I.sc->pq : di / I.dslot[cur];
@@ -301,12 +307,13 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_resid(const BInst* __restri
                                                            const uint32_t* __restrict__ work,
                                                            const int32_t* __restrict__ count) {
   __shared__ double sh[8];
-  const int nw = *count;
-  for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+  const int w = blockIdx.x;  // one work item per workgroup
+  if (w >= *count) return;
+  {
     const uint32_t e = work[w];
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
-    if (uniform_done(I, c, sh, true)) continue;
+    if (uniform_done(I, c, sh, true)) return;
     const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
     double acc = 0.0, apr = 0.0;
     for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
@@ -455,13 +462,12 @@ struct elastic_batch {
   std::vector<uint32_t> h_sw[kVariants], h_vwork;
   std::vector<int32_t> h_slist, h_counts;
   unsigned mask = 0, graph_mask = ~0u;  // non-empty variant lists (+ bit 8: mixed present)
-  // grid of each product variant: the workgroups the device keeps resident
-  // at the current LDS size (a workgroup walks its share of the tiles, so a
-  // larger grid would leave late starters running after the rest)
-  int vgrid[2][kVariants] = {{0}};
-  size_t vgrid_lds = ~(size_t)0;  // LDS size the grids were computed for
+  // grid of each product variant: a power-of-two size class of its list
+  // (one workgroup per item); a captured graph holds the classes it was
+  // captured with
+  int sgrid[kVariants] = {0}, graph_sgrid[kVariants] = {0};
   int ncu = 0;
-  int ggrid[3] = {0, 0, 0};  // update, pupdate, rel_resid
+  int vgrid = 0, graph_vgrid = 0;  // size class of the vector list (one chunk per workgroup)
   BStatus *d_status = nullptr, *h_status = nullptr;  // h_status pinned
   bool dirty = true, stepped = false;
   size_t lds = 0, graph_lds = 0;
@@ -493,47 +499,24 @@ static void enqueue_iteration(elastic_batch* b, int it) {
   const int32_t* vcount = b->d_counts + kVariants;
   for (int v = 0; v < kVariants; ++v)
     if (b->mask & (1u << v))
-      launch_variant(v, dim3(b->vgrid[0][v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
+      launch_variant(v, dim3(b->sgrid[v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
                      b->d_counts + v, 0);
   hipLaunchKernelGGL(k_bt_sum_shares, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist,
                      vcount + 1);
-  hipLaunchKernelGGL(k_bt_update, dim3(b->ggrid[0]), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
-  hipLaunchKernelGGL(k_bt_pupdate, dim3(b->ggrid[1]), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
+  hipLaunchKernelGGL(k_bt_update, dim3(b->vgrid), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
+  hipLaunchKernelGGL(k_bt_pupdate, dim3(b->vgrid), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
   if (cur == 1 && (b->mask & 0x100u)) {  // every second iteration: true residuals where asked
     for (int v = 0; v < kVariants; v += 3)  // the FP64 lists
       if (b->mask & (1u << v))
-        launch_variant(v, dim3(b->vgrid[1][v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
+        launch_variant(v, dim3(b->sgrid[v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
                        b->d_counts + v, 1);
-    hipLaunchKernelGGL(k_bt_rel_resid, dim3(b->ggrid[2]), TB, 0, s, b->d_insts, b->d_vwork, vcount);
+    hipLaunchKernelGGL(k_bt_rel_resid, dim3(b->vgrid), TB, 0, s, b->d_insts, b->d_vwork, vcount);
     hipLaunchKernelGGL(k_bt_rel_finish, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist,
                        vcount + 1, 0);
   }
 }
 
 static int set_lds_limit(elastic_batch* b) {
-  if (b->vgrid_lds != b->lds) {
-    for (int mode = 0; mode < 2; ++mode)
-      for (int v = 0; v < kVariants; ++v) {
-        const void* k = mode == 0 ? variant_kernel<0>(v) : variant_kernel<1>(v);
-        int per_cu = 0;
-        if (b->lds > 64 * 1024)
-          (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kThreads, b->lds) != hipSuccess ||
-            per_cu < 1)
-          per_cu = 1;
-        b->vgrid[mode][v] = std::max(1, std::min(per_cu * b->ncu, kBatchGrid * 4));
-      }
-    const void* vk[3] = {(const void*)k_bt_update, (const void*)k_bt_pupdate,
-                         (const void*)k_bt_rel_resid};
-    for (int q = 0; q < 3; ++q) {
-      int per_cu = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, vk[q], kThreads, 0) != hipSuccess ||
-          per_cu < 1)
-        per_cu = 1;
-      b->ggrid[q] = std::max(1, std::min(per_cu * b->ncu, kBatchGrid * 4));
-    }
-    b->vgrid_lds = b->lds;
-  }
   if (b->lds > 64 * 1024)
     for (int v = 0; v < kVariants; ++v) {
       BATCH_HIP(b, hipFuncSetAttribute(variant_kernel<0>(v),
@@ -569,6 +552,15 @@ static int upload_lists(elastic_batch* b) {
   unsigned mask = mixed ? 0x100u : 0u;
   for (int v = 0; v < kVariants; ++v)
     if (!b->h_sw[v].empty()) mask |= 1u << v;
+  // grid size classes: the next power of two of the count, kept while the
+  // count stays above half of it (fewer re-captures as instances leave)
+  auto size_class = [](int n, int cur) {
+    int g = 64;
+    while (g < n) g <<= 1;
+    return (cur >= n && 2 * n > cur) ? cur : (n ? g : 0);
+  };
+  for (int v = 0; v < kVariants; ++v) b->sgrid[v] = size_class((int)b->h_sw[v].size(), b->sgrid[v]);
+  b->vgrid = size_class((int)b->h_vwork.size(), b->vgrid);
   auto grow = [&](uint32_t*& p, size_t& cap, size_t need) -> int {
     if (need <= cap) return ELASTIC_OK;
     size_t nc = std::max(need, 2 * cap + 4096);
@@ -624,6 +616,8 @@ static int capture(elastic_batch* b) {
   BATCH_HIP(b, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
   b->graph_lds = b->lds;
   b->graph_mask = b->mask;
+  for (int v = 0; v < kVariants; ++v) b->graph_sgrid[v] = b->sgrid[v];
+  b->graph_vgrid = b->vgrid;
   return ELASTIC_OK;
 }
 
@@ -775,7 +769,9 @@ int elastic_batch_step(elastic_batch* b) {
   }
   if (b->h_slist.empty()) return ELASTIC_OK;
   if (b->use_graph) {
-    if (!b->gexec || b->graph_lds < b->lds || b->graph_mask != b->mask) {
+    bool grids = b->vgrid != b->graph_vgrid;
+    for (int v = 0; v < kVariants; ++v) grids |= (b->mask & (1u << v)) && b->sgrid[v] != b->graph_sgrid[v];
+    if (!b->gexec || b->graph_lds < b->lds || b->graph_mask != b->mask || grids) {
       int rc = capture(b);
       if (rc) return rc;
     }
