@@ -153,7 +153,7 @@ def main():
 
     phase("cpu baseline done")
     serving = None
-    if args.serving > 0:
+    if args.serving > 0 or args.serving_compare > 0:
         ctx.close()
         del sysm, ctx
         serving = serving_bench(args.serving, args.serving_compare)
@@ -221,6 +221,8 @@ def serving_bench(nreq: int, ncompare: int):
     out = {"workload": ("%d plane-strain requests (even quad4, odd tria3, side U{32..512}), "
                         "Poisson arrivals 200/s, to %g" % (nreq, TOL))}
     for name, prec in (("stream_batched_fp64", FP64), ("stream_batched_mixed", MIXED)):
+        if nreq <= 0:
+            break
         srv = Server(max_slots=64, iters_per_step=16, prep_workers=workers, precision=prec,
                      tol=TOL, hbm_budget=64e9, progress=sys.stderr)
         a = time.perf_counter()
@@ -231,6 +233,8 @@ def serving_bench(nreq: int, ncompare: int):
         out[name]["fp64_operator_handovers"] = int(sum(r.stats.get("op64_from", -1) >= 0
                                                        for r in res["done"]))
 
+    if ncompare <= 0:
+        return out
     print("solve-only comparison: preparing %d systems" % ncompare, file=sys.stderr, flush=True)
     systems = []
     for k in range(ncompare):

@@ -634,6 +634,8 @@ int elastic_batch_create(elastic_batch** out, int32_t device, int32_t max_slots,
   b->iters = iters_per_step & ~1;  // even: steps start at even parity
   b->use_graph = use_graph != 0;
   if (const char* e = getenv("ELASTIC_RELIABLE_DELTA")) b->delta = atof(e);
+  // ELASTIC_BATCH_GRAPH=0: steps are eager launches (for profilers)
+  if (const char* e = getenv("ELASTIC_BATCH_GRAPH")) b->use_graph = b->use_graph && atoi(e) != 0;
   b->slots.resize(max_slots);
   b->h_insts.resize(max_slots);
   if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
