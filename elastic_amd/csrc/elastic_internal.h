@@ -274,7 +274,9 @@ struct DotOut {
   double* result;
 };
 void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epart,
-                 const int32_t* skip = nullptr, const DotOut* dot = nullptr);
+                 const int32_t* skip = nullptr, const DotOut* dot = nullptr, bool y32 = false);
+// (y32: a product with FP32 sums on the node-blocked layout stores its own
+// rows of y as FP32 too, exactly; the caller then reads y as float)
 // gathers the CSRC values into the node-blocked planes (after every assembly)
 void launch_block_pack(Context* c);
 template <typename T>

@@ -138,8 +138,10 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int n, const double* __
 // k_pcg_update on the node-blocked layout (n = NB * nn), one lane per node:
 // the node's merge list is walked once for its NB rows (contiguous partials),
 // and the vector loads are issued before that gather.  p.q comes from the
-// product's tiles (sc->pq_tiles).  EP: type of the stored partials.
-template <int NB, typename EP>
+// product's tiles (sc->pq_tiles).  EP / QT: types of the stored partials and
+// of the product's own rows (FP32 after a product with FP32 sums that was
+// asked for FP32 rows; q is then not written back).
+template <int NB, typename EP, typename QT>
 __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs mg,
                                                                 const double* __restrict__ part_pr,
                                                                 const double* __restrict__ dslot,
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
     double qv[NB], xv[NB], pv[NB], rv[NB], mv[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
-      qv[c] = q[i0 + c];
+      qv[c] = (double)reinterpret_cast<const QT*>(q)[i0 + c];
       xv[c] = x[i0 + c];
       pv[c] = p[i0 + c];
       rv[c] = r[i0 + c];
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
     }
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
-      if (keep_q) q[i0 + c] = qv[c];
+      if (sizeof(QT) == 8 && keep_q) q[i0 + c] = qv[c];
       x[i0 + c] = xv[c] + alpha * pv[c];
       const double ri = rv[c] - alpha * qv[c];
       r[i0 + c] = ri;
@@ -397,22 +399,26 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
       const bool timed = stride > 0 && it % stride == 0;
       if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j], s));
       if (tdots) {
-        launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done, &dot);
-        if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j + 1], s));
-        // q is read again only by the block preconditioner's (M^-1 r).q
         const int keep_q = (!jac && reliable) ? 1 : 0;
         const bool e32 = epart_f32(c, op32);
+        // q in FP32 when the product sums in FP32 and nothing reads q afterwards
+        const bool q32 = e32 && !keep_q;
+        launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done, &dot, q32);
+        if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j + 1], s));
+        // q is read again only by the block preconditioner's (M^-1 r).q
         const MergeArgs mg = merge_of(c, V.epart, e32);
-#define ELASTIC_UPD(NB, EP)                                                                    \
-  hipLaunchKernelGGL((k_pcg_update_nodes<NB, EP>), G, TB, 0, s, n / NB, mg,                    \
+#define ELASTIC_UPD(NB, EP, QT)                                                                \
+  hipLaunchKernelGGL((k_pcg_update_nodes<NB, EP, QT>), G, TB, 0, s, n / NB, mg,                \
                      (const double*)part_pr, dcur, V.x, V.p, V.r, V.q, keep_q, V.m, jac, part_b, \
                      part_zq, sc)
         if (c->layout_nb == 3) {
-          if (e32) ELASTIC_UPD(3, float);
-          else ELASTIC_UPD(3, double);
+          if (q32) ELASTIC_UPD(3, float, float);
+          else if (e32) ELASTIC_UPD(3, float, double);
+          else ELASTIC_UPD(3, double, double);
         } else {
-          if (e32) ELASTIC_UPD(2, float);
-          else ELASTIC_UPD(2, double);
+          if (q32) ELASTIC_UPD(2, float, float);
+          else if (e32) ELASTIC_UPD(2, float, double);
+          else ELASTIC_UPD(2, double, double);
         }
 #undef ELASTIC_UPD
       } else {

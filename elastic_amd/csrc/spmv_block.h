@@ -137,7 +137,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
                                              const double* __restrict__ x, double* __restrict__ y,
                                              double* __restrict__ epart, unsigned char* smem,
                                              double* __restrict__ tdot = nullptr,
-                                             Hook hook = Hook()) {
+                                             int y32 = 0, Hook hook = Hook()) {
   constexpr int NB2 = NB * NB;
   T* xs = reinterpret_cast<T*>(smem);       // ncols_max * NB
   T* pp = xs + a.ncols_max * NB;            // 2 * kBlkRowsMax * NB (row pieces)
@@ -262,8 +262,14 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
 #pragma unroll
       for (int r = 0; r < NB; ++r) s[r] += tr[q * NB + r];
     if (c < nr) {
+      if (sizeof(T) == 4 && y32) {  // FP32 sums, y asked in FP32 (exact)
+        float* yf = reinterpret_cast<float*>(y);
 #pragma unroll
-      for (int r = 0; r < NB; ++r) y[(int64_t)NB * (I0 + c) + r] = (double)s[r];
+        for (int r = 0; r < NB; ++r) yf[(int64_t)NB * (I0 + c) + r] = (float)s[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < NB; ++r) y[(int64_t)NB * (I0 + c) + r] = (double)s[r];
+      }
     } else if constexpr (sizeof(T) == 4) {  // FP32 sums: FP32 partials (exact)
       float* ef = reinterpret_cast<float*>(epart);
 #pragma unroll
@@ -286,10 +292,10 @@ __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __
                                            const TV* __restrict__ diag,
                                            const double* __restrict__ x, double* __restrict__ y,
                                            double* __restrict__ epart, unsigned char* smem,
-                                           double* __restrict__ tdot = nullptr) {
+                                           double* __restrict__ tdot = nullptr, int y32 = 0) {
   TileRegs<TV, NB, PER> R;
   load_tile<TV, NB, PER>(a, t, vals, diag, R);
-  compute_tile<TV, T, NB, PER, DOT>(a, t, R, x, y, epart, smem, tdot);
+  compute_tile<TV, T, NB, PER, DOT>(a, t, R, x, y, epart, smem, tdot, y32);
 }
 
 }  // namespace elastic
