@@ -139,7 +139,9 @@ def main():
     # (x not freshly written by the previous kernel) is reported beside it
     tot32, ker32 = ctx.time_spmv(MIXED, reps=50)
     tot64, ker64 = ctx.time_spmv(FP64, reps=50)
-    b32, b64 = spmv_bytes(neq, nad, 4), spmv_bytes(neq, nad, 8)
+    # with FP32 sums the operator stages x from an FP32 copy and writes its
+    # rows in FP32: x and y count 4 bytes per entry then, 8 otherwise
+    b32, b64 = spmv_bytes(neq, nad, 4, 4 if acc32 else 8), spmv_bytes(neq, nad, 8)
     gbs32 = b32 / (live32_ms * 1e-3) / 1e9
     gbs64 = b64 / (live64_ms * 1e-3) / 1e9
     t_asm = sysm.time_assembly(reps=3)

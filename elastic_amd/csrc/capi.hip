@@ -63,12 +63,13 @@ static void free_layout(Context* c) {
   free_tiles(c->tiles);
   free_blocks(c->blk);
   c->inc_ptr = c->inc_idx = nullptr;
-  void* ptrs[] = {c->A32.ad, c->A32.al, c->epart64, c->tdot};
+  void* ptrs[] = {c->A32.ad, c->A32.al, c->epart64, c->tdot, c->p32};
   for (void* p : ptrs)
     if (p) hipFree(p);
   c->A32 = DevMatrix<float>();
   c->epart64 = nullptr;
   c->tdot = nullptr;
+  c->p32 = nullptr;
 }
 
 static void free_system(Context* c) {
@@ -170,6 +171,7 @@ static int build_layout(elastic_ctx* x) {
     c->inc_idx = b.inc_idx;  // node-granular lists
     nepart = (int64_t)nb * L.next_nodes;
     rc = rc ? rc : dalloc(c, &c->tdot, (size_t)L.ntiles);
+    rc = rc ? rc : dalloc(c, &c->p32, (size_t)neq);
   }
   if (nb == 1) {  // FP32 operator of the scalar tiles (the blocked layout has its planes)
     rc = rc ? rc : dalloc(c, &c->A32.ad, neq);

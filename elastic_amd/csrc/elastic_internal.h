@@ -159,6 +159,7 @@ struct Context {
   // workspace
   double *epart64 = nullptr, *red64 = nullptr;
   double* tdot = nullptr;  // per-tile shares of p.(A p) (node-blocked layouts)
+  float* p32 = nullptr;    // FP32 copy of p for the FP32-sum operator (node-blocked)
   double *v64[8] = {nullptr};
   SolveScalars* scal = nullptr;
   SolveScalars* scal_host = nullptr;  // pinned
@@ -274,9 +275,11 @@ struct DotOut {
   double* result;
 };
 void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epart,
-                 const int32_t* skip = nullptr, const DotOut* dot = nullptr, bool y32 = false);
+                 const int32_t* skip = nullptr, const DotOut* dot = nullptr, bool y32 = false,
+                 const float* x32 = nullptr);
 // (y32: a product with FP32 sums on the node-blocked layout stores its own
-// rows of y as FP32 too, exactly; the caller then reads y as float)
+// rows of y as FP32 too, exactly; the caller then reads y as float.  x32: an
+// FP32 copy of x that such a product stages from instead of x)
 // gathers the CSRC values into the node-blocked planes (after every assembly)
 void launch_block_pack(Context* c);
 template <typename T>

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const double* __re
                                                         const double* __restrict__ m,
                                                         const double* __restrict__ bm, double* x,
                                                         double* r, double* p, double* part,
-                                                        double* part_pr) {
+                                                        double* part_pr, float* p32) {
   __shared__ double sh[8];
   double acc = 0.0;
   GRID_LOOP(i, n) {
@@ -41,6 +41,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const double* __re
     x[i] = 0.0;
     r[i] = b[i];
     p[i] = zi;
+    if (p32) p32[i] = (float)zi;
     acc += b[i] * zi;
   }
   const double s = block_sum(acc, sh);
@@ -232,7 +233,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
                                                            const double* __restrict__ r,
                                                            const double* __restrict__ m,
                                                            const double* __restrict__ bm,
-                                                           double* part_pr, SolveScalars* sc) {
+                                                           double* part_pr, float* p32,
+                                                           SolveScalars* sc) {
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
   const double di = grid_sum(part_rz, sh);
@@ -252,17 +254,22 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
       const double2 qb = {rb.x * mb.x + beta * pb.x, rb.y * mb.y + beta * pb.y};
       p2[0] = qa;
       p2[1] = qb;
+      if (p32)  // 16-byte aligned: 4 floats from a 256-byte aligned base
+        reinterpret_cast<float4*>(p32)[k] = make_float4((float)qa.x, (float)qa.y, (float)qb.x,
+                                                        (float)qb.y);
       apr += qa.x * ra.x + qa.y * ra.y + qb.x * rb.x + qb.y * rb.y;
     }
     for (int i = 4 * n4 + blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
       const double pi = r[i] * m[i] + beta * p[i];
       p[i] = pi;
+      if (p32) p32[i] = (float)pi;
       apr += pi * r[i];
     }
   } else {
     GRID_LOOP(i, n) {
       const double pi = apply_m(i, n, r, m, bm) + beta * p[i];
       p[i] = pi;
+      if (p32) p32[i] = (float)pi;
       apr += pi * r[i];
     }
   }
@@ -363,7 +370,10 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   const DotOut dot{c->tdot, &sc->pq_tiles};
   // p.r after the p update (and after a replacement), read by the next update
   double* pr_next = tdots ? part_pr : nullptr;
-  hipLaunchKernelGGL(k_pcg_init, G, TB, 0, s, n, V.b, V.m, V.bm, V.x, V.r, V.p, part_a, pr_next);
+  // FP32 copy of p for the FP32-sum operator's staging (node-blocked layouts)
+  float* p32 = (tdots && fp32_operator && c->accum32) ? c->p32 : nullptr;
+  hipLaunchKernelGGL(k_pcg_init, G, TB, 0, s, n, V.b, V.m, V.bm, V.x, V.r, V.p, part_a, pr_next,
+                     p32);
   hipLaunchKernelGGL(k_pcg_init2, dim3(1), TB, 0, s, part_a, tol, delta, reliable ? 1 : 0, dslots,
                      sc);
   auto replace = [&](int iters_so_far) {
@@ -403,7 +413,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
         const bool e32 = epart_f32(c, op32);
         // q in FP32 when the product sums in FP32 and nothing reads q afterwards
         const bool q32 = e32 && !keep_q;
-        launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done, &dot, q32);
+        launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done, &dot, q32, e32 ? p32 : nullptr);
         if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j + 1], s));
         // q is read again only by the block preconditioner's (M^-1 r).q
         const MergeArgs mg = merge_of(c, V.epart, e32);
@@ -434,7 +444,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
         hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)V.q,
                            part_zq, (const int32_t*)nullptr, sc);
       hipLaunchKernelGGL(k_pcg_pupdate, G, TB, 0, s, n, part_b, (const double*)part_zq, dcur, dnxt,
-                         V.p, V.r, V.m, V.bm, pr_next, sc);
+                         V.p, V.r, V.m, V.bm, pr_next, p32, sc);
       // the replacement kernels gate themselves on the flag set by k_pcg_pupdate
       if (reliable && ((it + 1) & 1) == 0) replace(it + 1);
     }

@@ -137,7 +137,8 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
                                              const double* __restrict__ x, double* __restrict__ y,
                                              double* __restrict__ epart, unsigned char* smem,
                                              double* __restrict__ tdot = nullptr,
-                                             int y32 = 0, Hook hook = Hook()) {
+                                             int y32 = 0, const float* __restrict__ x32 = nullptr,
+                                             Hook hook = Hook()) {
   constexpr int NB2 = NB * NB;
   T* xs = reinterpret_cast<T*>(smem);       // ncols_max * NB
   T* pp = xs + a.ncols_max * NB;            // 2 * kBlkRowsMax * NB (row pieces)
@@ -160,7 +161,8 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
     if (c < nx) {
       const int node = c / NB, comp = c - node * NB;
       const int g = node < nr ? I0 + node : ext[node - nr];
-      xv[k] = x[(int64_t)NB * g + comp];
+      // x32: an FP32 copy of x (FP32 staging takes the same values from it)
+      xv[k] = x32 ? (double)x32[(int64_t)NB * g + comp] : x[(int64_t)NB * g + comp];
     }
   }
   hook();
@@ -173,7 +175,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   for (int c = tid + 256 * XR; c < nx; c += 256) {
     const int node = c / NB, comp = c - node * NB;
     const int g = node < nr ? I0 + node : ext[node - nr];
-    xs[c] = (T)x[(int64_t)NB * g + comp];
+    xs[c] = x32 ? (T)x32[(int64_t)NB * g + comp] : (T)x[(int64_t)NB * g + comp];
   }
   __syncthreads();
   const uint16_t* rp = a.rowptr + a.rp0[t];
@@ -292,10 +294,11 @@ __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __
                                            const TV* __restrict__ diag,
                                            const double* __restrict__ x, double* __restrict__ y,
                                            double* __restrict__ epart, unsigned char* smem,
-                                           double* __restrict__ tdot = nullptr, int y32 = 0) {
+                                           double* __restrict__ tdot = nullptr, int y32 = 0,
+                                           const float* __restrict__ x32 = nullptr) {
   TileRegs<TV, NB, PER> R;
   load_tile<TV, NB, PER>(a, t, vals, diag, R);
-  compute_tile<TV, T, NB, PER, DOT>(a, t, R, x, y, epart, smem, tdot, y32);
+  compute_tile<TV, T, NB, PER, DOT>(a, t, R, x, y, epart, smem, tdot, y32, x32);
 }
 
 }  // namespace elastic

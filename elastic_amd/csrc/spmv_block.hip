@@ -13,10 +13,11 @@ __global__ __launch_bounds__(256) void k_spmv_blocks(BlkArgs a, const TV* __rest
                                                       double* __restrict__ y,
                                                       double* __restrict__ epart,
                                                       const int32_t* __restrict__ skip,
-                                                      double* __restrict__ tdot, int y32) {
+                                                      double* __restrict__ tdot, int y32,
+                                                      const float* __restrict__ x32) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (skip != nullptr && *skip != 0) return;
-  block_tile<TV, T, NB, PER, DOT>(a, blockIdx.x, vals, diag, x, y, epart, smem, tdot, y32);
+  block_tile<TV, T, NB, PER, DOT>(a, blockIdx.x, vals, diag, x, y, epart, smem, tdot, y32, x32);
 }
 
 // *out = the n shares added in a fixed order (one workgroup of 1024 lanes):
@@ -98,7 +99,8 @@ __global__ __launch_bounds__(256) void k_pack_diag(int nn, const int32_t* __rest
 template <typename TV, typename TA, int NB>
 static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* diag,
                                const double* x, double* y, double* epart, hipStream_t s,
-                               const int32_t* skip, const DotOut* dot, int y32) {
+                               const int32_t* skip, const DotOut* dot, int y32,
+                               const float* x32) {
   const size_t lds = sizeof(TA) * NB * ((size_t)b.ncols_max + 2 * kBlkRowsMax + (size_t)b.nblk_max);
   const int per = (b.nblk_max + 255) / 256;
 #define ELASTIC_BLK_LAUNCH(K, P, D, G, ...)                                                     \
@@ -114,9 +116,9 @@ static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* dia
   double* td = dot ? dot->part : nullptr;
 #define ELASTIC_BLK_PER(D)                                                                               \
   do {                                                                                                   \
-    if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 1, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32); \
-    else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32); \
-    else ELASTIC_BLK_LAUNCH(k_spmv_blocks, 4, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32); \
+    if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 1, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
+    else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
+    else ELASTIC_BLK_LAUNCH(k_spmv_blocks, 4, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
   } while (0)
   if (dot) {
     ELASTIC_BLK_PER(true);
@@ -131,17 +133,17 @@ static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* dia
 }
 
 void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epart,
-                 const int32_t* skip, const DotOut* dot, bool y32) {
+                 const int32_t* skip, const DotOut* dot, bool y32, const float* x32) {
   hipStream_t s = c->stream;
   const DevBlocks& b = c->blk;
   if (c->layout_nb == 3) {
-    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0);
-    else if (fp32) spmv_blocks_launch<float, double, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0);
-    else spmv_blocks_launch<double, double, 3>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0);
+    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0, x32);
+    else if (fp32) spmv_blocks_launch<float, double, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0, nullptr);
+    else spmv_blocks_launch<double, double, 3>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr);
   } else if (c->layout_nb == 2) {
-    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0);
-    else if (fp32) spmv_blocks_launch<float, double, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0);
-    else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0);
+    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0, x32);
+    else if (fp32) spmv_blocks_launch<float, double, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0, nullptr);
+    else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr);
   } else {
     if (fp32)
       launch_spmv_tiles<float, double>(c->tiles, c->A32.ad, c->A32.al, x, y, epart, s, skip);
