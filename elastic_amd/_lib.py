@@ -95,6 +95,7 @@ def lib():
     L.elastic_j2_increment.argtypes = [vp, C.c_double]
     L.elastic_j2_assemble.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
     L.elastic_j2_correct.argtypes = [vp]
+    L.elastic_j2_step.argtypes = [vp, C.c_double]
     L.elastic_j2_commit.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.elastic_j2_get_u.argtypes = [vp, F64P]
     L.elastic_j2_get_residual.argtypes = [vp, F64P]

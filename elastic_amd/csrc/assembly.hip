@@ -1236,10 +1236,12 @@ __global__ void k_j2_prescribe(int n, const int32_t* __restrict__ id, const doub
 }
 
 // u(dof) += x(id(dof)) on free dofs
+// u += s du on the free dofs (s = 1: the Newton correction; other s: a
+// line-search move along the same du)
 __global__ void k_j2_correct(int n, const int32_t* __restrict__ id, const double* __restrict__ x,
-                             double* u) {
+                             double s, double* u) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    if (id[i] > 0) u[i] = u[i] + x[id[i] - 1];
+    if (id[i] > 0) u[i] = u[i] + (s == 1.0 ? x[id[i] - 1] : s * x[id[i] - 1]);
 }
 
 // per-block partials: sum of b^2 (mode 0); max alpha and count of alpha > 0 (mode 1)
@@ -1398,8 +1400,8 @@ extern "C" int elastic_j2_assemble(elastic_ctx* xc, int32_t lhs, double* rnorm) 
   return ELASTIC_OK;
 }
 
-extern "C" int elastic_j2_correct(elastic_ctx* xc) {
-  if (!xc) return ELASTIC_BAD_ARG;
+extern "C" int elastic_j2_step(elastic_ctx* xc, double s) {
+  if (!xc || !std::isfinite(s)) return ELASTIC_BAD_ARG;
   Context* c = &xc->c;
   MeshDev* m = (MeshDev*)c->mesh;
   J2State* j = (J2State*)c->j2;
@@ -1407,10 +1409,12 @@ extern "C" int elastic_j2_correct(elastic_ctx* xc) {
   hipSetDevice(c->device);
   const int n = m->nnode * 3;
   hipLaunchKernelGGL(k_j2_correct, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, m->id,
-                     (const double*)c->v64[VX], j->u);
+                     (const double*)c->v64[VX], s, j->u);
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return ELASTIC_OK;
 }
+
+extern "C" int elastic_j2_correct(elastic_ctx* xc) { return elastic_j2_step(xc, 1.0); }
 
 extern "C" int elastic_j2_commit(elastic_ctx* xc, double* alpha_max, int64_t* nplastic) {
   if (!xc) return ELASTIC_BAD_ARG;

@@ -44,6 +44,10 @@ class PlasticPath:
     def correct(self):
         self.ctx._chk(self._L.elastic_j2_correct(self.ctx._h))
 
+    def step(self, s: float):
+        """u += s du along the last correction (line search)."""
+        self.ctx._chk(self._L.elastic_j2_step(self.ctx._h, float(s)))
+
     def commit(self):
         a, n = C.c_double(), C.c_int64()
         self.ctx._chk(self._L.elastic_j2_commit(self.ctx._h, C.byref(a), C.byref(n)))
@@ -71,37 +75,54 @@ class PlasticPath:
         self.ctx.close()
 
 
-def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int = 30,
-              precision: int = MIXED, tol: float = 1e-11, batch=None, maxit: int = 50000):
+def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int = 60,
+              precision: int = MIXED, tol: float = 1e-11, batch=None, maxit: int = 50000,
+              counts: dict | None = None, max_halvings: int = 6):
     """Drives every path through `increments` equal load steps.  Per Newton
     iteration each unconverged path re-assembles K_t and R = -f_int on the
     GPU; the systems K_t du = R are solved together in `batch` (a
     serving.Batch) when given, else one after the other; u += du.  A path's
     increment has converged when ||R|| <= rtol * ||R|| of its first iteration.
-    Returns per path the list of (increment, iteration, ||R||)."""
+
+    Backtracking line search: a full step is kept when ||R|| decreases;
+    otherwise the step along du is halved (re-assembling only) up to
+    `max_halvings` times, and the last trial is kept.  Full Newton steps can
+    cycle once the set of yielding points changes from one iterate to the
+    next under a large load step; the halving restores descent.
+    Returns per path the list of (increment, assembly, ||R||); `counts`
+    (optional dict) receives the numbers of linear solves and of halvings."""
     hist = [[] for _ in paths]
+    nsolve = nhalf = 0
     for inc in range(1, increments + 1):
         lam = inc / increments
         for p in paths:
             p.increment(lam)
         active = list(range(len(paths)))
-        r0 = {}
+        r0, r_acc, frac = {}, {}, {}
         it = 0
         while active:
-            need = []
+            need, retry = [], []
             for i in active:
                 r = paths[i].assemble(lhs=True)
                 r0.setdefault(i, r)
                 hist[i].append((inc, it, r))
                 if r <= rtol * r0[i] or r == 0.0:
                     paths[i].commit()
+                elif (i in frac and not (r < (1.0 - 1e-4 * frac[i]) * r_acc[i])
+                      and frac[i] > 0.5 ** max_halvings):
+                    paths[i].step(-0.5 * frac[i])   # back to half the last trial step
+                    frac[i] *= 0.5
+                    nhalf += 1
+                    retry.append(i)
                 else:
+                    r_acc[i] = r                     # accepted point: new Newton direction
+                    frac.pop(i, None)
                     need.append(i)
-            if not need:
+            if not need and not retry:
                 break
             if it >= max_newton:
                 raise ElasticError(_lib.MAXIT, "Newton did not converge in increment %d" % inc)
-            if batch is not None:
+            if batch is not None and need:
                 queue, done = list(need), set()
                 while len(done) < len(need):
                     while queue and batch.submit(paths[queue[0]].ctx, precision, tol, maxit,
@@ -117,6 +138,82 @@ def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int =
                     paths[i].ctx.solve_resident(precision=precision, tol=tol, maxit=maxit)
             for i in need:
                 paths[i].correct()
-            active = need
+                frac[i] = 1.0
+            nsolve += len(need)
+            active = need + retry
             it += 1
+    if counts is not None:
+        counts["linear_solves"] = nsolve
+        counts["halvings"] = nhalf
     return hist
+
+
+def load_paths(n: int = 60, npaths: int = 32, stretch: float = 5e-3, H: float = 0.05,
+               seed: int = 20261207, device: int = 0):
+    """The plasticity workload: hex8 n^3 held between grips (z=0 clamped, z=1
+    pulled by `stretch` at load factor 1), E = 1, nu = 0.3, hardening H, and
+    `npaths` load paths whose yield stresses are drawn from U[5e-4, 2e-3]
+    (counter-based, so every run draws the same values)."""
+    from .mesh import hex8_block_pull, uniform01
+
+    m, ubar = hex8_block_pull(n, stretch)
+    u = uniform01(seed, np.arange(npaths, dtype=np.uint64))
+    sys_y = 5e-4 + u * (2e-3 - 5e-4)
+    return [PlasticPath(m, ubar, float(sy), H, device=device) for sy in sys_y], sys_y
+
+
+def main(argv=None) -> int:
+    """python -m elastic_amd.plasticity [--n 60] [--paths 32] [--increments 10]
+    [--mode batched|sequential] [--precision mixed|fp64]: runs the load paths
+    and prints one JSON line (wall time, Newton iterations, solved DOF/s)."""
+    import argparse
+    import json
+    import time
+
+    from .serving import Batch
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=60)
+    ap.add_argument("--paths", type=int, default=32)
+    ap.add_argument("--increments", type=int, default=10)
+    ap.add_argument("--stretch", type=float, default=5e-3)
+    ap.add_argument("--mode", choices=["batched", "sequential"], default="batched")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    a = ap.parse_args(argv)
+    prec = MIXED if a.precision == "mixed" else FP64
+    t0 = time.perf_counter()
+    paths, sys_y = load_paths(a.n, a.paths, a.stretch)
+    t_setup = time.perf_counter() - t0
+    neq = paths[0].sys.neq
+    t1 = time.perf_counter()
+    counts: dict = {}
+    if a.mode == "batched":
+        with Batch(0, max_slots=max(1, min(a.paths, 255)), iters_per_step=16) as b:
+            hist = run_paths(paths, increments=a.increments, rtol=a.rtol, precision=prec, batch=b,
+                             counts=counts)
+    else:
+        hist = run_paths(paths, increments=a.increments, rtol=a.rtol, precision=prec,
+                         counts=counts)
+    wall = time.perf_counter() - t1
+    solves = counts["linear_solves"]
+    newton = [max(it for (inc, it, _) in h if inc == k) for h in hist
+              for k in range(1, a.increments + 1)]  # assemblies per increment
+    plastic = [p.commit()[1] for p in paths]
+    out = {"workload": "J2 hex8 %d^3 pulled %g, %d paths, sy U[5e-4,2e-3], H 0.05, %d increments"
+                       % (a.n, a.stretch, a.paths, a.increments),
+           "mode": a.mode, "precision": a.precision, "neq_per_path": neq,
+           "setup_s": t_setup, "wall_s": wall, "linear_solves": solves,
+           "line_search_halvings": counts["halvings"],
+           "assemblies_per_increment_max": max(newton), "solved_dof_per_s": neq * solves / wall,
+           "plastic_points_min": int(min(plastic)), "plastic_points_max": int(max(plastic))}
+    print(json.dumps(out), flush=True)
+    for p in paths:
+        p.close()
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(main())

@@ -160,6 +160,9 @@ int elastic_j2_increment(elastic_ctx* ctx, double lambda);
 int elastic_j2_assemble(elastic_ctx* ctx, int32_t lhs, double* rnorm);
 /* u += x (the device x of the last resident solve) on free dofs. */
 int elastic_j2_correct(elastic_ctx* ctx);
+/* u += s x on free dofs: a line-search move along the last correction
+ * (s = 1 is elastic_j2_correct). */
+int elastic_j2_step(elastic_ctx* ctx, double s);
 /* committed state <- state of the last assembly; returns the largest
  * equivalent plastic strain and the number of yielded Gauss points. */
 int elastic_j2_commit(elastic_ctx* ctx, double* alpha_max, int64_t* nplastic);

@@ -150,3 +150,24 @@ def test_j2_batched_paths_equal_alone():
     for i, p in enumerate(paths):
         assert np.array_equal(p.u(), alone[i])
         p.close()
+
+
+def test_j2_line_search_recovers_stalling_newton():
+    """hex8 30^3 pulled 5e-3 in 10 steps, sy = 6.52e-4 (the lowest yield stress
+    of the workload's draws): full Newton steps stall in the first increment
+    (the relative residual sits near 4e-2 and grows); with the backtracking
+    line search every increment converges to rtol, and halvings were used."""
+    from elastic_amd import mesh as M
+    from elastic_amd.plasticity import PlasticPath, run_paths
+
+    m, ubar = M.hex8_block_pull(30, 5e-3)
+    p = PlasticPath(m, ubar, 6.5205e-4, 0.05)
+    counts = {}
+    hist = run_paths([p], increments=2, rtol=1e-8, precision=64, tol=1e-11, counts=counts)
+    assert counts["halvings"] > 0
+    for inc in (1, 2):
+        rs = [r for (i, it, r) in hist[0] if i == inc]
+        assert rs[-1] <= 1e-8 * rs[0]
+    a, n = p.commit()
+    assert n > 0
+    p.close()
