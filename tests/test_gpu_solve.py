@@ -246,3 +246,23 @@ def test_fp32_accumulation_option(golden, name):
         x, st = c.solve(g["b"], precision=MIXED, tol=float(g["tol"]))
         assert st["norm_m_r"] < st["conv"]
         assert rel(x, g["x"]) <= 1e-8
+
+
+def test_product_timing_option(golden):
+    """The measurement option samples every S-th PCG product with HIP events:
+    the count equals the sampled iterations that ran, the average is a
+    positive duration, reset clears it, and the solution is unchanged."""
+    from elastic_amd.solver import Context
+
+    g = golden("hex8_5j")
+    with Context(0) as c:
+        load_system(c, g)
+        x0, s0 = c.solve(g["b"], tol=1e-10)
+        c.set_option("time_products", 4)
+        c.product_timing(reset=True)
+        x1, s1 = c.solve(g["b"], tol=1e-10)
+        ms, n = c.product_timing(reset=True)
+        assert np.array_equal(x0, x1) and s0["iters"] == s1["iters"]
+        assert n == (s1["iters"] + 3) // 4 and ms > 0.0
+        assert c.product_timing() == (0.0, 0)
+        c.set_option("time_products", 0)
