@@ -67,6 +67,9 @@ __device__ inline double apply_m(int i, int n, const double* __restrict__ r,
 // operator from the current x.  A replacement is also forced after
 // kForceReplace iterations without one, so stagnation is always seen.
 constexpr int kForceReplace = 500;
+// Systems of at least this many rows let the host skip the gated
+// replacement passes in poll chunks where the trigger cannot come yet.
+constexpr int kPredictRows = 200000;
 __device__ inline void progress_guard(SolveScalars* sc, double tr) {
   sc->stall = tr > 0.9 * sc->last_true ? sc->stall + 1 : 0;
   sc->last_true = tr;

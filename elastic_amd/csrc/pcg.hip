@@ -400,6 +400,15 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
     for (auto& e : c->tev)
       ELASTIC_HIP_CHECK(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   }
+  // Gated replacement passes cost their launches even when they skip.  On
+  // large systems the host enqueues them only in chunks where, at the decay
+  // rate of the recursive residual seen between polls, the trigger can come
+  // within two chunks (a missed one only waits for the next chunk); small
+  // systems enqueue them every second iteration.
+  const bool predict = reliable && n >= kPredictRows;
+  bool gate = true;
+  double rn_prev = -1.0;
+  int it_prev = 0;
   while (it < maxit) {
     const int chunk = std::min(every, maxit - it);
     const int it0 = it;
@@ -446,7 +455,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
       hipLaunchKernelGGL(k_pcg_pupdate, G, TB, 0, s, n, part_b, (const double*)part_zq, dcur, dnxt,
                          V.p, V.r, V.m, V.bm, pr_next, p32, sc);
       // the replacement kernels gate themselves on the flag set by k_pcg_pupdate
-      if (reliable && ((it + 1) & 1) == 0) replace(it + 1);
+      if (reliable && gate && ((it + 1) & 1) == 0) replace(it + 1);
     }
     ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->scal_host, sc, sizeof(SolveScalars),
                                         hipMemcpyDeviceToHost, s));
@@ -465,6 +474,19 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
       break;
     }
     if (c->scal_host->op64) op32 = false;
+    if (predict) {
+      const SolveScalars& h = *c->scal_host;
+      const double rn = std::sqrt(std::fabs(h.di));
+      const double target = std::max(h.delta * h.rmax, h.conv);
+      gate = true;
+      if (!h.upd && rn > target && rn_prev > 0.0 && h.iter > it_prev &&
+          h.iter - h.last_upd + 2 * every < kForceReplace) {
+        const double rate = std::pow(rn / rn_prev, 1.0 / (double)(h.iter - it_prev));
+        if (rate < 1.0 && std::log(target / rn) / std::log(rate) > 2.0 * every) gate = false;
+      }
+      rn_prev = rn;
+      it_prev = h.iter;
+    }
   }
   if (!done && reliable) {  // maxit: leave the true residual of the final x
     hipLaunchKernelGGL(k_rel_force, dim3(1), dim3(64), 0, s, sc);
