@@ -32,7 +32,9 @@ namespace elastic {
 
 constexpr int kChunk = 2048;         // rows per vector work item at most (256 lanes x 8)
 constexpr int kMaxSlots = 255;       // slot index lives in the top 8 bits of a work item
-constexpr int kBatchPer = 2;         // node blocks per lane (tiles of <= 512 blocks)
+// node blocks per lane: 2-D tiles of <= 512 blocks, 3-D tiles of <= 1024
+template <int NB>
+constexpr int batch_per() { return NB == 3 ? 4 : 2; }
 
 struct BInst {
   BlkArgs a;
@@ -105,9 +107,10 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
     const TV* vals = f64 ? (const TV*)I.v64 : (const TV*)I.v32;
     const TV* diag = f64 ? (const TV*)I.d64 : (const TV*)I.d32;
     if (MODE == 0)
-      block_tile<TV, TA, NB, kBatchPer, true>(I.a, t, vals, diag, I.p, I.q, I.epart, smem, I.tdot);
+      block_tile<TV, TA, NB, batch_per<NB>(), true>(I.a, t, vals, diag, I.p, I.q, I.epart, smem,
+                                                    I.tdot);
     else
-      block_tile<TV, TA, NB, kBatchPer>(I.a, t, vals, diag, I.x, I.t, I.epart, smem);
+      block_tile<TV, TA, NB, batch_per<NB>()>(I.a, t, vals, diag, I.x, I.t, I.epart, smem);
   }
 }
 
@@ -687,8 +690,8 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   Context* c = &x->c;
   if (!c->values_set) return berr(b, ELASTIC_NOT_READY, "context has no values");
   if (c->layout_nb < 2) return berr(b, ELASTIC_BAD_ARG, "batched solves need a node-blocked layout");
-  if (c->blk.nblk_max > 256 * kBatchPer)
-    return berr(b, ELASTIC_BAD_ARG, "tiles of more than 512 blocks");
+  if (c->blk.nblk_max > 256 * (c->layout_nb == 3 ? batch_per<3>() : batch_per<2>()))
+    return berr(b, ELASTIC_BAD_ARG, "tiles longer than the batch product holds (512 blocks in 2-D, 1024 in 3-D)");
   if (c->zero_diag_row > 0) return berr(b, ELASTIC_ZERO_DIAG, "zero diagonal coefficient");
   if (precision != ELASTIC_FP64 && precision != ELASTIC_MIXED) return ELASTIC_BAD_ARG;
   if (maxit <= 0) return ELASTIC_BAD_ARG;

@@ -105,7 +105,11 @@ static int build_layout(elastic_ctx* x) {
   const char* lay = getenv("ELASTIC_LAYOUT");
   if (c->allow_blocks && !(lay && std::string(lay) == "scalar")) nb = detect_node_blocks(neq, ia, ja);
   try {
-    if (nb > 1) build_block_tiles(neq, nb, ia, ja, c->tile_blocks, c->blayout);
+    if (nb > 1)
+      build_block_tiles(neq, nb, ia, ja,
+                        c->tile_blocks > 0 ? c->tile_blocks
+                        : (nb == 3 && neq / 3 >= kLargeTileNodes ? kTileBlocks3 : kTileBlocks2),
+                        c->blayout);
     else build_tiles(neq, ia, ja, c->tile_nnz, c->layout);
   } catch (const std::exception& e) {
     return set_error(c, ELASTIC_BAD_ARG, e.what());

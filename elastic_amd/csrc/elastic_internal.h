@@ -136,6 +136,14 @@ struct SolveScalars {
 
 constexpr int kRedBlocks = 1024;  // fixed reduction grid (deterministic sums)
 
+// Default node blocks per tile: large 3-dof systems take the largest tiles
+// (fewer external partials to write and merge; the FP32 operator's fastest
+// size on hex8 100^3); smaller systems and 2-dof nodes keep 512 (more tiles,
+// so more workgroups in flight; the batch's 2-D product holds 2 blocks per
+// lane).
+constexpr int kTileBlocks3 = 1024, kTileBlocks2 = 512;
+constexpr int kLargeTileNodes = 32768;
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -143,7 +151,7 @@ struct Context {
   int neq = 0;
   int64_t nad = 0;
   int tile_nnz = 2048;
-  int tile_blocks = 512;
+  int tile_blocks = 0;         // node blocks per tile; 0: by node size (kTileBlocks3/2)
   int layout_nb = 1;           // 1: scalar tiles; 2 or 3: node-blocked tiles
   bool accum32 = false;        // FP32 operator: FP32 staging and sums (else FP64)
   bool allow_blocks = true;
