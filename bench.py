@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving", type=int, default=512,
                     help="requests of the plane-strain serving stream (0: skip)")
+    ap.add_argument("--serving-mixed", action="store_true",
+                    help="also serve the stream with mixed-precision instances")
     ap.add_argument("--serving-compare", type=int, default=48,
                     help="requests in the solve-only batched vs sequential comparison")
     args = ap.parse_args()
@@ -158,7 +160,7 @@ def main():
     if args.serving > 0 or args.serving_compare > 0:
         ctx.close()
         del sysm, ctx
-        serving = serving_bench(args.serving, args.serving_compare)
+        serving = serving_bench(args.serving, args.serving_compare, args.serving_mixed)
 
     value = neq * args.steps / t_total
     out = {
@@ -207,10 +209,10 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def serving_bench(nreq: int, ncompare: int):
+def serving_bench(nreq: int, ncompare: int, stream_mixed: bool = False):
     """The plane-strain request stream (half quad4, half tria3, side U{32..512},
     Poisson arrivals at 200 requests/s) served by the batch with continuous
-    admission, once with FP64 and once with mixed-precision instances:
+    admission, with FP64 instances (and with mixed-precision ones when asked):
     throughput and p50/p99 latency from arrival to x on the host (host set-up
     and GPU assembly included).  Then the solve-only comparison on the first
     `ncompare` requests with every system resident: plain FP64 PCG solved one
@@ -222,7 +224,9 @@ def serving_bench(nreq: int, ncompare: int):
     workers = min(16, os.cpu_count() or 1)
     out = {"workload": ("%d plane-strain requests (even quad4, odd tria3, side U{32..512}), "
                         "Poisson arrivals 200/s, to %g" % (nreq, TOL))}
-    for name, prec in (("stream_batched_fp64", FP64), ("stream_batched_mixed", MIXED)):
+    streams = [("stream_batched_fp64", FP64)] + ([("stream_batched_mixed", MIXED)]
+                                                   if stream_mixed else [])
+    for name, prec in streams:
         if nreq <= 0:
             break
         srv = Server(max_slots=64, iters_per_step=16, prep_workers=workers, precision=prec,
