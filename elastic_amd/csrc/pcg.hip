@@ -15,11 +15,15 @@
 // iterations.
 //
 // Mixed precision (TV = float): the operator of the iteration reads FP32
-// matrix values (products and sums in FP64).  Whenever the recursively updated
-// residual has dropped by `delta` since the last replacement, it is replaced by
-// the FP64 true residual r = b - A64 x (an FP64 product with the FP64 values)
-// and the search direction is kept, so the Krylov iteration is not restarted;
-// convergence is decided on that true residual only.
+// matrix values (sums in FP64, or in FP32 when the context asks for it: then
+// x is staged from an FP32 copy of p that the p update writes, and the own
+// rows of q and the external partials are stored in FP32, all exactly the
+// values the FP32 sums produce).  Whenever the recursively updated residual
+// has dropped by `delta` since the last replacement, it is replaced by the
+// FP64 true residual r = b - A64 x (an FP64 product with the FP64 values)
+// and the search direction is kept, so the Krylov iteration is not
+// restarted; the recurrences are the flexible ones (alpha = p.r / p.q,
+// beta = -z.q / p.q), and convergence is decided on the true residual only.
 #include <algorithm>
 #include <cmath>
 

@@ -221,7 +221,8 @@ int elastic_layout_info(elastic_ctx* ctx, int64_t* info4);
 /* Tile size of the scalar row tiles (nonzeros per tile, 64..3840); takes
  * effect at the next elastic_set_structure. */
 int elastic_set_tile_nnz(elastic_ctx* ctx, int32_t tile_nnz);
-/* Tile size of the node-blocked layout (node blocks per tile, 32..1024);
+/* Tile size of the node-blocked layout (node blocks per tile, 32..1024; by
+ * default 1024 for 3-dof systems of at least 32,768 nodes, else 512);
  * takes effect at the next elastic_set_structure. */
 int elastic_set_tile_blocks(elastic_ctx* ctx, int32_t tile_blocks);
 /* Tuning knobs (no counterpart in the reference): "tile_blocks" /
