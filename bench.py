@@ -52,6 +52,9 @@ def main():
                     help="also serve the stream with mixed-precision instances")
     ap.add_argument("--serving-compare", type=int, default=48,
                     help="requests in the solve-only batched vs sequential comparison")
+    ap.add_argument("--plastic-compare", type=int, default=32,
+                    help="J2 hex8 60^3 load paths whose tangent systems are solved "
+                         "sequentially and batched (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve path is not sharded across GPUs; "
@@ -161,6 +164,10 @@ def main():
         ctx.close()
         del sysm, ctx
         serving = serving_bench(args.serving, args.serving_compare, args.serving_mixed)
+    plastic = None
+    if args.plastic_compare > 0:
+        phase("serving done")
+        plastic = plastic_compare(args.plastic_compare)
 
     value = neq * args.steps / t_total
     out = {
@@ -205,6 +212,7 @@ def main():
         "copy_probe": copy,
         "cpu_baseline": cpu,
         "serving": serving,
+        "batched_3d": plastic,
     }
     print(json.dumps(out), flush=True)
 
@@ -281,6 +289,39 @@ def serving_bench(nreq: int, ncompare: int, stream_mixed: bool = False):
     for s_ in systems:
         s_.ctx.close()
     out["solve_only"] = compare
+    return out
+
+
+def plastic_compare(npaths: int, n: int = 60):
+    """The J2 workload's systems (hex8 60^3 between grips, E = 1, nu = 0.3,
+    H = 0.05, yield stresses U[5e-4, 2e-3], load in 10 increments): the paths
+    run increment 1 (batched mixed-precision Newton), then every path
+    assembles its tangent system of increment 2's first Newton iteration.
+    Those systems are solved to 1e-10 by plain FP64 PCG one after the other,
+    and together in a batch in FP64 and in mixed precision."""
+    from elastic_amd.plasticity import compare_solves, load_paths, run_paths
+    from elastic_amd.serving import Batch
+    from elastic_amd.solver import MIXED
+
+    a = time.perf_counter()
+    paths, _ = load_paths(n, npaths)
+    t_setup = time.perf_counter() - a
+    with Batch(0, max_slots=max(1, min(npaths, 255)), iters_per_step=16) as b:
+        run_paths(paths, increments=10, precision=MIXED, batch=b, last=1)
+    plastic = [int((p.state()[1] > 0).sum()) for p in paths]  # Gauss points, committed
+    for p in paths:
+        p.increment(0.2)
+        p.assemble(lhs=True)
+    print("plastic compare: %d systems of %d equations" % (npaths, paths[0].sys.neq),
+          file=sys.stderr, flush=True)
+    out = compare_solves(paths, tol=TOL)
+    out["workload"] = ("J2 hex8 %d^3 tangent systems of increment 2 (first Newton iteration), "
+                       "%d load paths, %d equations each" % (n, npaths, paths[0].sys.neq))
+    out["setup_s"] = t_setup
+    out["plastic_points_after_increment_1"] = {"min": min(plastic), "max": max(plastic),
+                                               "of": 8 * paths[0].mesh.numel}
+    for p in paths:
+        p.close()
     return out
 
 

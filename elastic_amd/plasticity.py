@@ -77,7 +77,7 @@ class PlasticPath:
 
 def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int = 60,
               precision: int = MIXED, tol: float = 1e-11, batch=None, maxit: int = 50000,
-              counts: dict | None = None, max_halvings: int = 6):
+              counts: dict | None = None, max_halvings: int = 6, last: int | None = None):
     """Drives every path through `increments` equal load steps.  Per Newton
     iteration each unconverged path re-assembles K_t and R = -f_int on the
     GPU; the systems K_t du = R are solved together in `batch` (a
@@ -90,10 +90,11 @@ def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int =
     cycle once the set of yielding points changes from one iterate to the
     next under a large load step; the halving restores descent.
     Returns per path the list of (increment, assembly, ||R||); `counts`
-    (optional dict) receives the numbers of linear solves and of halvings."""
+    (optional dict) receives the numbers of linear solves and of halvings.
+    `last` stops after that increment (the load still split in `increments`)."""
     hist = [[] for _ in paths]
     nsolve = nhalf = 0
-    for inc in range(1, increments + 1):
+    for inc in range(1, (last or increments) + 1):
         lam = inc / increments
         for p in paths:
             p.increment(lam)
@@ -146,6 +147,59 @@ def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int =
         counts["linear_solves"] = nsolve
         counts["halvings"] = nhalf
     return hist
+
+
+def compare_solves(paths, tol: float = 1e-10, maxit: int = 50000, fp32_sums: bool = True) -> dict:
+    """Solve-only comparison on the paths' systems as last assembled (K_t du =
+    R of each path, resident): plain FP64 PCG one system after the other (the
+    drop-in solver call), then all systems together in a batch, in FP64 and in
+    mixed precision (FP64 sums in the FP32 operator, and with `fp32_sums`
+    also FP32 sums).  Returns wall times, solved DOF/s, iteration totals and
+    the largest relative L2 difference of du against the sequential FP64
+    solutions."""
+    import time
+
+    import torch
+
+    from .serving import Batch
+
+    dof = float(sum(p.sys.neq for p in paths))
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    it64 = 0
+    for p in paths:
+        it64 += p.ctx.solve_resident(precision=FP64, tol=tol, maxit=maxit)["iters"]
+    t_seq = time.perf_counter() - a
+    x64 = [p.ctx.download_x() for p in paths]
+    out = {"systems": len(paths), "dof": dof, "tol": tol,
+           "sequential_fp64": {"s": t_seq, "dof_per_s": dof / t_seq, "iters": it64}}
+    runs = [("batched_fp64", FP64, 0), ("batched_mixed", MIXED, 0)]
+    if fp32_sums:
+        runs.append(("batched_mixed_fp32_sums", MIXED, 1))
+    for name, prec, acc in runs:
+        for p in paths:
+            p.ctx.set_option("accum32", acc)
+        a = time.perf_counter()
+        done, itb = {}, 0
+        with Batch(0, max_slots=max(1, min(len(paths), 255)), iters_per_step=16) as b:
+            queue = list(range(len(paths)))
+            while len(done) < len(paths):
+                while queue and b.submit(paths[queue[0]].ctx, prec, tol, maxit, queue[0]):
+                    queue.pop(0)
+                b.step()
+                for tag, st in b.poll():
+                    if st["status"] != 0:
+                        raise ElasticError(st["status"], "linear solve of path %d" % tag)
+                    done[tag] = st
+                    itb += st["iters"]
+        t_bat = time.perf_counter() - a
+        err = max(float(np.linalg.norm(p.ctx.download_x() - x64[i]) / np.linalg.norm(x64[i]))
+                  for i, p in enumerate(paths))
+        out[name] = {"s": t_bat, "dof_per_s": dof / t_bat, "iters": itb,
+                     "speedup_vs_sequential_fp64": t_seq / t_bat, "max_rel_err_vs_fp64": err}
+    for p in paths:
+        p.ctx.set_option("accum32", 0)
+    return out
 
 
 def load_paths(n: int = 60, npaths: int = 32, stretch: float = 5e-3, H: float = 0.05,
