@@ -79,39 +79,45 @@ __device__ inline double inst_sum(const double* __restrict__ part, int nch, doub
 // FP64, the FP32 lists hold the mixed instances that are still on FP32.
 // mode 1 (FP64 lists): t = A64 x for mixed instances that asked for a true
 // residual (reliable replacement).
+//
+// Every lane reads the instance's flags itself (uniform addresses that no
+// kernel writes while this one runs, so all lanes see the same values).
+// Where the item is almost always served (FP64 instances on the FP64 list,
+// FP32 instances on an FP32 list) the tile's loads are issued first and the
+// flags are checked while they are in flight; the other cases (mixed
+// instances on the FP64 list, true-residual passes) check before loading, so
+// the items they skip cost no value traffic.
 template <typename TV, typename TA, int NB, int MODE>
 __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts,
                                                  const uint32_t* __restrict__ work,
                                                  const int32_t* __restrict__ count) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int flag;
   constexpr bool f64 = sizeof(TV) == 8;
+  constexpr int PER = batch_per<NB>();
   const int w = blockIdx.x;
   if (w >= *count) return;
-  {
-    const uint32_t e = work[w];
-    const BInst& I = insts[wl_slot(e)];
-    const int t = wl_index(e);
-    if (threadIdx.x == 0) {
-      if (!I.live || t >= I.ntiles) {
-        flag = 0;  // not a work item of an admitted instance
-      } else {
-        const SolveScalars* sc = I.sc;
-        if (MODE == 1) flag = I.fp32 && sc->upd && !sc->done;
-        else if (f64) flag = !sc->done && (!I.fp32 || sc->op64);
-        else flag = !sc->done && !sc->op64;
-      }
-    }
-    __syncthreads();
-    if (!flag) return;
-    const TV* vals = f64 ? (const TV*)I.v64 : (const TV*)I.v32;
-    const TV* diag = f64 ? (const TV*)I.d64 : (const TV*)I.d32;
-    if (MODE == 0)
-      block_tile<TV, TA, NB, batch_per<NB>(), true>(I.a, t, vals, diag, I.p, I.q, I.epart, smem,
-                                                    I.tdot);
-    else
-      block_tile<TV, TA, NB, batch_per<NB>()>(I.a, t, vals, diag, I.x, I.t, I.epart, smem);
+  const uint32_t e = work[w];
+  const BInst& I = insts[wl_slot(e)];
+  const int t = wl_index(e);
+  if (!I.live || t >= I.ntiles) return;  // not a work item of an admitted instance
+  const SolveScalars* sc = I.sc;
+  const bool early = MODE == 1 || (f64 && I.fp32);
+  if (early) {
+    const bool run = MODE == 1 ? (I.fp32 && sc->upd && !sc->done) : (!sc->done && sc->op64);
+    if (!run) return;
   }
+  const TV* vals = f64 ? (const TV*)I.v64 : (const TV*)I.v32;
+  const TV* diag = f64 ? (const TV*)I.d64 : (const TV*)I.d32;
+  TileRegs<TV, NB, PER> R;
+  load_tile<TV, NB, PER>(I.a, t, vals, diag, R);
+  if (!early) {
+    const bool run = f64 ? !sc->done : (!sc->done && !sc->op64);
+    if (!run) return;
+  }
+  if (MODE == 0)
+    compute_tile<TV, TA, NB, PER, true>(I.a, t, R, I.p, I.q, I.epart, smem, I.tdot);
+  else
+    compute_tile<TV, TA, NB, PER>(I.a, t, R, I.x, I.t, I.epart, smem);
 }
 
 // operator variants: v = 3 * (nb == 3) + {0: FP64, 1: FP32 values FP64 sums,
