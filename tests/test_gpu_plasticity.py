@@ -171,3 +171,30 @@ def test_j2_line_search_recovers_stalling_newton():
     a, n = p.commit()
     assert n > 0
     p.close()
+
+
+def test_compare_solves_batched_against_sequential():
+    """The bench's solve-only comparison on J2 tangent systems: after one
+    increment (stopped by `last`), every path's next tangent system solved
+    by plain FP64 PCG one after the other, then batched in FP64 and mixed
+    precision (FP64 and FP32 sums): every batched du within 1e-8 of the
+    sequential FP64 one, and FP64 iteration totals within 2 per system."""
+    from elastic_amd.plasticity import PlasticPath, compare_solves, run_paths
+    from elastic_amd.serving import Batch
+
+    m, ubar = block(N=4, stretch=4e-3)
+    paths = [PlasticPath(m, ubar, sy, H) for sy in (5e-4, 1e-3, 2e-3)]
+    with Batch(0, max_slots=3, iters_per_step=8) as b:
+        hist = run_paths(paths, increments=3, rtol=1e-8, batch=b, last=1)
+    assert all(inc == 1 for h in hist for (inc, _, _) in h)
+    for p in paths:
+        p.increment(2.0 / 3.0)
+        p.assemble(lhs=True)
+    out = compare_solves(paths, tol=1e-10)
+    n = len(paths)
+    assert out["systems"] == n and out["dof"] == n * paths[0].sys.neq
+    for k in ("batched_fp64", "batched_mixed", "batched_mixed_fp32_sums"):
+        assert out[k]["max_rel_err_vs_fp64"] <= 1e-8, k
+    assert abs(out["batched_fp64"]["iters"] - out["sequential_fp64"]["iters"]) <= 2 * n
+    for p in paths:
+        p.close()
