@@ -215,7 +215,7 @@ def load_paths(n: int = 60, npaths: int = 32, stretch: float = 5e-3, H: float = 
 
 def main(argv=None) -> int:
     """python -m elastic_amd.plasticity [--n 60] [--paths 32] [--increments 10]
-    [--mode batched|sequential] [--precision mixed|fp64]: runs the load paths
+    [--mode batched|sequential] [--precision mixed|fp64] [--sums fp64|fp32]: runs the load paths
     and prints one JSON line (wall time, Newton iterations, solved DOF/s)."""
     import argparse
     import json
@@ -231,10 +231,14 @@ def main(argv=None) -> int:
     ap.add_argument("--mode", choices=["batched", "sequential"], default="batched")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--sums", choices=["fp64", "fp32"], default="fp64",
+                    help="sums of the FP32 operator in mixed-precision solves")
     a = ap.parse_args(argv)
     prec = MIXED if a.precision == "mixed" else FP64
     t0 = time.perf_counter()
     paths, sys_y = load_paths(a.n, a.paths, a.stretch)
+    for p in paths:
+        p.ctx.set_option("accum32", 1 if a.sums == "fp32" else 0)
     t_setup = time.perf_counter() - t0
     neq = paths[0].sys.neq
     t1 = time.perf_counter()
@@ -253,7 +257,8 @@ def main(argv=None) -> int:
     plastic = [p.commit()[1] for p in paths]
     out = {"workload": "J2 hex8 %d^3 pulled %g, %d paths, sy U[5e-4,2e-3], H 0.05, %d increments"
                        % (a.n, a.stretch, a.paths, a.increments),
-           "mode": a.mode, "precision": a.precision, "neq_per_path": neq,
+           "mode": a.mode, "precision": a.precision, "operator_sums": a.sums,
+           "neq_per_path": neq,
            "setup_s": t_setup, "wall_s": wall, "linear_solves": solves,
            "line_search_halvings": counts["halvings"],
            "assemblies_per_increment_max": max(newton), "solved_dof_per_s": neq * solves / wall,
