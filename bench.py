@@ -308,7 +308,9 @@ def plastic_compare(npaths: int, n: int = 60):
     t_setup = time.perf_counter() - a
     with Batch(0, max_slots=max(1, min(npaths, 255)), iters_per_step=16) as b:
         run_paths(paths, increments=10, precision=MIXED, batch=b, last=1)
-    plastic = [int((p.state()[1] > 0).sum()) for p in paths]  # Gauss points, committed
+    # plastic Gauss points of the committed state (committing the converged
+    # state again changes nothing and returns the count)
+    plastic = [int(p.commit()[1]) for p in paths]
     for p in paths:
         p.increment(0.2)
         p.assemble(lhs=True)
