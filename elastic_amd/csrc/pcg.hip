@@ -37,9 +37,9 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const double* __re
                                                         const double* __restrict__ m,
                                                         const double* __restrict__ bm, double* x,
                                                         double* r, double* p, double* part,
-                                                        double* part_pr, float* p32) {
+                                                        double* part_pr, float* p32, int p32only) {
   __shared__ double sh[8];
-  double acc = 0.0;
+  double acc = 0.0, apr = 0.0;
   GRID_LOOP(i, n) {
     const double zi = apply_m(i, n, b, m, bm);
     x[i] = 0.0;
@@ -47,10 +47,14 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const double* __re
     p[i] = zi;
     if (p32) p32[i] = (float)zi;
     acc += b[i] * zi;
+    if (p32only) apr += b[i] * (double)(float)zi;
   }
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (part_pr && threadIdx.x == 0) part_pr[blockIdx.x] = s;  // p.r = z.b
+  if (part_pr) {  // p.r = z.b (of the FP32 direction when only that one is kept)
+    const double t = p32only ? block_sum(apr, sh) : s;
+    if (threadIdx.x == 0) part_pr[blockIdx.x] = t;
+  }
 }
 
 __global__ void k_pcg_init2(const double* __restrict__ part, double tol, double delta,
@@ -145,13 +149,15 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int n, const double* __
 // and the vector loads are issued before that gather.  p.q comes from the
 // product's tiles (sc->pq_tiles).  EP / QT: types of the stored partials and
 // of the product's own rows (FP32 after a product with FP32 sums that was
-// asked for FP32 rows; q is then not written back).
-template <int NB, typename EP, typename QT>
+// asked for FP32 rows; q is then not written back).  PT: type of the search
+// direction (FP32 when the mixed method keeps only its FP32 copy, the vector
+// the product staged, so x += alpha p moves along exactly that vector).
+template <int NB, typename EP, typename QT, typename PT>
 __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs mg,
                                                                 const double* __restrict__ part_pr,
                                                                 const double* __restrict__ dslot,
                                                                 double* x,
-                                                                const double* __restrict__ p,
+                                                                const PT* __restrict__ p,
                                                                 double* r, double* q, int keep_q,
                                                                 const double* __restrict__ m,
                                                                 int jacobi, double* part_rz,
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
     for (int c = 0; c < NB; ++c) {
       qv[c] = (double)reinterpret_cast<const QT*>(q)[i0 + c];
       xv[c] = x[i0 + c];
-      pv[c] = p[i0 + c];
+      pv[c] = (double)p[i0 + c];
       rv[c] = r[i0 + c];
       mv[c] = m[i0 + c];
     }
@@ -238,13 +244,34 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
                                                            const double* __restrict__ m,
                                                            const double* __restrict__ bm,
                                                            double* part_pr, float* p32,
-                                                           SolveScalars* sc) {
+                                                           int p32only, SolveScalars* sc) {
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
   const double di = grid_sum(part_rz, sh);
   const double beta = part_zq ? -grid_sum(part_zq, sh) / sc->pq : di / *dold;
   double apr = 0.0;
-  if (bm == nullptr) {
+  if (p32only) {
+    // the direction lives only in FP32 (Jacobi): p = fl32(m r + beta p), and
+    // p.r of the stored values; same four rows per lane as below
+    const int n4 = n >> 2;
+    for (int k = blockIdx.x * kThreads + threadIdx.x; k < n4; k += gridDim.x * kThreads) {
+      const double2* r2 = reinterpret_cast<const double2*>(r) + 2 * k;
+      const double2* m2 = reinterpret_cast<const double2*>(m) + 2 * k;
+      const double2 ra = r2[0], rb = r2[1], ma = m2[0], mb = m2[1];
+      const float4 pf = reinterpret_cast<const float4*>(p32)[k];
+      const float4 o = make_float4((float)(ra.x * ma.x + beta * (double)pf.x),
+                                   (float)(ra.y * ma.y + beta * (double)pf.y),
+                                   (float)(rb.x * mb.x + beta * (double)pf.z),
+                                   (float)(rb.y * mb.y + beta * (double)pf.w));
+      reinterpret_cast<float4*>(p32)[k] = o;
+      apr += (double)o.x * ra.x + (double)o.y * ra.y + (double)o.z * rb.x + (double)o.w * rb.y;
+    }
+    for (int i = 4 * n4 + blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+      const float pi = (float)(r[i] * m[i] + beta * (double)p32[i]);
+      p32[i] = pi;
+      apr += (double)pi * r[i];
+    }
+  } else if (bm == nullptr) {
     // Jacobi: four rows per lane as two 16-byte loads of r, m and p each, so
     // that enough bytes are in flight at 4 waves per SIMD; vectors are
     // 256-byte aligned (hipMalloc), rows 4k..4k+3 are 32-byte aligned
@@ -306,13 +333,15 @@ __global__ void k_rel_force(SolveScalars* sc) {
 }
 
 // r = b - (t + merged partials); with Jacobi also partial r.(m r); with
-// part_pr also the partial p.r of the replaced residual
+// part_pr also the partial p.r of the replaced residual (p read from p32
+// when the direction lives only in FP32)
 __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
                                                          const double* __restrict__ t,
                                                          const double* __restrict__ b,
                                                          const double* __restrict__ m, double* r,
                                                          int jacobi, double* part,
                                                          const double* __restrict__ p,
+                                                         const float* __restrict__ p32,
                                                          double* part_pr, const SolveScalars* sc) {
   __shared__ double sh[8];
   if (!block_flag(&sc->upd, sh) || block_flag(&sc->done, sh)) return;
@@ -323,7 +352,7 @@ __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
     const double ri = b[i] - s;
     r[i] = ri;
     acc += ri * (ri * m[i]);
-    if (part_pr) apr += p[i] * ri;
+    if (part_pr) apr += (p32 ? (double)p32[i] : p[i]) * ri;
   }
   if (jacobi) {
     const double s = block_sum(acc, sh);
@@ -376,15 +405,19 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   double* pr_next = tdots ? part_pr : nullptr;
   // FP32 copy of p for the FP32-sum operator's staging (node-blocked layouts)
   float* p32 = (tdots && fp32_operator && c->accum32) ? c->p32 : nullptr;
+  // with Jacobi and the FP32-sum operator the direction lives only in p32
+  // (the FP64 p is filled from it if the progress guard hands over to FP64)
+  bool p32only = p32 != nullptr && jac && reliable && epart_f32(c, true);
   hipLaunchKernelGGL(k_pcg_init, G, TB, 0, s, n, V.b, V.m, V.bm, V.x, V.r, V.p, part_a, pr_next,
-                     p32);
+                     p32, p32only ? 1 : 0);
   hipLaunchKernelGGL(k_pcg_init2, dim3(1), TB, 0, s, part_a, tol, delta, reliable ? 1 : 0, dslots,
                      sc);
   auto replace = [&](int iters_so_far) {
     double* dnext = dslots + (iters_so_far & 1);
     launch_spmv(c, false, V.x, V.t, V.epart, &sc->skip64);
     hipLaunchKernelGGL(k_rel_resid, G, TB, 0, s, n, merge_of(c, V.epart),
-                       V.t, V.b, V.m, V.r, jac, part_b, (const double*)V.p, pr_next, sc);
+                       V.t, V.b, V.m, V.r, jac, part_b, (const double*)V.p,
+                       (const float*)(p32only ? p32 : nullptr), pr_next, sc);
     if (!jac)
       hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)nullptr,
                          (double*)nullptr, &sc->upd, sc);
@@ -430,18 +463,20 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
         if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j + 1], s));
         // q is read again only by the block preconditioner's (M^-1 r).q
         const MergeArgs mg = merge_of(c, V.epart, e32);
-#define ELASTIC_UPD(NB, EP, QT)                                                                \
-  hipLaunchKernelGGL((k_pcg_update_nodes<NB, EP, QT>), G, TB, 0, s, n / NB, mg,                \
-                     (const double*)part_pr, dcur, V.x, V.p, V.r, V.q, keep_q, V.m, jac, part_b, \
-                     part_zq, sc)
+#define ELASTIC_UPD(NB, EP, QT, PT, P)                                                         \
+  hipLaunchKernelGGL((k_pcg_update_nodes<NB, EP, QT, PT>), G, TB, 0, s, n / NB, mg,            \
+                     (const double*)part_pr, dcur, V.x, (const PT*)(P), V.r, V.q, keep_q, V.m,  \
+                     jac, part_b, part_zq, sc)
         if (c->layout_nb == 3) {
-          if (q32) ELASTIC_UPD(3, float, float);
-          else if (e32) ELASTIC_UPD(3, float, double);
-          else ELASTIC_UPD(3, double, double);
+          if (q32 && p32only) ELASTIC_UPD(3, float, float, float, p32);
+          else if (q32) ELASTIC_UPD(3, float, float, double, V.p);
+          else if (e32) ELASTIC_UPD(3, float, double, double, V.p);
+          else ELASTIC_UPD(3, double, double, double, V.p);
         } else {
-          if (q32) ELASTIC_UPD(2, float, float);
-          else if (e32) ELASTIC_UPD(2, float, double);
-          else ELASTIC_UPD(2, double, double);
+          if (q32 && p32only) ELASTIC_UPD(2, float, float, float, p32);
+          else if (q32) ELASTIC_UPD(2, float, float, double, V.p);
+          else if (e32) ELASTIC_UPD(2, float, double, double, V.p);
+          else ELASTIC_UPD(2, double, double, double, V.p);
         }
 #undef ELASTIC_UPD
       } else {
@@ -457,7 +492,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
         hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)V.q,
                            part_zq, (const int32_t*)nullptr, sc);
       hipLaunchKernelGGL(k_pcg_pupdate, G, TB, 0, s, n, part_b, (const double*)part_zq, dcur, dnxt,
-                         V.p, V.r, V.m, V.bm, pr_next, p32, sc);
+                         V.p, V.r, V.m, V.bm, pr_next, p32, p32only ? 1 : 0, sc);
       // the replacement kernels gate themselves on the flag set by k_pcg_pupdate
       if (reliable && gate && ((it + 1) & 1) == 0) replace(it + 1);
     }
@@ -477,7 +512,13 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
       done = true;
       break;
     }
-    if (c->scal_host->op64) op32 = false;
+    if (c->scal_host->op64 && op32) {
+      op32 = false;
+      if (p32only) {  // the FP64 operator reads the FP64 direction from here on
+        launch_cast<float, double>(n, p32, V.p, s);
+        p32only = false;
+      }
+    }
     if (predict) {
       const SolveScalars& h = *c->scal_host;
       const double rn = std::sqrt(std::fabs(h.di));
