@@ -167,7 +167,9 @@ struct Context {
   // workspace
   double *epart64 = nullptr, *red64 = nullptr;
   double* tdot = nullptr;  // per-tile shares of p.(A p) (node-blocked layouts)
-  float* p32 = nullptr;    // FP32 copy of p for the FP32-sum operator (node-blocked)
+  // FP32 vectors of the mixed PCG with the FP32-sum operator (node-blocked):
+  // p at p32, r at p32 + vec32_stride(neq), the Jacobi scale after that
+  float* p32 = nullptr;
   double *v64[8] = {nullptr};
   SolveScalars* scal = nullptr;
   SolveScalars* scal_host = nullptr;  // pinned
@@ -227,6 +229,8 @@ enum VecSlot { VX = 0, VR, VP, VQ, VB, VT, VU, VW };
 int set_error(Context* c, int code, const std::string& msg);
 // Partials of a product with FP32 sums (node-blocked layouts) are stored as
 // FP32 (exactly the values summed); every other product stores FP64.
+// distance between the FP32 vectors held at Context::p32 (256-byte aligned)
+inline size_t vec32_stride(int n) { return ((size_t)n + 63) & ~(size_t)63; }
 inline bool epart_f32(const Context* c, bool fp32_operator) {
   return fp32_operator && c->accum32 && c->layout_nb > 1;
 }

@@ -37,7 +37,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const double* __re
                                                         const double* __restrict__ m,
                                                         const double* __restrict__ bm, double* x,
                                                         double* r, double* p, double* part,
-                                                        double* part_pr, float* p32, int p32only) {
+                                                        double* part_pr, float* p32,
+                                                        float* r32) {
   __shared__ double sh[8];
   double acc = 0.0, apr = 0.0;
   GRID_LOOP(i, n) {
@@ -47,12 +48,15 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const double* __re
     p[i] = zi;
     if (p32) p32[i] = (float)zi;
     acc += b[i] * zi;
-    if (p32only) apr += b[i] * (double)(float)zi;
+    if (r32) {
+      r32[i] = (float)b[i];
+      apr += (double)(float)b[i] * (double)(float)zi;
+    }
   }
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (part_pr) {  // p.r = z.b (of the FP32 direction when only that one is kept)
-    const double t = p32only ? block_sum(apr, sh) : s;
+  if (part_pr) {  // p.r = z.b (of the FP32 vectors when the iteration keeps those)
+    const double t = r32 ? block_sum(apr, sh) : s;
     if (threadIdx.x == 0) part_pr[blockIdx.x] = t;
   }
 }
@@ -149,17 +153,18 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int n, const double* __
 // and the vector loads are issued before that gather.  p.q comes from the
 // product's tiles (sc->pq_tiles).  EP / QT: types of the stored partials and
 // of the product's own rows (FP32 after a product with FP32 sums that was
-// asked for FP32 rows; q is then not written back).  PT: type of the search
-// direction (FP32 when the mixed method keeps only its FP32 copy, the vector
-// the product staged, so x += alpha p moves along exactly that vector).
-template <int NB, typename EP, typename QT, typename PT>
+// asked for FP32 rows; q is then not written back).  VT: type of p, r and the
+// Jacobi scale m (FP32 when the mixed iteration keeps its vectors in FP32: p
+// is then the vector the product staged, so x += alpha p moves along exactly
+// that vector; x stays FP64).
+template <int NB, typename EP, typename QT, typename VT>
 __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs mg,
                                                                 const double* __restrict__ part_pr,
                                                                 const double* __restrict__ dslot,
                                                                 double* x,
-                                                                const PT* __restrict__ p,
-                                                                double* r, double* q, int keep_q,
-                                                                const double* __restrict__ m,
+                                                                const VT* __restrict__ p,
+                                                                VT* r, double* q, int keep_q,
+                                                                const VT* __restrict__ m,
                                                                 int jacobi, double* part_rz,
                                                                 double* part_zq, SolveScalars* sc) {
   __shared__ double sh[8];
@@ -177,8 +182,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
       qv[c] = (double)reinterpret_cast<const QT*>(q)[i0 + c];
       xv[c] = x[i0 + c];
       pv[c] = (double)p[i0 + c];
-      rv[c] = r[i0 + c];
-      mv[c] = m[i0 + c];
+      rv[c] = (double)r[i0 + c];
+      mv[c] = (double)m[i0 + c];
     }
     for (int k = k0; k < k1; ++k) {
       const EP* e = reinterpret_cast<const EP*>(mg.epart) + (int64_t)NB * mg.idx[k];
@@ -189,8 +194,9 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
     for (int c = 0; c < NB; ++c) {
       if (sizeof(QT) == 8 && keep_q) q[i0 + c] = qv[c];
       x[i0 + c] = xv[c] + alpha * pv[c];
-      const double ri = rv[c] - alpha * qv[c];
-      r[i0 + c] = ri;
+      const VT rs = (VT)(rv[c] - alpha * qv[c]);
+      r[i0 + c] = rs;
+      const double ri = (double)rs;
       const double zi = ri * mv[c];
       acc += ri * zi;
       azq += zi * qv[c];
@@ -244,32 +250,34 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
                                                            const double* __restrict__ m,
                                                            const double* __restrict__ bm,
                                                            double* part_pr, float* p32,
-                                                           int p32only, SolveScalars* sc) {
+                                                           const float* __restrict__ r32,
+                                                           const float* __restrict__ m32,
+                                                           SolveScalars* sc) {
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
   const double di = grid_sum(part_rz, sh);
   const double beta = part_zq ? -grid_sum(part_zq, sh) / sc->pq : di / *dold;
   double apr = 0.0;
-  if (p32only) {
-    // the direction lives only in FP32 (Jacobi): p = fl32(m r + beta p), and
-    // p.r of the stored values; same four rows per lane as below
+  if (r32) {
+    // the iteration's vectors are FP32 (Jacobi): p = fl32(m r + beta p), in
+    // FP64 arithmetic, and p.r of the stored values; four rows per lane
     const int n4 = n >> 2;
     for (int k = blockIdx.x * kThreads + threadIdx.x; k < n4; k += gridDim.x * kThreads) {
-      const double2* r2 = reinterpret_cast<const double2*>(r) + 2 * k;
-      const double2* m2 = reinterpret_cast<const double2*>(m) + 2 * k;
-      const double2 ra = r2[0], rb = r2[1], ma = m2[0], mb = m2[1];
+      const float4 rf = reinterpret_cast<const float4*>(r32)[k];
+      const float4 mf = reinterpret_cast<const float4*>(m32)[k];
       const float4 pf = reinterpret_cast<const float4*>(p32)[k];
-      const float4 o = make_float4((float)(ra.x * ma.x + beta * (double)pf.x),
-                                   (float)(ra.y * ma.y + beta * (double)pf.y),
-                                   (float)(rb.x * mb.x + beta * (double)pf.z),
-                                   (float)(rb.y * mb.y + beta * (double)pf.w));
+      const float4 o = make_float4((float)((double)rf.x * (double)mf.x + beta * (double)pf.x),
+                                   (float)((double)rf.y * (double)mf.y + beta * (double)pf.y),
+                                   (float)((double)rf.z * (double)mf.z + beta * (double)pf.z),
+                                   (float)((double)rf.w * (double)mf.w + beta * (double)pf.w));
       reinterpret_cast<float4*>(p32)[k] = o;
-      apr += (double)o.x * ra.x + (double)o.y * ra.y + (double)o.z * rb.x + (double)o.w * rb.y;
+      apr += (double)o.x * (double)rf.x + (double)o.y * (double)rf.y +
+             (double)o.z * (double)rf.z + (double)o.w * (double)rf.w;
     }
     for (int i = 4 * n4 + blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-      const float pi = (float)(r[i] * m[i] + beta * (double)p32[i]);
+      const float pi = (float)((double)r32[i] * (double)m32[i] + beta * (double)p32[i]);
       p32[i] = pi;
-      apr += (double)pi * r[i];
+      apr += (double)pi * (double)r32[i];
     }
   } else if (bm == nullptr) {
     // Jacobi: four rows per lane as two 16-byte loads of r, m and p each, so
@@ -333,8 +341,9 @@ __global__ void k_rel_force(SolveScalars* sc) {
 }
 
 // r = b - (t + merged partials); with Jacobi also partial r.(m r); with
-// part_pr also the partial p.r of the replaced residual (p read from p32
-// when the direction lives only in FP32)
+// part_pr also the partial p.r of the replaced residual.  With r32 the
+// iteration's vectors are FP32: the residual is stored there, rounded, and p
+// is read from p32; the norm is formed from the FP64 residual.
 __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
                                                          const double* __restrict__ t,
                                                          const double* __restrict__ b,
@@ -342,7 +351,8 @@ __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
                                                          int jacobi, double* part,
                                                          const double* __restrict__ p,
                                                          const float* __restrict__ p32,
-                                                         double* part_pr, const SolveScalars* sc) {
+                                                         float* r32, double* part_pr,
+                                                         const SolveScalars* sc) {
   __shared__ double sh[8];
   if (!block_flag(&sc->upd, sh) || block_flag(&sc->done, sh)) return;
   double acc = 0.0, apr = 0.0;
@@ -350,9 +360,15 @@ __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
     double s = t[i];
     s = merge_row(mg, i, s);
     const double ri = b[i] - s;
-    r[i] = ri;
     acc += ri * (ri * m[i]);
-    if (part_pr) apr += (p32 ? (double)p32[i] : p[i]) * ri;
+    if (r32) {
+      const float rs = (float)ri;
+      r32[i] = rs;
+      if (part_pr) apr += (double)p32[i] * (double)rs;
+    } else {
+      r[i] = ri;
+      if (part_pr) apr += p[i] * ri;
+    }
   }
   if (jacobi) {
     const double s = block_sum(acc, sh);
@@ -405,11 +421,16 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   double* pr_next = tdots ? part_pr : nullptr;
   // FP32 copy of p for the FP32-sum operator's staging (node-blocked layouts)
   float* p32 = (tdots && fp32_operator && c->accum32) ? c->p32 : nullptr;
-  // with Jacobi and the FP32-sum operator the direction lives only in p32
-  // (the FP64 p is filled from it if the progress guard hands over to FP64)
-  bool p32only = p32 != nullptr && jac && reliable && epart_f32(c, true);
+  // With Jacobi and the FP32-sum operator the iteration keeps p, r and the
+  // Jacobi scale in FP32 (x, the true residuals and all sums stay FP64).  If
+  // the progress guard hands over to the FP64 operator, p and r are copied to
+  // their FP64 vectors and the iteration continues there.
+  bool v32 = p32 != nullptr && jac && reliable && epart_f32(c, true);
+  float* r32 = v32 ? p32 + vec32_stride(n) : nullptr;
+  float* m32 = v32 ? p32 + 2 * vec32_stride(n) : nullptr;
+  if (v32) launch_cast<double, float>(n, V.m, m32, s);
   hipLaunchKernelGGL(k_pcg_init, G, TB, 0, s, n, V.b, V.m, V.bm, V.x, V.r, V.p, part_a, pr_next,
-                     p32, p32only ? 1 : 0);
+                     p32, r32);
   hipLaunchKernelGGL(k_pcg_init2, dim3(1), TB, 0, s, part_a, tol, delta, reliable ? 1 : 0, dslots,
                      sc);
   auto replace = [&](int iters_so_far) {
@@ -417,7 +438,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
     launch_spmv(c, false, V.x, V.t, V.epart, &sc->skip64);
     hipLaunchKernelGGL(k_rel_resid, G, TB, 0, s, n, merge_of(c, V.epart),
                        V.t, V.b, V.m, V.r, jac, part_b, (const double*)V.p,
-                       (const float*)(p32only ? p32 : nullptr), pr_next, sc);
+                       (const float*)(v32 ? p32 : nullptr), v32 ? r32 : nullptr, pr_next, sc);
     if (!jac)
       hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)nullptr,
                          (double*)nullptr, &sc->upd, sc);
@@ -463,20 +484,20 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
         if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j + 1], s));
         // q is read again only by the block preconditioner's (M^-1 r).q
         const MergeArgs mg = merge_of(c, V.epart, e32);
-#define ELASTIC_UPD(NB, EP, QT, PT, P)                                                         \
-  hipLaunchKernelGGL((k_pcg_update_nodes<NB, EP, QT, PT>), G, TB, 0, s, n / NB, mg,            \
-                     (const double*)part_pr, dcur, V.x, (const PT*)(P), V.r, V.q, keep_q, V.m,  \
-                     jac, part_b, part_zq, sc)
+#define ELASTIC_UPD(NB, EP, QT, VT, P, R, M)                                                   \
+  hipLaunchKernelGGL((k_pcg_update_nodes<NB, EP, QT, VT>), G, TB, 0, s, n / NB, mg,            \
+                     (const double*)part_pr, dcur, V.x, (const VT*)(P), (VT*)(R), V.q, keep_q,  \
+                     (const VT*)(M), jac, part_b, part_zq, sc)
         if (c->layout_nb == 3) {
-          if (q32 && p32only) ELASTIC_UPD(3, float, float, float, p32);
-          else if (q32) ELASTIC_UPD(3, float, float, double, V.p);
-          else if (e32) ELASTIC_UPD(3, float, double, double, V.p);
-          else ELASTIC_UPD(3, double, double, double, V.p);
+          if (q32 && v32) ELASTIC_UPD(3, float, float, float, p32, r32, m32);
+          else if (q32) ELASTIC_UPD(3, float, float, double, V.p, V.r, V.m);
+          else if (e32) ELASTIC_UPD(3, float, double, double, V.p, V.r, V.m);
+          else ELASTIC_UPD(3, double, double, double, V.p, V.r, V.m);
         } else {
-          if (q32 && p32only) ELASTIC_UPD(2, float, float, float, p32);
-          else if (q32) ELASTIC_UPD(2, float, float, double, V.p);
-          else if (e32) ELASTIC_UPD(2, float, double, double, V.p);
-          else ELASTIC_UPD(2, double, double, double, V.p);
+          if (q32 && v32) ELASTIC_UPD(2, float, float, float, p32, r32, m32);
+          else if (q32) ELASTIC_UPD(2, float, float, double, V.p, V.r, V.m);
+          else if (e32) ELASTIC_UPD(2, float, double, double, V.p, V.r, V.m);
+          else ELASTIC_UPD(2, double, double, double, V.p, V.r, V.m);
         }
 #undef ELASTIC_UPD
       } else {
@@ -492,7 +513,8 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
         hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)V.q,
                            part_zq, (const int32_t*)nullptr, sc);
       hipLaunchKernelGGL(k_pcg_pupdate, G, TB, 0, s, n, part_b, (const double*)part_zq, dcur, dnxt,
-                         V.p, V.r, V.m, V.bm, pr_next, p32, p32only ? 1 : 0, sc);
+                         V.p, V.r, V.m, V.bm, pr_next, p32, (const float*)r32,
+                         (const float*)m32, sc);
       // the replacement kernels gate themselves on the flag set by k_pcg_pupdate
       if (reliable && gate && ((it + 1) & 1) == 0) replace(it + 1);
     }
@@ -514,9 +536,11 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
     }
     if (c->scal_host->op64 && op32) {
       op32 = false;
-      if (p32only) {  // the FP64 operator reads the FP64 direction from here on
+      if (v32) {  // the FP64 operator and kernels read the FP64 vectors from here on
         launch_cast<float, double>(n, p32, V.p, s);
-        p32only = false;
+        launch_cast<float, double>(n, r32, V.r, s);
+        v32 = false;
+        r32 = m32 = nullptr;
       }
     }
     if (predict) {
