@@ -44,6 +44,7 @@ struct BInst {
   double* epart;
   double* tdot;   // ntiles shares of p.q written by the product's tiles
   double *x, *r, *p, *q, *b, *t;
+  float* p32;     // FP32 copy of p, staged by the FP32-sum product (else null)
   const double* m;
   double* part;   // 5 * nch partials: (unused), r.Mr, true-residual r.Mr, p.r, (Mr).q
   double* dslot;  // 2 ping-pong slots of r.Mr
@@ -114,8 +115,9 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
     const bool run = f64 ? !sc->done : (!sc->done && !sc->op64);
     if (!run) return;
   }
-  if (MODE == 0)
-    compute_tile<TV, TA, NB, PER, true>(I.a, t, R, I.p, I.q, I.epart, smem, I.tdot);
+  if (MODE == 0)  // the FP32-sum variant stages x from the FP32 copy of p
+    compute_tile<TV, TA, NB, PER, true>(I.a, t, R, I.p, I.q, I.epart, smem, I.tdot, 0,
+                                        sizeof(TA) == 4 ? I.p32 : nullptr);
   else
     compute_tile<TV, TA, NB, PER>(I.a, t, R, I.x, I.t, I.epart, smem);
 }
@@ -282,6 +284,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
       const double ri = I.r[i];
       const double pi = ri * I.m[i] + beta * I.p[i];
       I.p[i] = pi;
+      if (I.p32) I.p32[i] = (float)pi;
       apr += pi * ri;
     }
     if (I.fp32) {  // flexible recurrences of the mixed method
@@ -382,6 +385,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_init(const BInst* __restrict__ 
     I.x[i] = 0.0;
     I.r[i] = bi;
     I.p[i] = zi;
+    if (I.p32) I.p32[i] = (float)zi;
     acc += bi * zi;
   }
   const double s = block_sum(acc, sh);
@@ -733,6 +737,8 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   I.q = c->v64[VQ];
   I.b = c->v64[VB];
   I.t = c->v64[VT];
+  // the FP32 copy of p only for instances whose products sum in FP32
+  I.p32 = (precision == ELASTIC_MIXED && c->accum32) ? c->p32 : nullptr;
   I.m = c->A64.m;
   I.part = c->red64;
   I.dslot = c->red64 + 8 * kRedBlocks + 8;
