@@ -44,7 +44,12 @@ struct BInst {
   double* epart;
   double* tdot;   // ntiles shares of p.q written by the product's tiles
   double *x, *r, *p, *q, *b, *t;
-  float* p32;     // FP32 copy of p, staged by the FP32-sum product (else null)
+  // Instances whose products sum in FP32 keep p, r and the Jacobi scale as
+  // FP32 vectors while they use the FP32 operator (x, the true residuals and
+  // all sums stay FP64); null otherwise.  Every true-residual replacement also
+  // refreshes the FP64 r and p, so a hand-over to the FP64 operator (which
+  // only happens there) continues from current FP64 vectors.
+  float *p32, *r32, *m32;
   const double* m;
   double* part;   // 5 * nch partials: (unused), r.Mr, true-residual r.Mr, p.r, (Mr).q
   double* dslot;  // 2 ping-pong slots of r.Mr
@@ -193,6 +198,11 @@ __global__ __launch_bounds__(kThreads) void k_bt_sum_shares(const BInst* __restr
 // (FP32 after a product with FP32 sums).
 template <int NB, typename EP>
 __device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, double* sh) {
+  // after a product with FP32 sums the iteration's p, r and m are FP32 too
+  typedef EP VT;
+  const VT* P = sizeof(VT) == 4 ? (const VT*)I.p32 : (const VT*)I.p;
+  VT* R = sizeof(VT) == 4 ? (VT*)I.r32 : (VT*)I.r;
+  const VT* M = sizeof(VT) == 4 ? (const VT*)I.m32 : (const VT*)I.m;
   const int J0 = c * (I.cr / NB), J1 = min(I.n / NB, J0 + I.cr / NB);
   double acc = 0.0, azq = 0.0;
   for (int J = J0 + threadIdx.x; J < J1; J += kThreads) {
@@ -203,9 +213,9 @@ __device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, doub
     for (int r = 0; r < NB; ++r) {
       qv[r] = I.q[i0 + r];
       xv[r] = I.x[i0 + r];
-      pv[r] = I.p[i0 + r];
-      rv[r] = I.r[i0 + r];
-      mv[r] = I.m[i0 + r];
+      pv[r] = (double)P[i0 + r];
+      rv[r] = (double)R[i0 + r];
+      mv[r] = (double)M[i0 + r];
     }
     for (int k = k0; k < k1; ++k) {
       const EP* e = reinterpret_cast<const EP*>(I.epart) + (int64_t)NB * I.mg.idx[k];
@@ -215,8 +225,9 @@ __device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, doub
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
       I.x[i0 + r] = xv[r] + alpha * pv[r];
-      const double ri = rv[r] - alpha * qv[r];
-      I.r[i0 + r] = ri;
+      const VT rs = (VT)(rv[r] - alpha * qv[r]);
+      R[i0 + r] = rs;
+      const double ri = (double)rs;
       const double zi = ri * mv[r];
       acc += ri * zi;
       azq += zi * qv[r];
@@ -280,12 +291,20 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
         I.fp32 ? -inst_sum(I.part + 4 * I.nch, I.nch, sh) / I.sc->pq : di / I.dslot[cur];
     const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
     double apr = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
-      const double ri = I.r[i];
-      const double pi = ri * I.m[i] + beta * I.p[i];
-      I.p[i] = pi;
-      if (I.p32) I.p32[i] = (float)pi;
-      apr += pi * ri;
+    if (I.p32 && !I.sc->op64) {  // FP32 vectors (see BInst)
+      for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
+        const double ri = (double)I.r32[i];
+        const float pi = (float)(ri * (double)I.m32[i] + beta * (double)I.p32[i]);
+        I.p32[i] = pi;
+        apr += (double)pi * ri;
+      }
+    } else {
+      for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
+        const double ri = I.r[i];
+        const double pi = ri * I.m[i] + beta * I.p[i];
+        I.p[i] = pi;
+        apr += pi * ri;
+      }
     }
     if (I.fp32) {  // flexible recurrences of the mixed method
       const double t = block_sum(apr, sh);
@@ -327,6 +346,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_resid(const BInst* __restri
     const int c = wl_index(e);
     if (uniform_done(I, c, sh, true)) return;
     const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
+    const bool v32 = I.p32 && !I.sc->op64;
     double acc = 0.0, apr = 0.0;
     for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
       double s = I.t[i];
@@ -334,7 +354,15 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_resid(const BInst* __restri
       const double ri = I.b[i] - s;
       I.r[i] = ri;
       acc += ri * (ri * I.m[i]);
-      apr += I.p[i] * ri;
+      if (v32) {  // FP32 vectors: store the rounded residual; refresh the FP64 p
+        const float rs = (float)ri;
+        I.r32[i] = rs;
+        const double pi = (double)I.p32[i];
+        I.p[i] = pi;
+        apr += pi * (double)rs;
+      } else {
+        apr += I.p[i] * ri;
+      }
     }
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) I.part[2 * I.nch + c] = s;
@@ -379,19 +407,25 @@ __global__ __launch_bounds__(kThreads) void k_bt_init(const BInst* __restrict__ 
   const BInst& I = insts[slot];
   const int c = blockIdx.x;
   const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
-  double acc = 0.0;
+  double acc = 0.0, apr = 0.0;
   for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
     const double bi = I.b[i], zi = bi * I.m[i];
     I.x[i] = 0.0;
     I.r[i] = bi;
     I.p[i] = zi;
-    if (I.p32) I.p32[i] = (float)zi;
     acc += bi * zi;
+    if (I.p32) {  // FP32 vectors (see BInst)
+      I.p32[i] = (float)zi;
+      I.r32[i] = (float)bi;
+      I.m32[i] = (float)I.m[i];
+      apr += (double)(float)bi * (double)(float)zi;
+    }
   }
   const double s = block_sum(acc, sh);
+  const double t = I.p32 ? block_sum(apr, sh) : s;
   if (threadIdx.x == 0) {
     I.part[I.nch + c] = s;
-    I.part[3 * I.nch + c] = s;
+    I.part[3 * I.nch + c] = t;
   }
 }
 
@@ -737,8 +771,11 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   I.q = c->v64[VQ];
   I.b = c->v64[VB];
   I.t = c->v64[VT];
-  // the FP32 copy of p only for instances whose products sum in FP32
-  I.p32 = (precision == ELASTIC_MIXED && c->accum32) ? c->p32 : nullptr;
+  // FP32 vectors only for instances whose products sum in FP32
+  const bool v32 = precision == ELASTIC_MIXED && c->accum32;
+  I.p32 = v32 ? c->p32 : nullptr;
+  I.r32 = v32 ? c->p32 + vec32_stride(c->neq) : nullptr;
+  I.m32 = v32 ? c->p32 + 2 * vec32_stride(c->neq) : nullptr;
   I.m = c->A64.m;
   I.part = c->red64;
   I.dslot = c->red64 + 8 * kRedBlocks + 8;
