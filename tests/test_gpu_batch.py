@@ -195,3 +195,33 @@ def test_batch_fp32_accumulation_instances(golden):
         assert np.array_equal(xa, x) and sa["iters"] == st["iters"]
     for c in ctxs:
         c.close()
+
+
+def test_batch_hand_over_to_fp64(monkeypatch):
+    """The progress guard's hand-over inside the batch, forced from the host
+    (replacement threshold 2: true residuals every second iteration, each
+    falling by less than the guard's 10%) on hex8 12^3 systems: instances with
+    FP32 sums (whose FP32 p and r the replacements mirror in FP64) and with
+    FP64 sums switch to the FP64 operator early and still reach the tight
+    FP64 solution within 1e-8; alone and in the mix stay bit-identical."""
+    from elastic_amd import mesh as M
+    from elastic_amd.solver import FP64
+    from elastic_amd.system import System
+
+    systems = [System(M.hex8_cube(12, jitter=0.15, seed=M.BASE_SEED + k)) for k in (2, 3, 4)]
+    refs = [s.ctx.solve(s.b, precision=FP64, tol=1e-12)[0] for s in systems]
+    monkeypatch.setenv("ELASTIC_RELIABLE_DELTA", "2.0")
+    ctxs = [s.ctx for s in systems]
+    for c, s in zip(ctxs, systems):
+        c.upload_rhs(s.b)
+    ctxs[0].set_option("accum32", 1)
+    ctxs[2].set_option("accum32", 1)
+    mix = run_batch(ctxs, MIXED, 1e-10)
+    for i in range(len(ctxs)):
+        x, st = mix[i]
+        assert st["status"] == 0 and 0 <= st["op64_from"] <= 20
+        assert rel(x, refs[i]) <= 1e-8
+        xa, sa = run_batch([ctxs[i]], MIXED, 1e-10)[0]
+        assert np.array_equal(xa, x) and sa["iters"] == st["iters"]
+    for c in ctxs:
+        c.close()

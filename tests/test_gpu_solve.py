@@ -266,3 +266,28 @@ def test_product_timing_option(golden):
         assert n == (s1["iters"] + 3) // 4 and ms > 0.0
         assert c.product_timing() == (0.0, 0)
         c.set_option("time_products", 0)
+
+
+@pytest.mark.parametrize("accum32", [0, 1])
+def test_mixed_hands_over_to_fp64(monkeypatch, accum32):
+    """The progress guard's hand-over, forced from the host on hex8 12^3: with
+    the replacement threshold at 2 every second iteration replaces the
+    residual, which then falls by less than the guard's 10% per replacement,
+    so after three replacements the solve continues with the FP64 operator
+    (with FP32 sums the iteration's FP32 p and r are copied to their FP64
+    vectors first).  It still converges to the tight FP64 solution within
+    1e-8."""
+    from elastic_amd import mesh as M
+    from elastic_amd.solver import FP64, MIXED
+    from elastic_amd.system import System
+
+    s = System(M.hex8_cube(12, jitter=0.15, seed=M.BASE_SEED + 2))
+    c = s.ctx
+    xref, _ = c.solve(s.b, precision=FP64, tol=1e-12)
+    monkeypatch.setenv("ELASTIC_RELIABLE_DELTA", "2.0")
+    c.set_option("accum32", accum32)
+    x, st = c.solve(s.b, precision=MIXED, tol=1e-10)
+    assert 0 <= st["op64_from"] <= 20
+    assert st["norm_m_r"] < st["conv"]
+    assert rel(x, xref) <= 1e-8
+    c.close()
