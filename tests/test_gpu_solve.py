@@ -291,3 +291,29 @@ def test_mixed_hands_over_to_fp64(monkeypatch, accum32):
     assert st["norm_m_r"] < st["conv"]
     assert rel(x, xref) <= 1e-8
     c.close()
+
+
+@pytest.mark.parametrize("accum32", [0, 1])
+def test_rows_not_a_multiple_of_four(accum32):
+    """hex8 6^3 has 882 equations (882 = 4 * 220 + 2): the p updates handle
+    four rows per lane and the last two rows in a tail loop, in FP64 and with
+    the mixed iteration's FP32 vectors.  FP64 PCG matches the oracle's
+    iteration count within 2 and its x within 1e-10; mixed PCG lands within
+    1e-8 of the FP64 x."""
+    from elastic_amd import mesh as M
+    from elastic_amd.solver import FP64, MIXED
+    from elastic_amd.system import System
+    from oracle import binding as O
+
+    m = M.hex8_cube(6, jitter=0.15, seed=M.BASE_SEED + 2)
+    s = System(m)
+    assert s.neq == 882
+    ref = O.System(m)
+    xo, so = O.pcg(ref.ia, ref.ja, ref.ad, ref.al, ref.b, tol=1e-10, maxit=5000)
+    c = s.ctx
+    c.set_option("accum32", accum32)
+    x64, st64 = c.solve(s.b, precision=FP64, tol=1e-10)
+    assert abs(st64["iters"] - so["iters"]) <= 2 and rel(x64, xo) <= 1e-10
+    xm, stm = c.solve(s.b, precision=MIXED, tol=1e-10)
+    assert stm["norm_m_r"] < stm["conv"] and rel(xm, x64) <= 1e-8
+    c.close()
