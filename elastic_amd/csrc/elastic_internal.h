@@ -154,6 +154,9 @@ struct Context {
   int tile_blocks = 0;         // node blocks per tile; 0: by node size (kTileBlocks3/2)
   int layout_nb = 1;           // 1: scalar tiles; 2 or 3: node-blocked tiles
   bool accum32 = false;        // FP32 operator: FP32 staging and sums (else FP64)
+  // -1: accum32 follows the system (on for 3-dof layouts of at least
+  // kLargeTileNodes nodes, where FP32 sums pay); else set by the caller
+  int accum32_set = -1;
   bool allow_blocks = true;
   TileLayout layout;
   DevTiles tiles;

@@ -215,7 +215,7 @@ def load_paths(n: int = 60, npaths: int = 32, stretch: float = 5e-3, H: float = 
 
 def main(argv=None) -> int:
     """python -m elastic_amd.plasticity [--n 60] [--paths 32] [--increments 10]
-    [--mode batched|sequential] [--precision mixed|fp64] [--sums fp64|fp32]: runs the load paths
+    [--mode batched|sequential] [--precision mixed|fp64] [--sums auto|fp64|fp32]: runs the load paths
     and prints one JSON line (wall time, Newton iterations, solved DOF/s)."""
     import argparse
     import json
@@ -231,14 +231,16 @@ def main(argv=None) -> int:
     ap.add_argument("--mode", choices=["batched", "sequential"], default="batched")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--rtol", type=float, default=1e-8)
-    ap.add_argument("--sums", choices=["fp64", "fp32"], default="fp64",
-                    help="sums of the FP32 operator in mixed-precision solves")
+    ap.add_argument("--sums", choices=["auto", "fp64", "fp32"], default="auto",
+                    help="sums of the FP32 operator in mixed-precision solves (auto: the "
+                         "library's choice, FP32 for 3-dof systems of >= 32,768 nodes)")
     a = ap.parse_args(argv)
     prec = MIXED if a.precision == "mixed" else FP64
     t0 = time.perf_counter()
     paths, sys_y = load_paths(a.n, a.paths, a.stretch)
-    for p in paths:
-        p.ctx.set_option("accum32", 1 if a.sums == "fp32" else 0)
+    if a.sums != "auto":
+        for p in paths:
+            p.ctx.set_option("accum32", 1 if a.sums == "fp32" else 0)
     t_setup = time.perf_counter() - t0
     neq = paths[0].sys.neq
     t1 = time.perf_counter()

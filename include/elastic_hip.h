@@ -227,9 +227,11 @@ int elastic_set_tile_nnz(elastic_ctx* ctx, int32_t tile_nnz);
 int elastic_set_tile_blocks(elastic_ctx* ctx, int32_t tile_blocks);
 /* Tuning knobs (no counterpart in the reference): "tile_blocks" /
  * "tile_nnz" (tile size; rebuilds the layout at once when a structure is
- * set), "accum32" (1: the FP32 operator stages x and sums in FP32, faster
- * but less accurate, which ill-conditioned systems may not tolerate; 0, the
- * default: in FP64), "check_every" (iterations between convergence polls),
+ * set), "accum32" (1: the FP32 operator stages x and sums in FP32, and the
+ * mixed PCG with Jacobi keeps p, r and the Jacobi scale in FP32, faster but
+ * less accurate, which ill-conditioned systems may not tolerate; 0: in
+ * FP64; until set, 1 for 3-dof systems of at least 32,768 nodes, else 0),
+ * "check_every" (iterations between convergence polls),
  * "time_products" (S > 0: PCG brackets the product of every S-th iteration
  * with HIP events on its stream; 0, the default: off). */
 int elastic_set_option(elastic_ctx* ctx, const char* name, int64_t value);
