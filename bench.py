@@ -181,8 +181,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp32 matrix values, %s sums in the operator, fp64 vectors and "
-                  "true-residual replacement" % ("fp32" if acc32 else "fp64")),
+        "dtype": ("fp32 matrix values, fp32 sums in the operator, fp32 p/r/Jacobi scale in "
+                  "the iteration; fp64 x, dot products and true-residual replacement"
+                  if acc32 else
+                  "fp32 matrix values, fp64 sums in the operator, fp64 vectors, dot products "
+                  "and true-residual replacement"),
         "data": "synthetic (generated mesh, GPU-assembled system)",
         "config": {"workload": "hex8 100^3 jittered 0.15h, E=1 nu=0.3, Jacobi PCG to 1e-10 "
                                "(%d DOF, nad %d)" % (neq, nad),
