@@ -633,6 +633,16 @@ static int solve_other(Context* c, int solver, int precond, int precision, doubl
   ELASTIC_HIP_CHECK(c, hipEventRecord(c->ev0, s));
   KrylovResult kr;
   int rc;
+  // The automatic FP32 sums are PCG's: BiCGSTAB took 3,195 iterations with
+  // them against 472 without on hex8 40^3, so these methods sum in FP64
+  // unless the caller set the option.
+  const bool acc_saved = c->accum32;
+  if (c->accum32_set < 0) c->accum32 = false;
+  struct Restore {
+    Context* c;
+    bool v;
+    ~Restore() { c->accum32 = v; }
+  } restore{c, acc_saved};
   if (solver == ELASTIC_SOLVER_BICGSTAB) {
     double delta = 0.1;
     if (const char* e = getenv("ELASTIC_RELIABLE_DELTA")) delta = atof(e);

@@ -230,7 +230,8 @@ int elastic_set_tile_blocks(elastic_ctx* ctx, int32_t tile_blocks);
  * set), "accum32" (1: the FP32 operator stages x and sums in FP32, and the
  * mixed PCG with Jacobi keeps p, r and the Jacobi scale in FP32, faster but
  * less accurate, which ill-conditioned systems may not tolerate; 0: in
- * FP64; until set, 1 for 3-dof systems of at least 32,768 nodes, else 0),
+ * FP64; until set, 1 for PCG on 3-dof systems of at least 32,768 nodes,
+ * else 0),
  * "check_every" (iterations between convergence polls),
  * "time_products" (S > 0: PCG brackets the product of every S-th iteration
  * with HIP events on its stream; 0, the default: off). */

@@ -81,3 +81,33 @@ def test_tet4_80_all_methods():
         print("tet4_80 gmres(50)", prec, st["iters"], st["rel_res"], rel(x, xref), flush=True)
         assert np.isfinite(st["rel_res"]) and st["rel_res"] < 1e-2
     c.close()
+
+
+def test_large_system_default_sums_all_methods():
+    """hex8 40^3 (68,921 nodes) is above the size from which the mixed PCG
+    sums its operator in FP32 by default (BiCGSTAB and GMRES keep FP64 sums
+    unless asked); PCG, BiCGSTAB and GMRES(50) in mixed precision land
+    within 1e-8 of the tight FP64 PCG solution with the default and with
+    FP64 sums set explicitly, and the default BiCGSTAB takes the iterations
+    of the FP64-sum one."""
+    from elastic_amd import mesh as M
+    from elastic_amd.solver import BICGSTAB, GMRES
+    from elastic_amd.system import System
+
+    s = System(M.hex8_cube(40, jitter=0.15, seed=M.BASE_SEED + 2))
+    c = s.ctx
+    c.upload_rhs(s.b)
+    c.solve_resident(precision=FP64, tol=1e-12, maxit=100000)
+    xref = c.download_x()
+    its = {}
+    for acc in (None, 0):
+        if acc is not None:
+            c.set_option("accum32", acc)
+        for solver, kw in ((PCG, {}), (BICGSTAB, {}), (GMRES, {"ngram": 50})):
+            st = c.solve_resident(solver=solver, precision=MIXED, tol=1e-10, maxit=20000, **kw)
+            x = c.download_x()
+            print("hex8_40 sums", acc, "solver", solver, st["iters"], rel(x, xref), flush=True)
+            assert st["status"] == 0 and rel(x, xref) <= 1e-8
+            its[acc, solver] = st["iters"]
+    assert its[None, BICGSTAB] == its[0, BICGSTAB]
+    c.close()
