@@ -77,7 +77,8 @@ class PlasticPath:
 
 def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int = 60,
               precision: int = MIXED, tol: float = 1e-11, batch=None, maxit: int = 50000,
-              counts: dict | None = None, max_halvings: int = 6, last: int | None = None):
+              counts: dict | None = None, max_halvings: int = 6, last: int | None = None,
+              progress=None):
     """Drives every path through `increments` equal load steps.  Per Newton
     iteration each unconverged path re-assembles K_t and R = -f_int on the
     GPU; the systems K_t du = R are solved together in `batch` (a
@@ -91,7 +92,8 @@ def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int =
     next under a large load step; the halving restores descent.
     Returns per path the list of (increment, assembly, ||R||); `counts`
     (optional dict) receives the numbers of linear solves and of halvings.
-    `last` stops after that increment (the load still split in `increments`)."""
+    `last` stops after that increment (the load still split in `increments`).
+    `progress` (a file, optional) receives one line per finished increment."""
     hist = [[] for _ in paths]
     nsolve = nhalf = 0
     for inc in range(1, (last or increments) + 1):
@@ -143,6 +145,9 @@ def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int =
             nsolve += len(need)
             active = need + retry
             it += 1
+        if progress is not None:
+            print("increment %d done: %d assemblies, %d linear solves so far"
+                  % (inc, it, nsolve), file=progress, flush=True)
     if counts is not None:
         counts["linear_solves"] = nsolve
         counts["halvings"] = nhalf
@@ -219,6 +224,7 @@ def main(argv=None) -> int:
     and prints one JSON line (wall time, Newton iterations, solved DOF/s)."""
     import argparse
     import json
+    import sys
     import time
 
     from .serving import Batch
@@ -248,10 +254,10 @@ def main(argv=None) -> int:
     if a.mode == "batched":
         with Batch(0, max_slots=max(1, min(a.paths, 255)), iters_per_step=16) as b:
             hist = run_paths(paths, increments=a.increments, rtol=a.rtol, precision=prec, batch=b,
-                             counts=counts)
+                             counts=counts, progress=sys.stderr)
     else:
         hist = run_paths(paths, increments=a.increments, rtol=a.rtol, precision=prec,
-                         counts=counts)
+                         counts=counts, progress=sys.stderr)
     wall = time.perf_counter() - t1
     solves = counts["linear_solves"]
     newton = [max(it for (inc, it, _) in h if inc == k) for h in hist
