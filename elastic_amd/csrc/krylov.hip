@@ -13,6 +13,9 @@ namespace elastic {
 template <typename T>
 __global__ void k_bdiag(int nb, const T* __restrict__ ad, const T* __restrict__ al,
                         const int64_t* __restrict__ slots, T* __restrict__ bm) {
+  // every product and difference rounded on its own, as the reference's
+  // x86-64 build evaluates them: the inverse is bit-identical to block_precond's
+#pragma clang fp contract(off)
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
     const int ii = 3 * i;
     const double a11 = ad[ii], a22 = ad[ii + 1], a33 = ad[ii + 2];
@@ -69,6 +72,36 @@ int ensure_bdiag(elastic_ctx* x) {
   c->bdiag_set = true;
   return ELASTIC_OK;
 }
+
+}  // namespace elastic
+
+// m(6, neq/3 + mod(neq,3)) of block_precond (Precond.f:163-358): the device's
+// block inverse, then one 1/ad entry per remaining equation.
+extern "C" int elastic_get_bdiag(elastic_ctx* x, double* m) {
+  using namespace elastic;
+  if (!x || !m) return ELASTIC_BAD_ARG;
+  Context* c = &x->c;
+  if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
+  hipSetDevice(c->device);
+  int rc = ensure_bdiag(x);
+  if (rc) return rc;
+  const int nb = c->neq / 3, rest = c->neq % 3;
+  std::vector<double> inv(rest > 0 ? rest : 1);
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(m, c->A64.bm, sizeof(double) * 6 * (size_t)nb,
+                                      hipMemcpyDeviceToHost, c->stream));
+  if (rest)
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(inv.data(), c->A64.m + 3 * (size_t)nb,
+                                        sizeof(double) * rest, hipMemcpyDeviceToHost, c->stream));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < rest; ++i) {
+    double* o = m + 6 * ((size_t)nb + i);
+    o[0] = inv[i];
+    o[1] = o[2] = o[3] = o[4] = o[5] = 0.0;
+  }
+  return ELASTIC_OK;
+}
+
+namespace elastic {
 
 // ================================================================ BiCGSTAB
 // Iters.f:1079-1349: right-preconditioned BiCGSTAB from x0 = 0 with shadow
@@ -608,8 +641,12 @@ int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit,
         conv = true;
         break;
       }
-      stall = hs->gm_h > 0.9 * last_true ? stall + 1 : 0;
-      last_true = hs->gm_h;
+      if (hs->gm_h > 0.9 * last_true) {  // no progress on the best residual so far
+        ++stall;
+      } else {
+        stall = 0;
+        last_true = hs->gm_h;
+      }
       if (op32 && stall >= 2) {
         op32 = false;
         out->op64_from = nit;
@@ -639,6 +676,17 @@ int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit,
     hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), 0, s, ni, k, a);
     hipLaunchKernelGGL(k_gm_xupdate, G, TB, 0, s, n, ni, c->gm_basis, (const double*)a.y, x);
     if (!fp32_operator && hs->gm_e <= econv) conv = true;
+  }
+  if (fp32_operator && !conv && cycles > 0) {
+    // the cycle limit came before the next cycle's FP64 test: test the last
+    // update once on the FP64 residual
+    launch_spmv(c, false, x, t, c->epart64);
+    hipLaunchKernelGGL(k_gm_resid, G, TB, 0, s, n, merge_of(c, c->epart64), t, b, u);
+    hipLaunchKernelGGL(k_gm_applym_dot, G, TB, 0, s, n, u, m, bm, g(1), P[0]);
+    hipLaunchKernelGGL(k_gm_norm0, dim3(1), TB, 0, s, P[0], a, sc);
+    rc = read_scalars();
+    if (rc) return rc;
+    conv = hs->gm_h <= econv;
   }
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(s));
   ELASTIC_HIP_CHECK(c, hipGetLastError());

@@ -68,6 +68,13 @@ class Context:
         self._chk(self._L.elastic_get_values(self._h, ad, al))
         return ad, al[: self.nad]
 
+    def get_bdiag(self) -> np.ndarray:
+        """m(6, neq/3 + mod(neq,3)) of the 3x3 block-Jacobi preconditioner,
+        in block_precond's layout (Precond.f:163-358), flattened."""
+        m = np.zeros(6 * (self.neq // 3 + self.neq % 3))
+        self._chk(self._L.elastic_get_bdiag(self._h, m))
+        return m
+
     def layout_info(self):
         info = np.zeros(4, np.int64)
         self._chk(self._L.elastic_layout_info(self._h, info))
@@ -149,8 +156,13 @@ def solv(neq, nad, ia, ja, ad, al, m, b, x, tol=1e-11, maxit=10000, ngram=50, un
     if _default_ctx is None:
         _default_ctx = Context(0)
     c = _default_ctx
-    if c.neq != neq or c.nad != nad:
+    ia = np.ascontiguousarray(ia, np.int32)
+    ja = np.ascontiguousarray(ja[:nad], np.int32)
+    # the structure is rebuilt whenever its contents change, not only its sizes
+    key = getattr(c, "_solv_pattern", None)
+    if key is None or not (np.array_equal(key[0], ia) and np.array_equal(key[1], ja)):
         c.set_structure(ia, ja)
+        c._solv_pattern = (ia.copy(), ja.copy())
     c.set_values(ad, al)
     m[:] = 1.0 / ad
     xs, st = c.solve(b, solver=solver, precond=precond, precision=precision, tol=tol, maxit=maxit,

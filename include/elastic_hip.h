@@ -174,6 +174,14 @@ int elastic_j2_get_residual(elastic_ctx* ctx, double* r);
  * either pointer may be NULL. */
 int elastic_j2_get_state(elastic_ctx* ctx, int32_t trial, double* ep, double* alpha);
 
+/* 3x3 block-Jacobi set-up of preconditioner code 6, block_precond with
+ * nblock = 3 (Precond.f:163-358, cofactor inverse Precond.f:1676-1710): fills
+ * m(6, neq/3 + mod(neq,3)) in the reference's layout (c11 c22 c33 c21 c31
+ * c32 of each equation triple over the determinant, then 1/ad for each of
+ * the last mod(neq,3) equations), bit-identical to the reference's, from
+ * the device values that the solves apply. */
+int elastic_get_bdiag(elastic_ctx* ctx, double* m);
+
 /* ----------------------------------------------------------- solving */
 /* Solve A x = b.  Replaces solv -> pcg (Solv.f:13-139, Iters.f:252-524) for
  * solver 1, gmres (Iters.f:528-808) for 2 and pbicgstab (Iters.f:1079-1349)
@@ -194,7 +202,7 @@ int elastic_solve(elastic_ctx* ctx, int32_t solver, int32_t precond, int32_t pre
  * double, logical = int32, pool offsets = int64).  Behaviour: m is
  * overwritten with 1/ad (Solv.f:65-69); x receives the solution; the MPI
  * arguments are accepted and ignored (one process).  Uses one internal context
- * per device-0 process, rebuilt when neq/nad or the ia/ja pointers change.
+ * per device-0 process, rebuilt when neq, nad or any entry of ip/ja changes.
  * Returns the status (a Fortran caller may ignore it or declare a function). */
 int32_t elastic_solv_(int32_t* neq, int32_t* nad, int32_t* ip, int32_t* ja, double* ad,
                       double* au, double* al, double* m, double* b, double* x, double* tol,

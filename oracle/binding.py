@@ -72,6 +72,7 @@ def lib():
         L.or_pcg_omp.argtypes = solver_args + [C.c_double, C.c_int, C.c_int, C.POINTER(Stats)]
         L.or_gmres.argtypes = solver_args + [C.c_int, C.c_double, C.c_int, C.POINTER(Stats)]
         L.or_pbicgstab.argtypes = solver_args + [C.c_double, C.c_int, C.POINTER(Stats)]
+        L.or_pcg_bdiag3.argtypes = solver_args + [C.c_double, C.c_int, C.POINTER(Stats)]
         L.or_colormsh.argtypes = [I32P, C.c_int, C.c_int, C.c_int, C.c_int, I32P]
         L.or_tform.argtypes = [I32P, F64P, F64P, I32P, F64P, C.c_void_p, F64P, C.c_int, C.c_int,
                                C.c_int, C.c_int, C.c_int, C.c_int]
@@ -247,6 +248,15 @@ def pcg(ia, ja, ad, al, b, tol=1e-11, maxit=10000, m=None, nth=0):
     return x, st.as_dict()
 
 
+def pcg_bdiag3(ia, ja, ad, al, b, tol=1e-11, maxit=10000, bm=None):
+    """PCG with the 3x3 block-Jacobi application of op_block_precond."""
+    bm = block_precond3(ad, al, ia, ja) if bm is None else bm
+    x = np.zeros_like(b)
+    st = Stats()
+    lib().or_pcg_bdiag3(b.shape[0], ia, ja, ad, al, bm, b, x, tol, maxit, C.byref(st))
+    return x, st.as_dict()
+
+
 def gmres(ia, ja, ad, al, b, k=50, tol=1e-11, maxit=10000, m=None):
     m = pre_diag(ad) if m is None else m
     x = np.zeros_like(b)
@@ -268,3 +278,19 @@ def colormsh(mesh):
     n = lib().or_colormsh(np.ascontiguousarray(mesh.ix), mesh.nnode, mesh.numel, mesh.nen,
                           mesh.nen, ic)
     return n, ic
+
+
+def setcolor(icolor, numcolors):
+    """Colormsh.f:255-308: elcolor lists the elements colour by colour, in
+    ascending element order within a colour; colorg(2, ncol) holds each
+    colour's first and last position in elcolor (1-based)."""
+    numel = icolor.shape[0]
+    if numcolors <= 1:
+        return np.arange(1, numel + 1, dtype=np.int32), np.array([1, numel], np.int32)
+    el, cg, pos = [], [], 1
+    for c in range(1, numcolors + 1):
+        members = np.nonzero(icolor == c)[0] + 1
+        el.append(members)
+        cg += [pos, pos + len(members) - 1]
+        pos += len(members)
+    return np.concatenate(el).astype(np.int32), np.array(cg, np.int32)

@@ -1281,6 +1281,65 @@ void or_pcg(int neq, const int32_t* ia, const int32_t* ja, const double* ad,
   free(p);
 }
 
+/* Iters.f:252-524 with the Jacobi step z(i) = r(i)*m(i) replaced by the 3x3
+ * block-Jacobi application op_block_precond (Precond.f:419-540) of the m(6,.)
+ * that block_precond (Precond.f:163-358) builds: the preconditioned CG that
+ * the reference's precond code 6 names (its call site is dead).  Stopping
+ * rule and post-solve figures as in or_pcg, with M the block inverse. */
+void or_pcg_bdiag3(int neq, const int32_t* ia, const int32_t* ja, const double* ad,
+                   const double* al, const double* bm, const double* b, double* x, double tol,
+                   int maxit, or_stats* st) {
+  memset(st, 0, sizeof(*st));
+  for (int i = 0; i < neq; i++)
+    if (ad[i] == 0.0) {
+      st->status = 2;
+      return;
+    }
+  double* z = (double*)malloc(sizeof(double) * (size_t)neq);
+  double* r = (double*)malloc(sizeof(double) * (size_t)neq);
+  double* p = (double*)malloc(sizeof(double) * (size_t)neq);
+  for (int i = 0; i < neq; i++) x[i] = 0.0;
+  or_op_block_precond3(bm, b, z, neq);
+  double d = or_dot(b, z, neq);
+  double conv = tol * sqrt(fabs(d));
+  or_matvec_csrcsym1(neq, ia, ja, ad, al, x, z);
+  for (int i = 0; i < neq; i++) r[i] = b[i] - z[i];
+  or_op_block_precond3(bm, r, z, neq);
+  for (int i = 0; i < neq; i++) p[i] = z[i];
+  d = or_dot(r, z, neq);
+  int j, conv_ok = 0;
+  for (j = 1; j <= maxit; j++) {
+    or_matvec_csrcsym1(neq, ia, ja, ad, al, p, z);
+    double alpha = d / or_dot(z, p, neq);
+    for (int i = 0; i < neq; i++) {
+      x[i] = x[i] + alpha * p[i];
+      r[i] = r[i] - alpha * z[i];
+    }
+    or_op_block_precond3(bm, r, z, neq);
+    double di = or_dot(r, z, neq);
+    double beta = di / d;
+    for (int i = 0; i < neq; i++) p[i] = z[i] + beta * p[i];
+    d = di;
+    if (sqrt(fabs(d)) < conv) {
+      conv_ok = 1;
+      break;
+    }
+  }
+  if (!conv_ok) st->status = 1;
+  st->iters = conv_ok ? j : maxit;
+  or_matvec_csrcsym1(neq, ia, ja, ad, al, x, z);
+  st->xkx = or_dot(x, z, neq);
+  st->norm_x = sqrt(or_dot(x, x, neq));
+  for (int i = 0; i < neq; i++) r[i] = b[i] - z[i];
+  or_op_block_precond3(bm, r, z, neq);
+  st->norm_m_r = sqrt(fabs(or_dot(r, z, neq)));
+  st->norm_r = sqrt(or_dot(r, r, neq));
+  st->conv = conv;
+  free(z);
+  free(r);
+  free(p);
+}
+
 /* Matrix_partition.f:98-133 (partition_csrc_bynonzeros, one process) and
  * 196-211 (compute_effective_work).  Bounds are 1-based rows. */
 static void partition_rows(const int32_t* ia, const int32_t* ja, int neq, int nth, int* tbeg,

@@ -103,6 +103,12 @@ void or_pcg(int neq, const int32_t* ia, const int32_t* ja, const double* ad,
             const double* al, const double* m, const double* b, double* x, double tol,
             int maxit, or_stats* st);
 
+/* Iters.f:252-524 with z = M^-1 r applied by op_block_precond (Precond.f:419-540)
+ * on bm = m(6, .) of or_block_precond3 (3x3 block Jacobi, precond code 6). */
+void or_pcg_bdiag3(int neq, const int32_t* ia, const int32_t* ja, const double* ad,
+                   const double* al, const double* bm, const double* b, double* x, double tol,
+                   int maxit, or_stats* st);
+
 /* Iters_omp.f:17-331 + Matvec_omp.f:150-212 + Matrix_partition.f:98-211 with
  * nth threads (OpenMP).  The CPU baseline. */
 void or_pcg_omp(int neq, const int32_t* ia, const int32_t* ja, const double* ad,

@@ -11,16 +11,21 @@ c * usage: ref_harness deck outdir solvtol maxit nth mode [maxmem]     *
 c *   mode 1 = PCG through solv (Solv.f:13), nth>1 -> pcg_omp          *
 c *   mode 2 = + gmres(k=50) called with its own signature (Iters.f:528)*
 c *   mode 4 = + pbicgstab with its own signature (Iters.f:1079)       *
-c *   mode 8 = + block_precond nblock=3 (Precond.f:163)                *
+c *   mode 8 = + block_precond nblock=3 (Precond.f:163) and one      *
+c *            application op_block_precond (Precond.f:419)            *
 c *   mode 16 = element loop by colours (omp_elmt, Pform.f:108-184)    *
 c *   mode 32 = + nodal stresses (tform_mec, Pform.f:258) and the      *
 c *             result files of write_mesh_res_mec (Write_vtk.f:742):  *
 c *             legacy ASCII, legacy binary and XML ASCII              *
+c *   mode 64 = set-up only: stop after the set-up dumps and the     *
+c *             product (no solve; for the full-size workloads)        *
 c *   (modes add up)                                                   *
 c * Output (raw little-endian stream files in outdir):                 *
 c *   meta.txt, inum.i4, id.i4, ia.i4, ja.i4, ad.f8, al.f8, b.f8,      *
+c *   elcolor.i4, colorg.i4 (coloredmesh, Colormsh.f:9-63),            *
 c *   x.f8, spmv_v.f8, spmv_y.f8, resid.f8, and optionally             *
-c *   x_gmres.f8, x_bicgstab.f8, bdiag.f8, tx.f8, u.f8 and the files  *
+c *   x_gmres.f8, x_bicgstab.f8, bdiag.f8, bdiag_y.f8, tx.f8, u.f8,    *
+c *   and the files                                                    *
 c *   res_istep_0.vtk, resb_istep_0.vtk, res_istep_0.vtu               *
 c **********************************************************************
       program ref_harness
@@ -170,6 +175,8 @@ c ... dumps of the set-up
       call wf8(trim(outd)//'/ad.f8',ia(i_ad),neq)
       call wf8(trim(outd)//'/al.f8',ia(i_al),nad)
       call wf8(trim(outd)//'/b.f8',ia(i_b),neq)
+      call wi4(trim(outd)//'/elcolor.i4',ia(i_elcolor),numel)
+      call wi4(trim(outd)//'/colorg.i4',ia(i_colorg),2*numcolors)
 c ... one product y = A v (Matvec.f:528), v(i) = 1 + mod(i,7)/8
       i_v = alloc_8('spmvv   ',1,neq)
       i_y = alloc_8('spmvy   ',1,neq)
@@ -182,6 +189,7 @@ c ... one product y = A v (Matvec.f:528), v(i) = 1 + mod(i,7)/8
       call wf8(trim(outd)//'/spmv_y.f8',ia(i_y),neq)
       i_y = dealloc('spmvy   ')
       i_v = dealloc('spmvv   ')
+      if (iand(mode,64) .ne. 0) goto 900
 c ... the solve, timed as soltime is (Main.f:819-827)
       open(10,file=trim(outd)//'/solver.log')
       tt = MPI_Wtime()
@@ -266,6 +274,15 @@ c ... 3x3 block-Jacobi set-up (Precond.f:163), dead in the driver
      .       ia(i_bd),neq,ia(i_aux),param)
         nbd = 6*int(mm)
         call wf8(trim(outd)//'/bdiag.f8',ia(i_bd),nbd)
+c ... one application y = M^-1 v (Precond.f:419), v as for the product
+        i_v = alloc_8('bdv     ',1,neq)
+        i_y = alloc_8('bdy     ',1,neq)
+        call fillv(ia(i_v),neq)
+        call azero(ia(i_y),neq)
+        call op_block_precond(ia(i_bd),ia(i_v),ia(i_y),ia(i_aux),param)
+        call wf8(trim(outd)//'/bdiag_y.f8',ia(i_y),neq)
+        i_y = dealloc('bdy     ')
+        i_v = dealloc('bdv     ')
       endif
 c ... nodal stresses and the result files (Main.f:930-963), after the
 c     displacements of the solve went to u (update_mec above)
@@ -296,6 +313,7 @@ c     displacements of the solve went to u (update_mec above)
      .       nnode,numel,nen,ndm,ndf,ntn,prename,0,.true.,.true.,
      .       pflag,31)
       endif
+  900 continue
       close(10)
       close(20)
       call MPI_FINALIZE(ierr)

@@ -24,6 +24,15 @@ CASES = {
                 1e-11, 1, False),
 }
 
+# full-size workloads whose reference set-up (RCM, numeq, CSRC, colouring,
+# coloured assembly, one product) is kept as digests: hex8 100^3 jittered,
+# tet4 80^3 x 6 and the plasticity mesh (hex8 60^3 between grips)
+DIGESTS = {
+    "cfg2_hex8_100": lambda: M.config_mesh(2),
+    "cfg3_tet4_80": lambda: M.config_mesh(3),
+    "cfg5_pull_60": lambda: M.hex8_block_pull(60, 5e-3)[0],
+}
+
 # fixtures written from the reference's own decks
 REF_DECKS = {
     "quad_el": "quad/exemplo1_apostila_el.dat",

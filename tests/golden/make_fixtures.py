@@ -8,7 +8,8 @@ element stiffness + load assembly, PCG through ``solv``) and additionally calls
 ``tform_mec`` + ``write_mesh_res_mec`` for the nodal stresses and result files.
 For every case this script writes a deck with elastic_amd.deck, runs the
 harness, and stores its outputs as a small .npz fixture (large cases keep
-SHA-256 digests of the integer arrays and summaries of the floating-point ones).
+SHA-256 digests of the arrays and summaries of the floating-point ones; the
+full-size workloads of DIGESTS run the reference's set-up only).
 
 Run from the repository root (needs /root/reference, so only in the build
 container):  python tests/golden/make_fixtures.py
@@ -27,7 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 from elastic_amd import deck  # noqa: E402
-from tests.cases import CASES, REF_DECKS  # noqa: E402
+from tests.cases import CASES, DIGESTS, REF_DECKS  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -38,9 +39,9 @@ def sha(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def run_harness(deck_path, tol, maxit, mode, workdir, nth=1):
+def run_harness(deck_path, tol, maxit, mode, workdir, nth=1, maxmem=400000000):
     r = subprocess.run([HARNESS, deck_path, workdir, repr(tol), str(maxit), str(nth), str(mode),
-                        "400000000"], capture_output=True, text=True)
+                        str(maxmem)], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
     meta = open(os.path.join(workdir, "meta.txt")).read().split()
@@ -48,10 +49,11 @@ def run_harness(deck_path, tol, maxit, mode, workdir, nth=1):
     rd = lambda n, dt: np.fromfile(os.path.join(workdir, n), dtype=dt)  # noqa: E731
     out = dict(inum=rd("inum.i4", np.int32), id=rd("id.i4", np.int32), ia=rd("ia.i4", np.int32),
                ja=rd("ja.i4", np.int32), ad=rd("ad.f8", np.float64), al=rd("al.f8", np.float64),
-               b=rd("b.f8", np.float64), x=rd("x.f8", np.float64),
-               spmv_v=rd("spmv_v.f8", np.float64), spmv_y=rd("spmv_y.f8", np.float64),
-               resid=rd("resid.f8", np.float64))
-    for name in ("x_gmres", "x_bicgstab", "bdiag", "tx", "u"):
+               b=rd("b.f8", np.float64))
+    for name in ("elcolor", "colorg"):
+        out[name] = rd(name + ".i4", np.int32)
+    for name in ("x", "spmv_v", "spmv_y", "resid", "x_gmres", "x_bicgstab", "bdiag", "bdiag_y",
+                 "tx", "u"):
         p = os.path.join(workdir, name + ".f8")
         if os.path.exists(p):
             out[name] = np.fromfile(p, dtype=np.float64)
@@ -61,7 +63,8 @@ def run_harness(deck_path, tol, maxit, mode, workdir, nth=1):
         p = os.path.join(workdir, fname)
         if os.path.exists(p):
             out[key] = np.frombuffer(open(p, "rb").read(), dtype=np.uint8)
-    log = open(os.path.join(workdir, "solver.log")).read()
+    lp = os.path.join(workdir, "solver.log")
+    log = open(lp).read() if os.path.exists(lp) else ""
     stats = {}
     for key, pat in (("pcg", r"PCG:\s+it\s+(\d+) x \* Kx\s+(\S+) \|\|x\|\|\s+(\S+)"),
                      ("gmres", r"GMRES:\s+it\s+(\d+) x \* Kx\s+(\S+) \|\|x\|\|\s+(\S+)"),
@@ -82,10 +85,13 @@ def summarize(out, stats, keep_full):
         for k, v in out.items():
             res[k] = v
         return res
-    for k in ("inum", "id", "ia", "ja"):
+    for k in ("inum", "id", "ia", "ja", "elcolor", "colorg"):
         res["sha_" + k] = np.array(sha(out[k]))
     for k in ("ad", "al", "b", "spmv_y", "resid"):
+        if k not in out:
+            continue
         v = out[k]
+        res["sha_" + k] = np.array(sha(v))
         res["norm_" + k] = np.array([np.linalg.norm(v), np.abs(v).max(), v.sum()])
         res["sample_" + k] = v[np.linspace(0, v.size - 1, 64).astype(np.int64)]
     for k in ("x", "x_gmres", "x_bicgstab", "bdiag"):
@@ -98,6 +104,24 @@ def main(only=None):
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-s", "-f", "oracle/Makefile.ref"], cwd=ROOT, check=True)
     with tempfile.TemporaryDirectory() as tmp:
+        for name, build in DIGESTS.items():
+            if only and name not in only:
+                continue
+            # full-size workloads: the reference's set-up only (mode 64), with
+            # the coloured element loop (mode 16) whose sums the GPU assembly
+            # reproduces; digests of every array
+            m = build()
+            dpath = os.path.join(tmp, name + ".dat")
+            deck.write_deck(m, dpath)
+            wd = os.path.join(tmp, name)
+            os.makedirs(wd)
+            out, stats = run_harness(dpath, 1e-10, 1, 16 | 64, wd, maxmem=1500000000)
+            res = summarize(out, stats, False)
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), **res)
+            print(name, out["dims"].tolist(), flush=True)
+            for f in os.listdir(wd):
+                os.remove(os.path.join(wd, f))
+            os.remove(dpath)
         for name, (build, tol, mode, full) in CASES.items():
             if (only and name not in only) or name in REF_DECKS:
                 continue

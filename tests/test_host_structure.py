@@ -71,3 +71,23 @@ def test_structure_invariants_tet():
     assert low.sum() == ia[-1] - 1
     rows = np.repeat(np.arange(neq), np.diff(ia))
     assert low[rows, ja - 1].all()
+
+
+@pytest.mark.parametrize("name", ["cfg2_hex8_100", "cfg3_tet4_80", "cfg5_pull_60"])
+def test_full_size_structure_digests(name, golden):
+    """The full-size workloads (hex8 100^3 jittered, tet4 80^3 x 6, the
+    plasticity mesh hex8 60^3): RCM order, equation numbers and the CSRC
+    pattern are the reference's, bit for bit (digests of the reference's own
+    set-up, Reord.f:1-434, Numeq.f:139-191, Csr.f:13-305)."""
+    from tests.cases import DIGESTS
+
+    g = golden(name)
+    m = DIGESTS[name]()
+    perm = _lib.reord(m)
+    idn, neq = _lib.numeq(m, perm)
+    ia, ja = _lib.csrstruct(m, idn, neq)
+    assert [neq, ia[-1] - 1] == [int(g["dims"][5]), int(g["dims"][6])]
+    assert sha(perm) == str(g["sha_inum"])
+    assert sha(idn.ravel()) == str(g["sha_id"])
+    assert sha(ia) == str(g["sha_ia"])
+    assert sha(ja) == str(g["sha_ja"])

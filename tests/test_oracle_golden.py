@@ -93,6 +93,85 @@ def test_block_jacobi_bit_exact(name, golden, systems):
     assert np.array_equal(m, g["bdiag"])
 
 
+@pytest.mark.parametrize("name", [n for n in SMALL if CASES[n][2] & 16])
+def test_colormsh_bit_exact(name, golden):
+    """or_colormsh (Colormsh.f:65-247) with setcolor's ordering
+    (Colormsh.f:255-308) gives the reference's elcolor and colorg."""
+    g = golden(name)
+    mesh = CASES[name][0]()
+    ncol, ic = O.colormsh(mesh)
+    assert ncol >= 2 and np.all(ic >= 1)
+    elcolor, colorg = O.setcolor(ic, ncol)
+    assert np.array_equal(elcolor, g["elcolor"])
+    assert np.array_equal(colorg, g["colorg"])
+
+
+@pytest.mark.parametrize("name", ["hex8_5j", "tet4_3j", "quad4_8"])
+def test_colormsh_is_a_colouring(name):
+    """No two elements of one colour share a node."""
+    mesh = CASES[name][0]()
+    ncol, ic = O.colormsh(mesh)
+    nodes = mesh.ix[:, : mesh.nen]
+    for c in range(1, ncol + 1):
+        el = nodes[ic == c].ravel()
+        el = el[el > 0]
+        assert np.unique(el).size == el.size, c
+
+
+@pytest.mark.parametrize("name", [n for n in SMALL if CASES[n][2] & 8])
+def test_op_block_precond_bit_exact(name, golden):
+    """or_op_block_precond3 (Precond.f:419-540) applied to the reference's own
+    m(6, .) and vector gives the reference's op_block_precond output."""
+    g = golden(name)
+    y = O.op_block_precond3(g["bdiag"], g["spmv_v"])
+    assert np.array_equal(y, g["bdiag_y"])
+
+
+def dense(ia, ja, ad, al):
+    n = ad.shape[0]
+    A = np.diag(ad.copy())
+    for i in range(n):
+        for k in range(ia[i] - 1, ia[i + 1] - 1):
+            A[i, ja[k] - 1] = A[ja[k] - 1, i] = al[k]
+    return A
+
+
+@pytest.mark.parametrize("name", ["hex8_5j", "tet4_3j", "quad4_8", "tria3_8"])
+def test_block_precond_is_the_block_inverse(name, systems):
+    """block_precond + op_block_precond = the inverse of each diagonal 3x3
+    block of A (consecutive equation triples, rest rows by 1/ad), applied to a
+    vector; a transposed or mis-slotted cofactor fails this."""
+    s = systems(name)
+    A = dense(s.ia, s.ja, s.ad, s.al[: s.nad])
+    n = s.neq
+    v = 1.0 + np.arange(n) % 7 / 8.0 + np.sin(np.arange(n))
+    y = O.op_block_precond3(O.block_precond3(s.ad, s.al, s.ia, s.ja), v)
+    z = np.empty(n)
+    nb = n // 3
+    for i in range(nb):
+        sl = slice(3 * i, 3 * i + 3)
+        z[sl] = np.linalg.solve(A[sl, sl], v[sl])
+    z[3 * nb:] = v[3 * nb:] / s.ad[3 * nb:]
+    assert np.linalg.norm(y - z) <= 1e-13 * np.linalg.norm(z)
+
+
+@pytest.mark.parametrize("name", ["hex8_5j", "tet4_3j", "quad4_8"])
+def test_pcg_bdiag3_solves_the_system(name, systems):
+    """PCG with the block-Jacobi application reaches the dense solution; on
+    3-dof systems, whose triples are node blocks, in fewer iterations than
+    Jacobi PCG."""
+    s = systems(name)
+    A = dense(s.ia, s.ja, s.ad, s.al[: s.nad])
+    xd = np.linalg.solve(A, s.b)
+    x, st = O.pcg_bdiag3(s.ia, s.ja, s.ad, s.al, s.b, tol=1e-12)
+    assert st["status"] == 0
+    assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+    assert st["norm_m_r"] < 3.16 * st["conv"]
+    if s.mesh.ndf == 3:
+        _, sj = O.pcg(s.ia, s.ja, s.ad, s.al, s.b, tol=1e-12)
+        assert st["iters"] < sj["iters"]
+
+
 @pytest.mark.parametrize("name", ["hex8_20", "quad_el"])
 def test_large_case_digests(name, golden, systems):
     """Workload 1 (hex8 20^3) and the reference's quad deck, regenerated."""
