@@ -806,7 +806,7 @@ extern "C" int elastic_set_mesh(elastic_ctx* xc, int32_t nnode, int32_t numel, i
   if (!xc || nnode <= 0 || numel <= 0 || !ix || !x || !id || !ie || !e || numat <= 0)
     return ELASTIC_BAD_ARG;
   Context* c = &xc->c;
-  if (!c->tiles.ntiles && !c->blk.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
+  if (!layout_ready(c)) return set_error(c, ELASTIC_NOT_READY, "structure not set");
   hipSetDevice(c->device);
   // element kind from the materials used (one kind per mesh)
   int kind = -1;
@@ -838,7 +838,7 @@ extern "C" int elastic_set_mesh(elastic_ctx* xc, int32_t nnode, int32_t numel, i
     if (id[q] > c->neq) return set_error(c, ELASTIC_BAD_ARG, "equation number above neq");
   std::vector<int32_t> icolor;
   int ncol = color_elements(ix, nnode, numel, nen, nen, icolor);
-  if (ncol < 0) return set_error(c, ELASTIC_BAD_ARG, "more than 200 element colours");
+  if (ncol < 0) return set_error(c, ELASTIC_BAD_ARG, "element colouring failed");
   std::vector<int32_t> order;
   order.reserve(numel);
   std::vector<int> cstart(ncol + 1, 0);

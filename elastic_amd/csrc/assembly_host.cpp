@@ -43,7 +43,11 @@ int color_elements(const int32_t* ix, int nnode, int numel, int nen, int nenv,
       if (node > 0) incid[start[node] + fill[node]++] = e + 1;
     }
   icolor.assign(numel + 1, 0);  // 1-based element numbers
-  for (int c = 1; c <= 200; ++c) {
+  // The reference stops at 200 colours (maxcolors, Colormsh.f:176); the same
+  // greedy rule continues here, so meshes with more than 200 elements around
+  // one node still assemble (every colour takes at least one element, so at
+  // most numel colours are needed).  The first 200 colours are the reference's.
+  for (int c = 1; c <= numel; ++c) {
     for (int node = 1; node <= nnode; ++node) {
       bool found = false;
       int first = 0;
@@ -69,7 +73,7 @@ int color_elements(const int32_t* ix, int nnode, int numel, int nen, int nenv,
       }
     if (quit) return c;
   }
-  return -1;  // more than 200 colours (Colormsh.f:244)
+  return numel > 0 ? -1 : 0;  // not reached for numel > 0
 }
 
 // slots: numel * npair targets, pairs enumerated i = 0..nst-1, j = 0..i.

@@ -287,8 +287,11 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
     const int c = wl_index(e);
     if (uniform_done(I, c, sh)) return;
     const double di = inst_sum(I.part + I.nch, I.nch, sh);
+    // restart (beta = 0) right after the hand-over to the FP64 operator
+    const bool restart = I.sc->restart[cur] != 0;
     const double beta =
-        I.fp32 ? -inst_sum(I.part + 4 * I.nch, I.nch, sh) / I.sc->pq : di / I.dslot[cur];
+        restart ? 0.0
+        : I.fp32 ? -inst_sum(I.part + 4 * I.nch, I.nch, sh) / I.sc->pq : di / I.dslot[cur];
     const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
     double apr = 0.0;
     if (I.p32 && !I.sc->op64) {  // FP32 vectors (see BInst)
@@ -312,6 +315,11 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
     }
     if (c == 0 && threadIdx.x == 0) {
       SolveScalars* sc = I.sc;
+      sc->restart[cur ^ 1] = 0;  // the previous step's flag (no chunk reads it now)
+      if (restart) {  // plain FP64 PCG from here: no more replacements
+        sc->reliable = 0;
+        sc->upd = 0;
+      }
       I.dslot[cur ^ 1] = di;
       sc->beta = beta;
       sc->di = di;
@@ -386,7 +394,9 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_finish(const BInst* __restr
       SolveScalars* sc = I.sc;
       sc->true_rmr = rz;
       sc->nupd += 1;
+      const int32_t was64 = sc->op64;
       progress_guard(sc, sqrt(fabs(rz)));
+      if (!was64 && sc->op64) sc->restart[nxt] = 1;  // read by the next p update
       if (sqrt(fabs(rz)) < sc->conv) {
         sc->done = 1;
         sc->status = 0;

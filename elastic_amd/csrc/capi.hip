@@ -58,10 +58,19 @@ static void free_blocks(DevBlocks& b) {
   b = DevBlocks();
 }
 
+static void free_gather(DevGather& g) {
+  void* ptrs[] = {g.ia, g.ja, g.colptr, g.colk, g.colrow};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  g = DevGather();
+}
+
 // device layout of the product (tiles or node blocks, FP32 copies, partials)
 static void free_layout(Context* c) {
   free_tiles(c->tiles);
   free_blocks(c->blk);
+  free_gather(c->gth);
+  c->gather = false;
   c->inc_ptr = c->inc_idx = nullptr;
   void* ptrs[] = {c->A32.ad, c->A32.al, c->epart64, c->tdot, c->p32};
   for (void* p : ptrs)
@@ -94,7 +103,17 @@ using namespace elastic;
 
 // Builds the device layout of the product from x->ia / x->ja (node-blocked
 // when the pattern allows it, else scalar row tiles) with the context's tile
-// sizes; values are packed separately (derive_operator).
+// sizes; values are packed separately (derive_operator).  A pattern that the
+// node-blocked tiles cannot hold (a node row of more than 64 lower blocks, or
+// longer than the block tile) falls back to scalar tiles, and the scalar
+// tiles grow to the longest CSRC row; a row of more than
+// kTileColsMax - kTileRowsMax lower entries takes the gather-only product
+// (spmv_gather.hip), which holds any pattern.
+static bool getenv_is(const char* name, const char* value) {
+  const char* v = getenv(name);
+  return v && std::string(v) == value;
+}
+
 static int build_layout(elastic_ctx* x) {
   Context* c = &x->c;
   const int neq = c->neq;
@@ -104,15 +123,58 @@ static int build_layout(elastic_ctx* x) {
   int nb = 1;
   const char* lay = getenv("ELASTIC_LAYOUT");
   if (c->allow_blocks && !(lay && std::string(lay) == "scalar")) nb = detect_node_blocks(neq, ia, ja);
-  try {
-    if (nb > 1)
+  c->layout_note.clear();
+  if (nb > 1) {
+    try {
       build_block_tiles(neq, nb, ia, ja,
                         c->tile_blocks > 0 ? c->tile_blocks
                         : (nb == 3 && neq / 3 >= kLargeTileNodes ? kTileBlocks3 : kTileBlocks2),
                         c->blayout);
-    else build_tiles(neq, ia, ja, c->tile_nnz, c->layout);
-  } catch (const std::exception& e) {
-    return set_error(c, ELASTIC_BAD_ARG, e.what());
+    } catch (const std::exception& e) {
+      c->layout_note = std::string("scalar tiles: ") + e.what();
+      c->blayout = BlockLayout();
+      nb = 1;
+    }
+  }
+  if (nb == 1) {
+    int longest = 0;
+    for (int i = 0; i < neq; ++i) longest = std::max(longest, ia[i + 1] - ia[i]);
+    if (longest > kTileColsMax - kTileRowsMax || getenv_is("ELASTIC_LAYOUT", "gather")) {
+      c->layout_note += (c->layout_note.empty() ? "" : "; ") + std::string("gather product (row of ") +
+                        std::to_string(longest) + " entries)";
+      GatherLayout G;
+      try {
+        build_gather(neq, ia, ja, G);
+      } catch (const std::bad_alloc&) {
+        return set_error(c, ELASTIC_OOM, "host memory for the transposed pattern");
+      }
+      DevGather& g = c->gth;
+      int rc = upload(c, &g.ia, ia, (size_t)neq + 1);
+      rc = rc ? rc : upload(c, &g.ja, ja, (size_t)nad);
+      rc = rc ? rc : upload(c, &g.colptr, G.colptr.data(), G.colptr.size());
+      rc = rc ? rc : upload(c, &g.colk, G.colk.data(), G.colk.size());
+      rc = rc ? rc : upload(c, &g.colrow, G.colrow.data(), G.colrow.size());
+      rc = rc ? rc : dalloc(c, &c->A32.ad, neq);
+      rc = rc ? rc : dalloc(c, &c->A32.al, (size_t)nad);
+      rc = rc ? rc : dalloc(c, &c->epart64, 1);
+      if (rc) return rc;
+      g.longest = G.longest;
+      c->gather = true;
+      c->layout_nb = 1;
+      c->inc_ptr = c->inc_idx = nullptr;
+      if (c->accum32_set < 0) c->accum32 = false;
+      ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+      return ELASTIC_OK;
+    }
+    const int tn = std::max(c->tile_nnz, longest);
+    if (tn != c->tile_nnz)
+      c->layout_note += (c->layout_note.empty() ? "" : "; ") +
+                        std::string("row tiles of ") + std::to_string(tn) + " nonzeros";
+    try {
+      build_tiles(neq, ia, ja, tn, c->layout);
+    } catch (const std::exception& e) {
+      return set_error(c, ELASTIC_BAD_ARG, e.what());
+    }
   }
   c->layout_nb = nb;
   // FP32 sums by default where they pay: large 3-dof systems (hex8 100^3:
@@ -192,6 +254,39 @@ static int build_layout(elastic_ctx* x) {
   c->layout.word.shrink_to_fit();
   c->blayout.word.clear();
   c->blayout.word.shrink_to_fit();
+  return ELASTIC_OK;
+}
+
+// Rebuilds the layout with new tile sizes.  If that fails, the previous
+// sizes and layout are restored (the values are re-derived), so the context
+// stays usable and the call reports the failure; if even the restore fails,
+// the values are dropped so that solves report NOT_READY.
+static int relayout(elastic_ctx* x, int tile_blocks, int tile_nnz) {
+  Context* c = &x->c;
+  const int ob = c->tile_blocks, on = c->tile_nnz;
+  hipSetDevice(c->device);
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  c->tile_blocks = tile_blocks;
+  c->tile_nnz = tile_nnz;
+  free_layout(c);
+  int rc = build_layout(x);
+  if (rc) {
+    const std::string why = c->err;
+    c->tile_blocks = ob;
+    c->tile_nnz = on;
+    free_layout(c);
+    const int rc2 = build_layout(x);
+    if (rc2) {
+      free_layout(c);
+      c->values_set = c->values32 = c->bdiag_set = false;
+      return set_error(c, rc2, why + "; rebuilding the previous layout failed too: " + c->err);
+    }
+    if (c->values_set) derive_operator(c);
+    ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return set_error(c, rc, why + " (previous layout kept)");
+  }
+  if (c->values_set) derive_operator(c);
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return ELASTIC_OK;
 }
 
@@ -319,6 +414,10 @@ int elastic_set_structure(elastic_ctx* x, int32_t neq, int64_t nad, const int32_
   int rc = build_layout(x);
   if (rc) {
     free_system(c);
+    x->ia.clear();
+    x->ja.clear();
+    c->neq = 0;
+    c->nad = 0;
     return rc;
   }
   rc = rc ? rc : dalloc(c, &c->A64.ad, neq);
@@ -327,6 +426,10 @@ int elastic_set_structure(elastic_ctx* x, int32_t neq, int64_t nad, const int32_
   for (auto& v : c->v64) rc = rc ? rc : dalloc(c, &v, neq);
   if (rc) {
     free_system(c);
+    x->ia.clear();
+    x->ja.clear();
+    c->neq = 0;
+    c->nad = 0;
     return rc;
   }
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -347,15 +450,7 @@ int elastic_autotune(elastic_ctx* x, int32_t precision, int32_t reps, int32_t* b
   int arg = -1;
   double tmin = 1e300;
   auto apply = [&](int v) -> int {
-    if (blocked) c->tile_blocks = v;
-    else c->tile_nnz = v;
-    ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    free_layout(c);
-    int rc = build_layout(x);
-    if (rc) return rc;
-    derive_operator(c);
-    ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return ELASTIC_OK;
+    return blocked ? relayout(x, v, c->tile_nnz) : relayout(x, c->tile_blocks, v);
   };
   for (int v : cands) {
     int rc = apply(v);
@@ -390,17 +485,11 @@ int elastic_set_option(elastic_ctx* x, const char* name, int64_t value) {
     const bool blocks = n == "tile_blocks";
     if (blocks && (value < 32 || value > kBlkTileMax)) return ELASTIC_BAD_ARG;
     if (!blocks && (value < 64 || value > kTileColsMax - kTileRowsMax)) return ELASTIC_BAD_ARG;
-    if (blocks) c->tile_blocks = (int)value;
-    else c->tile_nnz = (int)value;
-    if (c->neq > 0 && (c->tiles.ntiles || c->blk.ntiles)) {  // rebuild the layout now
-      hipSetDevice(c->device);
-      ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-      free_layout(c);
-      int rc = build_layout(x);
-      if (rc) return rc;
-      if (c->values_set) derive_operator(c);
-      ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    }
+    const int tb = blocks ? (int)value : c->tile_blocks;
+    const int tn = blocks ? c->tile_nnz : (int)value;
+    if (c->neq > 0 && layout_ready(c)) return relayout(x, tb, tn);  // now
+    c->tile_blocks = tb;
+    c->tile_nnz = tn;
   } else {
     return set_error(c, ELASTIC_BAD_ARG, "unknown option " + n);
   }
@@ -410,7 +499,7 @@ int elastic_set_option(elastic_ctx* x, const char* name, int64_t value) {
 int elastic_set_values(elastic_ctx* x, const double* ad, const double* al) {
   if (!x || !ad || (x->c.nad > 0 && !al)) return ELASTIC_BAD_ARG;
   Context* c = &x->c;
-  if (!c->tiles.ntiles && !c->blk.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
+  if (!layout_ready(c)) return set_error(c, ELASTIC_NOT_READY, "structure not set");
   hipSetDevice(c->device);
   c->zero_diag_row = -1;
   for (int i = 0; i < c->neq; ++i)
@@ -507,7 +596,12 @@ int elastic_product_timing(elastic_ctx* x, double* avg_ms, int64_t* count, int32
 int elastic_layout_info(elastic_ctx* x, int64_t* info) {
   if (!x || !info) return ELASTIC_BAD_ARG;
   Context* c = &x->c;
-  if (c->layout_nb > 1) {
+  if (c->gather) {  // no tiles: the longest row, and 0 as the block size
+    info[0] = 0;
+    info[1] = c->gth.longest;
+    info[2] = 0;
+    info[3] = 0;
+  } else if (c->layout_nb > 1) {
     info[0] = c->blk.ntiles;
     info[1] = c->blk.nblk_max;
     info[2] = c->blk.next_nodes;
@@ -524,7 +618,7 @@ int elastic_layout_info(elastic_ctx* x, int64_t* info) {
 int elastic_upload_rhs(elastic_ctx* x, const double* b) {
   if (!x || !b) return ELASTIC_BAD_ARG;
   Context* c = &x->c;
-  if (!c->tiles.ntiles && !c->blk.ntiles) return set_error(c, ELASTIC_NOT_READY, "structure not set");
+  if (!layout_ready(c)) return set_error(c, ELASTIC_NOT_READY, "structure not set");
   hipSetDevice(c->device);
   ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->v64[VB], b, sizeof(double) * c->neq, hipMemcpyHostToDevice, c->stream));
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -739,10 +833,19 @@ int elastic_solve(elastic_ctx* x, int32_t solver, int32_t precond, int32_t preci
 namespace {
 struct DropIn {
   elastic_ctx* ctx = nullptr;
-  int32_t neq = -1, nad = -1;
-  const int32_t *ip = nullptr, *ja = nullptr;
 };
 DropIn g_dropin;
+
+// true when the caller's pattern is the one the context holds: the sizes and
+// every entry of ip and ja (a Fortran pool may hand the same addresses to a
+// different mesh of the same size)
+bool same_pattern(const elastic_ctx* x, int32_t neq, int32_t nad, const int32_t* ip,
+                  const int32_t* ja) {
+  if (x->c.neq != neq || x->c.nad != nad || x->ia.size() != (size_t)neq + 1 ||
+      x->ja.size() != (size_t)nad)
+    return false;
+  return std::equal(ip, ip + neq + 1, x->ia.begin()) && std::equal(ja, ja + nad, x->ja.begin());
+}
 }  // namespace
 
 extern "C" int32_t elastic_solv_(int32_t* neq, int32_t* nad, int32_t* ip, int32_t* ja, double* ad,
@@ -766,13 +869,12 @@ extern "C" int32_t elastic_solv_(int32_t* neq, int32_t* nad, int32_t* ip, int32_
     rc = elastic_create(&g.ctx, 0);
     if (rc) return rc;
   }
-  if (g.neq != *neq || g.nad != *nad || g.ip != ip || g.ja != ja) {
+  if (!same_pattern(g.ctx, *neq, *nad, ip, ja)) {
     rc = elastic_set_structure(g.ctx, *neq, *nad, ip, ja);
     if (rc) {
       fprintf(stderr, "elastic_solv: %s\n", elastic_last_error(g.ctx));
       return rc;
     }
-    g.neq = *neq, g.nad = *nad, g.ip = ip, g.ja = ja;
   }
   rc = elastic_set_values(g.ctx, ad, al);
   if (rc) return rc;

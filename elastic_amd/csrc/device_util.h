@@ -61,18 +61,28 @@ __device__ inline double apply_m(int i, int n, const double* __restrict__ r,
 
 // Progress guard of the mixed methods, run at every FP64 true-residual
 // replacement with the new true residual norm tr.  When it has not dropped
-// below 0.9 x the previous one at three replacements in a row, the FP32
+// below 0.9 x the best one so far at three replacements in a row, the FP32
 // operator is not accurate enough for this system (its perturbation times
 // the condition number approaches 1), and the solve continues with the FP64
-// operator from the current x.  A replacement is also forced after
-// kForceReplace iterations without one, so stagnation is always seen.
+// operator from the current x.  Comparing with the best value, not the last
+// one, also catches a true residual that oscillates without progress.  A
+// replacement is also forced after kForceReplace iterations without one, so
+// stagnation is always seen.  The FP64 iteration then restarts from the true
+// residual (p = M^-1 r) and makes no more replacements: the search directions
+// built with the FP32 operator, and replacements that keep them, slow CG down
+// badly on such systems (a fan mesh of condition number 5.5e9 did not
+// converge in 50,000 iterations without the restart, and in 8,126 with it).
 constexpr int kForceReplace = 500;
 // Systems of at least this many rows let the host skip the gated
 // replacement passes in poll chunks where the trigger cannot come yet.
 constexpr int kPredictRows = 200000;
 __device__ inline void progress_guard(SolveScalars* sc, double tr) {
-  sc->stall = tr > 0.9 * sc->last_true ? sc->stall + 1 : 0;
-  sc->last_true = tr;
+  if (tr > 0.9 * sc->last_true) {
+    sc->stall += 1;
+  } else {
+    sc->stall = 0;
+    sc->last_true = tr;  // the best true residual so far
+  }
   if (sc->stall >= 3 && !sc->op64) {
     sc->op64 = 1;
     sc->op64_from = sc->iter;
@@ -80,6 +90,7 @@ __device__ inline void progress_guard(SolveScalars* sc, double tr) {
   sc->last_upd = sc->iter;
 }
 __device__ inline void progress_init(SolveScalars* sc, double r0) {
+  sc->restart[0] = sc->restart[1] = 0;
   sc->last_true = r0;
   sc->stall = 0;
   sc->op64 = 0;

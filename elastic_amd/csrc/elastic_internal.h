@@ -97,6 +97,27 @@ struct DevBlocks {
   float *v32 = nullptr, *d32 = nullptr;
 };
 
+// ------------------------------------------------------ gather-only product
+// (spmv_gather.hip) for patterns the row tiles cannot hold: the transposed
+// lists of the CSRC pattern, built on the host.  colptr (neq+1, 0-based
+// offsets), then per entry its CSRC position colk and row colrow (0-based),
+// rows ascending within a column.
+struct GatherLayout {
+  std::vector<int32_t> colptr, colk, colrow;
+  int longest = 0;  // longest CSRC row
+};
+void build_gather(int neq, const int32_t* ia1, const int32_t* ja1, GatherLayout& L);
+
+struct DevGather {
+  int32_t *ia = nullptr, *ja = nullptr;  // the CSRC pattern (1-based values)
+  int32_t *colptr = nullptr, *colk = nullptr, *colrow = nullptr;
+  int longest = 0;
+};
+template <typename TV>
+void launch_spmv_gather(int n, const int32_t* ia, const int32_t* ja, const TV* ad, const TV* al,
+                        const int32_t* colptr, const int32_t* colk, const int32_t* colrow,
+                        const double* x, double* y, hipStream_t s, const int32_t* skip);
+
 // -------------------------------------------------------------- device state
 template <typename T>
 struct DevMatrix {
@@ -132,6 +153,9 @@ struct SolveScalars {
   int32_t stall, op64, op64_from, last_upd;
   // p.(A p) formed by the product's tiles (node-blocked layouts)
   double pq_tiles;
+  // batch: restart flags by iteration parity (set when the progress guard
+  // hands over to the FP64 operator; the next p update takes beta = 0)
+  int32_t restart[2];
 };
 
 constexpr int kRedBlocks = 1024;  // fixed reduction grid (deterministic sums)
@@ -158,10 +182,13 @@ struct Context {
   // kLargeTileNodes nodes, where FP32 sums pay); else set by the caller
   int accum32_set = -1;
   bool allow_blocks = true;
+  std::string layout_note;  // why the layout differs from the default (fallbacks), or empty
   TileLayout layout;
   DevTiles tiles;
   BlockLayout blayout;
   DevBlocks blk;
+  bool gather = false;  // the gather-only product (layout_nb stays 1)
+  DevGather gth;
   const int32_t* inc_ptr = nullptr;  // merge lists of the active layout
   const int32_t* inc_idx = nullptr;
   DevMatrix<double> A64;
@@ -230,6 +257,10 @@ __device__ inline double merge_row(const MergeArgs& m, int i, double s) {
 enum VecSlot { VX = 0, VR, VP, VQ, VB, VT, VU, VW };
 
 int set_error(Context* c, int code, const std::string& msg);
+// a product layout has been built (structure set)
+inline bool layout_ready(const Context* c) {
+  return c->tiles.ntiles > 0 || c->blk.ntiles > 0 || c->gth.colptr != nullptr;
+}
 // Partials of a product with FP32 sums (node-blocked layouts) are stored as
 // FP32 (exactly the values summed); every other product stores FP64.
 // distance between the FP32 vectors held at Context::p32 (256-byte aligned)

@@ -401,6 +401,33 @@ __global__ void k_rel_finish(const double* __restrict__ part, double* dslot, Sol
   }
 }
 
+// restart after the hand-over to the FP64 operator: p = M^-1 r from the true
+// residual just computed (partial p.r for the next update when asked), and
+// no more replacements (the stop test is made on the recursive residual, as
+// FP64 PCG makes it)
+__global__ __launch_bounds__(kThreads) void k_pcg_restart(int n, const double* __restrict__ r,
+                                                           const double* __restrict__ m,
+                                                           const double* __restrict__ bm,
+                                                           double* p, double* part_pr,
+                                                           SolveScalars* sc) {
+  __shared__ double sh[8];
+  if (block_flag(&sc->done, sh)) return;
+  double apr = 0.0;
+  GRID_LOOP(i, n) {
+    const double zi = apply_m(i, n, r, m, bm);
+    p[i] = zi;
+    apr += zi * r[i];
+  }
+  if (part_pr) {
+    const double t = block_sum(apr, sh);
+    if (threadIdx.x == 0) part_pr[blockIdx.x] = t;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->reliable = 0;
+    sc->upd = 0;
+  }
+}
+
 // ------------------------------------------------------------- host side
 int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int maxit,
             bool reliable, double delta, double* dslots) {
@@ -448,6 +475,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   const int every = std::max(2, c->check_every & ~1);
   bool done = false;
   bool op32 = fp32_operator;  // the progress guard may hand over to the FP64 operator
+  bool replacing = reliable;  // replacements stop at the hand-over's restart
   // sampled event timing of the products (measurement option)
   const int stride = c->time_stride;
   if (stride > 0 && c->tev.size() < (size_t)2 * every) {
@@ -516,7 +544,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
                          V.p, V.r, V.m, V.bm, pr_next, p32, (const float*)r32,
                          (const float*)m32, sc);
       // the replacement kernels gate themselves on the flag set by k_pcg_pupdate
-      if (reliable && gate && ((it + 1) & 1) == 0) replace(it + 1);
+      if (replacing && gate && ((it + 1) & 1) == 0) replace(it + 1);
     }
     ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->scal_host, sc, sizeof(SolveScalars),
                                         hipMemcpyDeviceToHost, s));
@@ -542,6 +570,12 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
         v32 = false;
         r32 = m32 = nullptr;
       }
+      // restart: r = b - A64 x, p = M^-1 r, FP64 PCG from here on
+      hipLaunchKernelGGL(k_rel_force, dim3(1), dim3(64), 0, s, sc);
+      replace(it);
+      hipLaunchKernelGGL(k_pcg_restart, G, TB, 0, s, n, (const double*)V.r, V.m, V.bm, V.p,
+                         pr_next, sc);
+      replacing = false;
     }
     if (predict) {
       const SolveScalars& h = *c->scal_host;
@@ -557,7 +591,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
       it_prev = h.iter;
     }
   }
-  if (!done && reliable) {  // maxit: leave the true residual of the final x
+  if (!done && replacing) {  // maxit: leave the true residual of the final x
     hipLaunchKernelGGL(k_rel_force, dim3(1), dim3(64), 0, s, sc);
     replace(it);
     ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->scal_host, sc, sizeof(SolveScalars),

@@ -144,6 +144,14 @@ void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epar
     if (fp32 && c->accum32) spmv_blocks_launch<float, float, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0, x32);
     else if (fp32) spmv_blocks_launch<float, double, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0, nullptr);
     else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr);
+  } else if (c->gather) {
+    const DevGather& g = c->gth;
+    if (fp32)
+      launch_spmv_gather<float>(c->neq, g.ia, g.ja, c->A32.ad, c->A32.al, g.colptr, g.colk,
+                                g.colrow, x, y, s, skip);
+    else
+      launch_spmv_gather<double>(c->neq, g.ia, g.ja, c->A64.ad, c->A64.al, g.colptr, g.colk,
+                                 g.colrow, x, y, s, skip);
   } else {
     if (fp32)
       launch_spmv_tiles<float, double>(c->tiles, c->A32.ad, c->A32.al, x, y, epart, s, skip);

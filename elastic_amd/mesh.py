@@ -295,6 +295,31 @@ def tria3_square(n: int, E: float = 1.0, nu: float = 0.3, traction=(0.0, -1.0),
     return _square(n, True, E, nu, traction, edge, plane_stress, thickness)
 
 
+def tria3_fan(nring: int, E: float = 1.0, nu: float = 0.3, force=(0.3, -1.0)) -> Mesh:
+    """A high-valence mesh: one hub node joined to a ring of `nring` nodes by
+    `nring` tria3 elements (plane strain).  Three consecutive ring nodes are
+    clamped and the ring node opposite them carries a nodal force.  RCM puts
+    the hub near the end, so its CSRC row holds about 2 x nring lower
+    entries (nring lower node blocks)."""
+    t = 2.0 * np.pi * np.arange(nring) / nring
+    x = np.concatenate([[[0.0, 0.0]], np.stack([np.cos(t), np.sin(t)], 1)]).astype(np.float64)
+    k = np.arange(nring)
+    conn = np.stack([np.ones(nring, np.int64), 2 + k, 2 + (k + 1) % nring], 1)
+    ix = np.empty((nring, 4), np.int32)
+    ix[:, :3] = conn
+    ix[:, 3] = 1
+    nnode = nring + 1
+    idf = np.zeros((nnode, 2), np.int32)
+    idf[1:4] = 1
+    f = np.zeros((nnode, 2))
+    f[1 + nring // 2] = force
+    e = np.zeros((1, PROP))
+    e[0, 0], e[0, 1] = E, nu
+    return Mesh(nnode, nring, 3, 2, 2, ix, x, idf, np.array([IEL_TRIA3_STRAIN], np.int32), e,
+                np.zeros((nring, 7), np.int32), np.zeros((nnode, 2), np.int32), f, [],
+                name="tria3_fan_%d" % nring)
+
+
 # ------------------------------------------------------------------ workloads
 BASE_SEED = 20261202
 

@@ -275,8 +275,10 @@ def test_mixed_hands_over_to_fp64(monkeypatch, accum32):
     residual, which then falls by less than the guard's 10% per replacement,
     so after three replacements the solve continues with the FP64 operator
     (with FP32 sums the iteration's FP32 p and r are copied to their FP64
-    vectors first).  It still converges to the tight FP64 solution within
-    1e-8."""
+    vectors first), restarted from the true residual, as FP64 PCG with its
+    stopping rule.  It still converges to the tight FP64 solution within
+    1e-8, and its true residual meets the reference's final check
+    (3.16 tol ||b||_M, Iters_omp.f:254)."""
     from elastic_amd import mesh as M
     from elastic_amd.solver import FP64, MIXED
     from elastic_amd.system import System
@@ -288,7 +290,7 @@ def test_mixed_hands_over_to_fp64(monkeypatch, accum32):
     c.set_option("accum32", accum32)
     x, st = c.solve(s.b, precision=MIXED, tol=1e-10)
     assert 0 <= st["op64_from"] <= 20
-    assert st["norm_m_r"] < st["conv"]
+    assert st["norm_m_r"] < 3.16 * st["conv"]
     assert rel(x, xref) <= 1e-8
     c.close()
 
