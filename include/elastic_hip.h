@@ -211,6 +211,24 @@ int32_t elastic_solv_(int32_t* neq, int32_t* nad, int32_t* ip, int32_t* ja, doub
                       int32_t* neq_doti, int64_t* i_fmapi, int64_t* i_xfi, int64_t* i_rcvsi,
                       int64_t* i_dspli);
 
+/* No counterpart in the reference: y = (D + L + L^T) x (the product of
+ * elastic_spmv, Matvec.f:528-621) for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Every pointer is a device pointer to arrays the caller
+ * holds: ia(neq+1), ja(nad) (1-based CSRC pattern, which must be valid: the
+ * call does not check it), ad(neq), al(nad), x(neq), y(neq).  The work is
+ * enqueued on `stream` (a hipStream_t; NULL for the null stream) and the call
+ * returns at once.  No context, no hipMalloc/hipFree: the transposed pattern
+ * is built per call in a fixed 128 MiB device array of the library (one per
+ * GPU, held until the process ends), which bounds the size:
+ * 2 neq + 2 nad + neq/1024 + 2 <= 33,554,432 (ELASTIC_BAD_ARG beyond, with a
+ * message on stderr).  The array keeps the last call's contents; calls that
+ * overlap on one GPU (other streams or threads) race on it.  The FP64 result
+ * is the reference's matvec_csrcsym1 bit for bit. */
+int elastic_spmv_csrc_noalloc(int32_t neq, int64_t nad, const int32_t* ia, const int32_t* ja,
+                              const double* ad, const double* al, const double* x, double* y,
+                              void* stream);
+
 /* ------------------------------------------------ resident benchmarking */
 /* Uploads b to the device copy used by elastic_solve_resident. */
 int elastic_upload_rhs(elastic_ctx* ctx, const double* b);
