@@ -8,9 +8,11 @@ nodes jittered by U[-0.15h, 0.15h], E = 1, nu = 0.3, z=0 clamped, traction
 One step = one solve of that system with b and the matrix resident in HBM:
 FP32 CSRC SpMV/preconditioner inside FP64 iterative refinement (the measured
 method), timed between device synchronisations.  The plain FP64 GPU PCG is
-timed beside it, and the oracle's OpenMP FP64 PCG (the reference's pcg_omp
-restated) is timed on the host cores on a bounded number of iterations of the
-same system.
+timed beside it.  The CPU baseline is the reference's own OpenMP pcg_omp
+(compiled from its sources into oracle/_ref) solving the same system on the
+host cores; its solution is the FP64 oracle that both GPU results are
+compared with.  Where oracle/_ref is absent the C restatement of pcg_omp
+takes its place.
 
 Prints one JSON line (see the keys below).
 """
@@ -38,13 +40,33 @@ def spmv_bytes(neq: int, nad: int, sv: int, sx: int = 8) -> int:
     return 4 * (neq + 1) + 4 * nad + sv * nad + sv * neq + sx * neq + sx * neq
 
 
+def blocked_bytes(neq: int, nad: int, nb: int, ntiles: int, next_nodes: int, sv: int,
+                  sx: int) -> int:
+    """Bytes one product on the node-blocked layout must move: per off-diagonal
+    node block one 4-byte word and nb*nb sv-byte values; per node the
+    nb(nb+1)/2 diagonal-block values; x staged for the own rows and for every
+    (tile, external node) pair, y written for the own rows and one external
+    partial per (tile, external node) pair (sx bytes per entry, nb entries per
+    node); per tile its share of x.(Ax) (8 bytes), its 36 bytes of descriptors,
+    its rowptr/segptr (2 bytes per row and per column, plus one each) and its
+    external node list (4 bytes per node)."""
+    nn = neq // nb
+    nblk = (nad - nn * nb * (nb - 1) // 2) // (nb * nb)
+    return (nblk * (4 + sv * nb * nb) + nn * sv * nb * (nb + 1) // 2
+            + sx * neq + sx * nb * next_nodes          # x: own rows, external nodes
+            + sx * neq + sx * nb * next_nodes          # y and the external partials
+            + 8 * ntiles + 36 * ntiles + 2 * (nn + ntiles) + 2 * (nn + next_nodes + ntiles)
+            + 4 * next_nodes)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=100, help="cells per cube edge")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="length of the one-core sample of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving", type=int, default=512,
                     help="requests of the plane-strain serving stream (0: skip)")
@@ -147,16 +169,29 @@ def main():
     # with FP32 sums the operator stages x from an FP32 copy and writes its
     # rows in FP32: x and y count 4 bytes per entry then, 8 otherwise
     b32, b64 = spmv_bytes(neq, nad, 4, 4 if acc32 else 8), spmv_bytes(neq, nad, 8)
-    gbs32 = b32 / (live32_ms * 1e-3) / 1e9
-    gbs64 = b64 / (live64_ms * 1e-3) / 1e9
+    info = ctx.layout_info()
+    nb = info["nb"]
+    if nb > 1:  # the bytes the node-blocked kernel must move (the roofline's)
+        m32 = blocked_bytes(neq, nad, nb, info["ntiles"], info["ext_partials"], 4,
+                            4 if acc32 else 8)
+        m64 = blocked_bytes(neq, nad, nb, info["ntiles"], info["ext_partials"], 8, 8)
+    else:
+        m32, m64 = b32, b64
+    gbs32 = m32 / (live32_ms * 1e-3) / 1e9
+    gbs64 = m64 / (live64_ms * 1e-3) / 1e9
     t_asm = sysm.time_assembly(reps=3)
-    nb = ctx.layout_info()["nb"]
     copy = copy_probe(torch)
 
     phase("product, assembly and copy timed")
-    cpu = None
+    cpu, x_or = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(sysm, st64["iters"], args.cpu_seconds)
+        cpu, x_or = cpu_baseline(sysm, st64["iters"], args.cpu_seconds)
+    oracle_err = None
+    if x_or is not None:
+        nx = float(np.linalg.norm(x_or))
+        oracle_err = {"mixed": float(np.linalg.norm(xm - x_or)) / nx,
+                      "fp64": float(np.linalg.norm(x64 - x_or)) / nx,
+                      "oracle": cpu["kind"] + " pcg_omp FP64 solve to %g" % TOL}
 
     phase("cpu baseline done")
     serving = None
@@ -193,6 +228,7 @@ def main():
         "latency_ms_p50": 1e3 * float(np.percentile(lat, 50)),
         "latency_ms_p99": 1e3 * float(np.percentile(lat, 99)),
         "rel_err_vs_fp64": err_m,
+        "rel_err_vs_oracle": oracle_err,
         "iters_mixed": stm["iters"], "refinements": stm["outer"],
         "fp64": {"dof_per_s": neq / float(np.mean(lat64)), "iters": st64["iters"],
                  "latency_ms_p50": 1e3 * float(np.percentile(lat64, 50)),
@@ -202,13 +238,19 @@ def main():
                           "trial_ms": {("fp32" if k else "fp64"): 1e3 * v for k, v in accum.items()}},
         "spmv": {"fp32_kernel_ms": live32_ms, "fp32_launches_timed": live32_n, "fp32_GBs": gbs32,
                  "fp64_kernel_ms": live64_ms, "fp64_launches_timed": live64_n, "fp64_GBs": gbs64,
-                 "bytes_fp32": b32, "bytes_fp64": b64,
+                 "bytes_fp32": m32, "bytes_fp64": m64,
+                 "csrc_bytes_fp32": b32, "csrc_bytes_fp64": b64,
+                 "csrc_equivalent_GBs": {"fp32": b32 / (live32_ms * 1e-3) / 1e9,
+                                         "fp64": b64 / (live64_ms * 1e-3) / 1e9},
+                 "layout": {"nb": nb, "ntiles": info["ntiles"],
+                            "external_nodes": info["ext_partials"]},
                  "standalone": {"fp32_kernel_ms": ker32, "fp32_with_sum_and_merge_ms": tot32,
                                 "fp64_kernel_ms": ker64, "fp64_with_sum_and_merge_ms": tot64}},
         "assembly_ms": t_asm,
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": measured_traffic(neq, nad),
+                     "frac": gbs32 / HBM_PEAK_GBS, "bytes_per_launch": m32,
+                     "traffic": measured_traffic(neq, nad, m32),
                      "kernel": ("k_spmv_blocks<float,%s,%d> with tile shares of p.q"
                                 % ("float" if acc32 else "double", nb))
                                if nb > 1 else "k_spmv_tiles<float,double>"},
@@ -336,16 +378,21 @@ def torch_sync():
     torch.cuda.synchronize()
 
 
-def measured_traffic(neq: int, nad: int):
+def measured_traffic(neq: int, nad: int, model_bytes: int):
     """HBM bytes per launch of the FP32 operator (FETCH_SIZE and WRITE_SIZE
     counter runs scaled by the copy-probe calibration), recorded for this
-    workload in profiles/spmv_traffic_hex8_100.json; None for other sizes."""
+    workload in profiles/spmv_traffic_hex8_100.json together with the model
+    bytes of the layout they were measured on.  None for another workload or
+    when the live layout's model bytes differ from the recorded ones (the
+    counters would describe another kernel)."""
     path = os.path.join(ROOT, "profiles", "spmv_traffic_hex8_100.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
     if ("neq %d" % neq) not in d["workload"] or ("nad %d" % nad) not in d["workload"]:
+        return None
+    if d["fp32"].get("model_bytes") != model_bytes:
         return None
     return d["fp32"]["read_bytes"] + d["fp32"]["write_bytes"]
 
@@ -371,39 +418,110 @@ def copy_probe(torch):
     return {"bytes": nbytes, "ms": ms, "GBs": nbytes / (ms * 1e-3) / 1e9}
 
 
+def host_cpu():
+    """Host processor name and the number of CPUs the OS reports."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
+def reference_solve(sysm, ad, al, nth: int, tol: float, maxit: int):
+    """The reference's own solv -> pcg_omp (nth > 1) or pcg (nth = 1),
+    compiled from its sources into oracle/_ref/ref_harness, on this system
+    (written to a scratch directory): (solution, solv seconds, iterations),
+    or None when the harness is absent or fails."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    d = tempfile.mkdtemp(prefix="elastic_ref_")
+    try:
+        sysd, outd = os.path.join(d, "sys"), os.path.join(d, "out")
+        os.makedirs(sysd)
+        os.makedirs(outd)
+        with open(os.path.join(sysd, "sys.txt"), "w") as fh:
+            fh.write("%d %d\n" % (sysm.neq, sysm.nad))
+        sysm.ia.astype(np.int32).tofile(os.path.join(sysd, "ia.i4"))
+        sysm.ja.astype(np.int32).tofile(os.path.join(sysd, "ja.i4"))
+        ad.tofile(os.path.join(sysd, "ad.f8"))
+        al.tofile(os.path.join(sysd, "al.f8"))
+        sysm.b.tofile(os.path.join(sysd, "b.f8"))
+        # pool words: the system, 7 vectors, nth per-thread y buffers, slack
+        words = sysm.nad * 3 + sysm.neq * (2 * (7 + nth) + 8) + (1 << 24)
+        env = dict(os.environ, OMP_NUM_THREADS=str(nth))
+        r = subprocess.run([exe, sysd, outd, repr(tol), str(maxit), str(nth), str(128 | 1),
+                            str(words)], capture_output=True, text=True, env=env,
+                           timeout=900)
+        if r.returncode != 0 or not os.path.exists(os.path.join(outd, "x.f8")):
+            return None
+        meta = open(os.path.join(outd, "meta.txt")).read().split()
+        m = re.search(r"it\s+(\d+)", open(os.path.join(outd, "solver.log")).read())
+        x = np.fromfile(os.path.join(outd, "x.f8"), dtype=np.float64)
+        return x, float(meta[3]), int(m.group(1)) if m else -1
+    except (OSError, subprocess.SubprocessError, ValueError):
+        return None
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def cpu_baseline(sysm, iters_fp64: int, seconds: float):
-    """The oracle's OpenMP PCG (Iters_omp.f restated) on the same system, on the
-    host cores: the whole solve when it takes at most about 2 x `seconds`,
-    else a bounded number of iterations extrapolated with the FP64 GPU
-    iteration count."""
+    """The reference's CPU path on the same system, on the host cores: its
+    own OpenMP pcg_omp (oracle/_ref) solving to 1e-10 with the threads the
+    host gives this GPU (OMP_NUM_THREADS, else every CPU), and its serial pcg
+    restated (the oracle's or_pcg, bit-identical to the reference's) timed on
+    one core over a bounded number of iterations.  Returns (baseline, the
+    reference's solution x or None).  Without oracle/_ref the C restatement
+    of pcg_omp stands in ("port")."""
     from oracle import binding as O
 
-    nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    model, ncpu = host_cpu()
     ad, al = sysm.values()
-    m = 1.0 / ad
     b = sysm.b
-    # calibrate on 20 iterations; run the whole solve when it fits in about
-    # twice `seconds`, else a bounded sample extrapolated to the full count
-    a = time.perf_counter()
-    O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=1e-30, maxit=20, m=m, nth=nth)
-    t20 = (time.perf_counter() - a) / 20.0
-    full = t20 * iters_fp64 <= 2.0 * seconds
-    k = iters_fp64 + 100 if full else max(5, int(seconds / max(t20, 1e-6)))
-    a = time.perf_counter()
-    _, st = O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=TOL if full else 1e-30, maxit=k, m=m, nth=nth)
-    t = time.perf_counter() - a
-    k = st["iters"] if full else k
-    t_it = t / k
-    if full:
-        sample = ("the whole FP64 solve to %g (%d iterations; OpenMP pcg_omp restatement, "
-                  "%d threads, %.1f s)" % (TOL, k, nth, t))
-        value = sysm.neq / t
+    x_ref = None
+    ref = reference_solve(sysm, ad, al, nth, TOL, 20000)
+    if ref is not None:
+        x_ref, t, k = ref
+        kind = "reference"
+        sample = ("the whole FP64 solve to %g by the reference's own pcg_omp (Iters_omp.f, "
+                  "built from its sources), %d iterations, %d threads, %.1f s" % (TOL, k, nth, t))
     else:
-        sample = ("%d of the %d FP64 PCG iterations (OpenMP pcg_omp restatement, %d threads, "
-                  "%.1f s), extrapolated to a full solve" % (k, iters_fp64, nth, t))
-        value = sysm.neq / (t_it * iters_fp64)
-    return {"value": value, "unit": "DOF/s", "cores": nth, "kind": "port", "sample": sample,
-            "s_per_iter": t_it}
+        m = 1.0 / ad
+        a = time.perf_counter()
+        x_ref, st = O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=TOL, maxit=20000, m=m, nth=nth)
+        t = time.perf_counter() - a
+        k = st["iters"]
+        kind = "port"
+        sample = ("the whole FP64 solve to %g by the C restatement of pcg_omp, %d iterations, "
+                  "%d threads, %.1f s" % (TOL, k, nth, t))
+    # one core: the serial pcg (Iters.f:252-524) restated bit for bit, a
+    # sample of about `seconds` of iterations extrapolated to the FP64
+    # iteration count
+    a = time.perf_counter()
+    O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=1e-30, maxit=2, m=1.0 / ad, nth=0)
+    ks = int(min(200, max(5, seconds / ((time.perf_counter() - a) / 2.0))))
+    a = time.perf_counter()
+    O.pcg(sysm.ia, sysm.ja, ad, al, b, tol=1e-30, maxit=ks, m=1.0 / ad, nth=0)
+    t1 = (time.perf_counter() - a) / ks
+    base = {"value": sysm.neq / t, "unit": "DOF/s", "cores": nth, "kind": kind,
+            "sample": sample, "s_per_iter": t / max(k, 1), "iters": k,
+            "host": {"cpu": model, "nproc": ncpu, "threads_used": nth},
+            "serial_1core": {"value": sysm.neq / (t1 * iters_fp64), "unit": "DOF/s",
+                             "s_per_iter": t1, "kind": "port (or_pcg, bit-identical to pcg)",
+                             "sample": "%d iterations on one core, extrapolated to the %d "
+                                       "iterations of the FP64 solve" % (ks, iters_fp64)}}
+    return base, x_ref
 
 
 if __name__ == "__main__":

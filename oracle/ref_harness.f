@@ -19,6 +19,12 @@ c *             result files of write_mesh_res_mec (Write_vtk.f:742):  *
 c *             legacy ASCII, legacy binary and XML ASCII              *
 c *   mode 64 = set-up only: stop after the set-up dumps and the     *
 c *             product (no solve; for the full-size workloads)        *
+c *   mode 128 = solve only: `deck` is a directory holding a CSRC      *
+c *             system (sys.txt: neq nad; ia.i4 ja.i4 ad.f8 al.f8      *
+c *             b.f8), solved by solv (Solv.f:13) with nth threads;    *
+c *             writes meta.txt (neq nad nth / solv time), solver.log  *
+c *             and x.f8 (the reference's CPU baseline on a system     *
+c *             assembled elsewhere)                                   *
 c *   (modes add up)                                                   *
 c * Output (raw little-endian stream files in outdir):                 *
 c *   meta.txt, inum.i4, id.i4, ia.i4, ja.i4, ad.f8, al.f8, b.f8,      *
@@ -98,6 +104,46 @@ c ... arguments
       nth_solv = max(nth,1)
 c ... memory pool and deck (Main.f:300-330)
       call init_malloc()
+      if (iand(mode,128) .ne. 0) then
+c ... solve only, timed as soltime is (Main.f:819-827)
+        open(21,file=trim(deck)//'/sys.txt',status='old')
+        read(21,*) neq,nad
+        close(21)
+        i_ia = alloc_4('ia      ',1,neq+1)
+        i_ja = alloc_4('ja      ',1,nad)
+        i_ad = alloc_8('ad      ',1,neq)
+        i_al = alloc_8('al      ',1,nad)
+        i_b  = alloc_8('b       ',1,neq)
+        i_m  = alloc_8('m       ',1,neq)
+        i_x0 = alloc_8('x0      ',1,neq)
+        i_fmap = alloc_4('fmap    ',1,1)
+        i_xf   = alloc_8('xf      ',1,1)
+        i_rcvs = alloc_4('rcvs    ',1,1)
+        i_dspl = alloc_4('dspl    ',1,1)
+        call ri4(trim(deck)//'/ia.i4',ia(i_ia),neq+1)
+        call ri4(trim(deck)//'/ja.i4',ia(i_ja),nad)
+        call rf8(trim(deck)//'/ad.f8',ia(i_ad),neq)
+        call rf8(trim(deck)//'/al.f8',ia(i_al),nad)
+        call rf8(trim(deck)//'/b.f8',ia(i_b),neq)
+        call azero(ia(i_x0),neq)
+        neqf1 = 0
+        neqf2 = 0
+        neq3 = 0
+        neq4 = 0
+        neq_dot = neq
+        open(10,file=trim(outd)//'/solver.log')
+        tt = MPI_Wtime()
+        call solv(neq,nad,ia(i_ia),ia(i_ja),ia(i_ad),ia(i_al),
+     .            ia(i_al),ia(i_m),ia(i_b),ia(i_x0),solvtol,maxit,50,
+     .            .false.,1,neqf1,neqf2,neq3,neq4,neq_dot,i_fmap,i_xf,
+     .            i_rcvs,i_dspl)
+        tt = MPI_Wtime() - tt
+        open(20,file=trim(outd)//'/meta.txt')
+        write(20,*) neq,nad,nth
+        write(20,*) tt
+        call wf8(trim(outd)//'/x.f8',ia(i_x0),neq)
+        goto 900
+      endif
       nin = 1
       open(nin,file=deck,status='old',action='read')
    10 continue
@@ -338,6 +384,29 @@ c **********************************************************************
       open(30,file=fname,access='stream',form='unformatted',
      .     status='replace')
       write(30) a
+      close(30)
+      return
+      end
+c **********************************************************************
+      subroutine ri4(fname,a,n)
+      implicit none
+      character*(*) fname
+      integer n,a(n)
+      open(30,file=fname,access='stream',form='unformatted',
+     .     status='old')
+      read(30) a
+      close(30)
+      return
+      end
+c **********************************************************************
+      subroutine rf8(fname,a,n)
+      implicit none
+      character*(*) fname
+      integer n
+      real*8 a(n)
+      open(30,file=fname,access='stream',form='unformatted',
+     .     status='old')
+      read(30) a
       close(30)
       return
       end
