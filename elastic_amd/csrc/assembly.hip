@@ -1,15 +1,17 @@
 // Element stiffness and load assembly on gfx950 (Pform.f:1-254, Assbly.f:66-146,
 // Elmlib02.f elmt02..07, Elmlib.f shape functions and Jacobians).
 //
-// Stiffness (isw = 2): one 64-lane wavefront per element, one launch per
-// colour.  Every lane evaluates the Jacobian of each Gauss point itself (the
-// same operations in the same order as the reference, FP contraction off), so
-// the element matrix s(nst,nst) is bit-identical to the reference's; each lane
-// accumulates its share of the nst*nst entries over the Gauss points, the
-// matrix goes to LDS, p = s u is formed in the reference's order
-// (Elmlib.f:336-369), and the entries are added to ad/al/b through the
-// element's slot map with plain read-modify-writes (elements of one colour
-// share no node).
+// Stiffness (isw = 2), one launch per colour.  hex8 with u = 0 (the
+// assembly of K: b - K u = b) takes k_stiffness_hex8: 16 elements per
+// workgroup, Gauss-point geometry formed once and shared in LDS, one lane per
+// (element, node pair, row) forming only the entries the CSRC stores.  The
+// other cases take k_stiffness: one 64-lane wavefront per element, every
+// lane evaluating the Jacobian of each Gauss point itself, the element matrix
+// s(nst,nst) in LDS and p = s u in the reference's order (Elmlib.f:336-369).
+// Either way the operations and their order are the reference's (FP
+// contraction off), so every entry is bit-identical to its element matrix,
+// and the entries are added to ad/al/b through the element's slot map with
+// plain read-modify-writes (elements of one colour share no node).
 // Loads (isw = 4): one lane per element of a colour, face/edge tractions and
 // self weight as Elmlib02.f computes them.
 #include <cmath>
@@ -393,12 +395,122 @@ __global__ __launch_bounds__(64) void k_stiffness(AsmArgs a, int c0, int lhs, in
       const int k = -t - 2;
       a.ad[k] = a.ad[k] + s[i * NST + i];
     } else {
-      const bool rev = (t >> 30) & 1;
-      const int64_t slot = t & ((1 << 30) - 1);
-      const double v = rev ? s[i * NST + j] : s[j * NST + i];  // s(j,i) : s(i,j)
-      a.al[slot] = a.al[slot] + v;
+      // the row is the dof with the larger equation number (Assbly.f:116-143)
+      const double v = ldv[i] > ldv[j] ? s[j * NST + i] : s[i * NST + j];  // s(i,j) : s(j,i)
+      a.al[t] = a.al[t] + v;
     }
   }
+}
+
+// Stiffness of hex8 elements (elmt07_mec isw 2, Elmlib02.f:1691-1785), kHexE
+// elements per 256-lane workgroup.  The shape-function derivatives and the
+// weight det * a of each (element, Gauss point) are formed once, by one lane
+// (sfhexa8_m + jacob3d_m, Elmlib.f:1151-1259, 1942-2037), and kept in LDS.
+// Each lane then forms the three entries (r, 0..2) of one node pair (na >= nb)
+// of one element that the CSRC stores: for every pair of dofs only the entry
+// of the row with the larger equation number, s(i,j) or s(j,i), the way
+// csr (Assbly.f:116-143) picks it, summed over the Gauss points in the
+// reference's order with its operations (FP contraction off), so every value
+// is bit-identical to the reference's element matrix.  The sums go to ad/al
+// through the element's slot map (elements of one colour share no node).
+// b -= s u needs the whole matrix and is left to k_stiffness.
+constexpr int kHexE = 16;
+constexpr int kHexSub = 108;  // (na >= nb) node pairs x 3 rows
+
+__global__ __launch_bounds__(256) void k_stiffness_hex8(AsmArgs a, int c0, int cnt) {
+  constexpr int NEN = 8, NST = 24, NPAIR = NST * (NST + 1) / 2;
+  __shared__ double dX[kHexE][8][3][NEN];  // derivative d of node n at Gauss point g
+  __shared__ double wt[kHexE][8];          // det * a at Gauss point g
+  __shared__ double xl[kHexE][NEN * 3];
+  __shared__ double cbs[kHexE], ccs[kHexE], cas[kHexE];
+  __shared__ int32_t ldv[kHexE][NST];
+  __shared__ int32_t elno[kHexE];
+  __shared__ int bad[kHexE];
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * kHexE;
+  const int ne = min(kHexE, cnt - e0);
+  // element data: coordinates and equation numbers
+  for (int q = tid; q < ne * NST; q += 256) {
+    const int e = q / NST, k = q - e * NST;
+    const int el = a.order[c0 + e0 + e];
+    const int32_t* ixe = a.ix + (size_t)el * (NEN + 1);
+    const size_t g = (size_t)(ixe[k / 3] - 1) * 3 + k % 3;
+    xl[e][k] = a.x[g];
+    ldv[e][k] = a.id[g];
+    if (k == 0) {
+      elno[e] = el;
+      bad[e] = 0;
+      const double* em = a.e + (size_t)(ixe[NEN] - 1) * 15;
+      const double ym = em[0], ps = em[1];
+      const double a1 = 1.0 - ps, a2 = 1.0 - 2.0 * ps, a3 = 1.0 + ps;
+      cas[e] = (ym * a1) / (a3 * a2);
+      cbs[e] = ps / a1;
+      ccs[e] = 0.5 * (a2 / a1);
+    }
+  }
+  __syncthreads();
+  // one lane per (element, Gauss point): derivatives and weight
+  if (tid < ne * 8) {
+    const int e = tid >> 3, gp = tid & 7;
+    const int lx = gp & 1, ly = (gp >> 1) & 1, lz = gp >> 2;
+    double h[NEN], hx[NEN], hy[NEN], hz[NEN], det;
+    sfhexa8_d(kPG2[lx], kPG2[ly], kPG2[lz], h, hx, hy, hz);
+    if (!jacob3d_d(hx, hy, hz, xl[e], NEN, &det)) bad[e] = 1;
+    const double w = 1.0 * 1.0 * 1.0 * det;
+    wt[e][gp] = w * cas[e];
+#pragma unroll
+    for (int n = 0; n < NEN; ++n) {
+      dX[e][gp][0][n] = hx[n];
+      dX[e][gp][1][n] = hy[n];
+      dX[e][gp][2][n] = hz[n];
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < ne * kHexSub; q += 256) {
+    const int e = q / kHexSub, sub = q - e * kHexSub;
+    if (bad[e]) continue;
+    const int pr = sub / 3, r = sub - 3 * pr;  // node pair pr, row r of na
+    int na = (int)((sqrtf(8.0f * pr + 1.0f) - 1.0f) * 0.5f);  // pr = na(na+1)/2 + nb
+    while ((na + 1) * (na + 2) / 2 <= pr) ++na;
+    while (na * (na + 1) / 2 > pr) --na;
+    const int nb = pr - na * (na + 1) / 2;
+    const int i = 3 * na + r;
+    const int el = elno[e];
+    const int32_t* sl = a.slots + (size_t)el * NPAIR + i * (i + 1) / 2 + 3 * nb;
+    const int cmax = na == nb ? r : 2;  // lower triangle: j = 3 nb + c <= i
+    int32_t t[3];
+    bool up[3];  // the entry is s(j,i): the row is dof j
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      t[c] = c <= cmax ? sl[c] : -1;
+      up[c] = ldv[e][i] < ldv[e][3 * nb + c];
+    }
+    if (t[0] == -1 && t[1] == -1 && t[2] == -1) continue;
+    const double cb = cbs[e], cc = ccs[e];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int gp = 0; gp < 8; ++gp) {
+      const double xi = dX[e][gp][0][na], yi = dX[e][gp][1][na], zi = dX[e][gp][2][na];
+      const double xj = dX[e][gp][0][nb], yj = dX[e][gp][1][nb], zj = dX[e][gp][2][nb];
+      const double w1 = wt[e][gp];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (t[c] == -1) continue;
+        acc[c] = acc[c] + (up[c] ? term3(true, c, r, xj, yj, zj, xi, yi, zi, cb, cc, w1)
+                                 : term3(true, r, c, xi, yi, zi, xj, yj, zj, cb, cc, w1));
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (t[c] == -1) continue;
+      if (t[c] <= -2) {
+        const int k = -t[c] - 2;
+        a.ad[k] = a.ad[k] + acc[c];
+      } else {
+        a.al[t[c]] = a.al[t[c]] + acc[c];
+      }
+    }
+  }
+  if (tid < ne && bad[tid]) a.err[0] = elno[tid] + 1;  // any offending element is reported
 }
 
 // ---------------------------------------------------------- loads (isw = 4)
@@ -587,6 +699,11 @@ void free_mesh(MeshDev* m) {
   delete m;
 }
 
+static bool getenv_flag(const char* name) {
+  const char* v = getenv(name);
+  return v && v[0] == '1';
+}
+
 template <typename T>
 static hipError_t up(T** d, const T* h, size_t n, hipStream_t s) {
   hipError_t e = hipMalloc((void**)d, sizeof(T) * std::max<size_t>(n, 1));
@@ -607,9 +724,17 @@ static AsmArgs make_args(Context* c, MeshDev* m) {
 
 template <int KIND>
 static void launch_stiffness(const AsmArgs& a, const MeshDev* m, int lhs, int rhs, hipStream_t s) {
+  // hex8 stiffness with u = 0 (b - K u = b): the pair kernel
+  const bool pairs = KIND == K_HEX8 && lhs && (!rhs || a.u == nullptr) &&
+                     !getenv_flag("ELASTIC_ASM_WAVE");
   for (int c = 0; c < m->ncolors; ++c) {
     int c0 = m->color_start[c], cnt = m->color_start[c + 1] - c0;
-    if (cnt > 0) hipLaunchKernelGGL(k_stiffness<KIND>, dim3(cnt), dim3(64), 0, s, a, c0, lhs, rhs);
+    if (cnt <= 0) continue;
+    if (pairs)
+      hipLaunchKernelGGL(k_stiffness_hex8, dim3((cnt + kHexE - 1) / kHexE), dim3(256), 0, s, a,
+                         c0, cnt);
+    else
+      hipLaunchKernelGGL(k_stiffness<KIND>, dim3(cnt), dim3(64), 0, s, a, c0, lhs, rhs);
   }
 }
 
@@ -1221,9 +1346,7 @@ __global__ __launch_bounds__(64) void k_j2_hex8(AsmArgs a, J2Dev j, int c0, int 
       const int k = -t - 2;
       a.ad[k] = a.ad[k] + s[i * NST + i];
     } else {
-      const bool rev = (t >> 30) & 1;
-      const int64_t slot = t & ((1 << 30) - 1);
-      a.al[slot] = a.al[slot] + (rev ? s[i * NST + jx] : s[jx * NST + i]);
+      a.al[t] = a.al[t] + (ldv[i] > ldv[jx] ? s[jx * NST + i] : s[i * NST + jx]);
     }
   }
 }

@@ -13,9 +13,11 @@
 // {i >= j} (nst(nst+1)/2 pairs) one int32 target:
 //   -1                 no equation (restrained dof)
 //   <= -2              diagonal: ad index = -t - 2
-//   >= 0               al index | (1 << 30) when the lower entry is s(j,i)
+//   >= 0               al index (0 .. nad-1; nad < 2^31, as the reference's
+//                      int32 ja allows)
 // (al is filled with s(row, col) for the local dof whose equation is the row,
-// as Assbly.f:116-143 does).
+// as Assbly.f:116-143 does: s(i,j) when eq(i) > eq(j), else s(j,i); the
+// kernels read the equation numbers, so the slot carries no orientation).
 #include <algorithm>
 #include <stdexcept>
 #include <thread>
@@ -106,10 +108,9 @@ void element_slots(const int32_t* ix, const int32_t* id, int numel, int nen, int
             const int32_t* en = ja1 + ia1[row] - 1;
             const int32_t* f = std::lower_bound(b, en, col);
             if (f == en || *f != col) continue;  // not in the pattern (cannot happen)
-            int64_t slot = (int64_t)(f - ja1);
-            if (slot >= (1ll << 30)) throw std::runtime_error("al too large for slot maps");
-            // the row's local dof is i when ki > kj (entry s(i,j)), else j (s(j,i))
-            out[p] = (int32_t)slot | (ki > kj ? 0 : (1 << 30));
+            const int64_t slot = (int64_t)(f - ja1);
+            if (slot > INT32_MAX) throw std::runtime_error("al too large for slot maps");
+            out[p] = (int32_t)slot;
           }
       }
     });
