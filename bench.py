@@ -70,8 +70,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving", type=int, default=512,
                     help="requests of the plane-strain serving stream (0: skip)")
-    ap.add_argument("--serving-mixed", action="store_true",
-                    help="also serve the stream with mixed-precision instances")
+    ap.add_argument("--serving-fixed", action="store_true",
+                    help="also serve the stream with fixed FP64 and fixed mixed-precision "
+                         "instances (the default stream picks per request)")
     ap.add_argument("--serving-compare", type=int, default=48,
                     help="requests in the solve-only batched vs sequential comparison")
     ap.add_argument("--plastic-compare", type=int, default=32,
@@ -198,7 +199,7 @@ def main():
     if args.serving > 0 or args.serving_compare > 0:
         ctx.close()
         del sysm, ctx
-        serving = serving_bench(args.serving, args.serving_compare, args.serving_mixed)
+        serving = serving_bench(args.serving, args.serving_compare, args.serving_fixed)
     plastic = None
     if args.plastic_compare > 0:
         phase("serving done")
@@ -262,23 +263,27 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def serving_bench(nreq: int, ncompare: int, stream_mixed: bool = False):
+def serving_bench(nreq: int, ncompare: int, fixed: bool = False):
     """The plane-strain request stream (half quad4, half tria3, side U{32..512},
     Poisson arrivals at 200 requests/s) served by the batch with continuous
-    admission, with FP64 instances (and with mixed-precision ones when asked):
-    throughput and p50/p99 latency from arrival to x on the host (host set-up
-    and GPU assembly included).  Then the solve-only comparison on the first
-    `ncompare` requests with every system resident: plain FP64 PCG solved one
-    after the other (the drop-in path) against the batch, FP64 and mixed."""
-    from elastic_amd.serving import Batch, MeshRequest, Server, request_stream
+    admission, precision and preconditioner chosen per request
+    (serving.request_policy; with `fixed` also all-FP64 and all-mixed
+    streams): throughput and p50/p99 latency from arrival, from admission
+    and from submission to x on the host (host set-up and GPU assembly
+    included).  Then the solve-only comparison on the first `ncompare`
+    requests with every system resident: plain FP64 PCG solved one after the
+    other (the drop-in path) against the batch with FP64, mixed and policy
+    instances."""
+    from elastic_amd.serving import Batch, MeshRequest, Server, request_policy, request_stream
     from elastic_amd.solver import FP64, MIXED
     from elastic_amd.system import System
 
     workers = min(16, os.cpu_count() or 1)
     out = {"workload": ("%d plane-strain requests (even quad4, odd tria3, side U{32..512}), "
                         "Poisson arrivals 200/s, to %g" % (nreq, TOL))}
-    streams = [("stream_batched_fp64", FP64)] + ([("stream_batched_mixed", MIXED)]
-                                                   if stream_mixed else [])
+    streams = [("stream_batched_policy", None)] + ([("stream_batched_fp64", FP64),
+                                                     ("stream_batched_mixed", MIXED)]
+                                                    if fixed else [])
     for name, prec in streams:
         if nreq <= 0:
             break
@@ -312,13 +317,19 @@ def serving_bench(nreq: int, ncompare: int, stream_mixed: bool = False):
     x64 = [s_.ctx.download_x() for s_ in systems]
     compare = {"requests": ncompare, "dof": dof,
                "sequential_fp64": {"s": t_seq, "dof_per_s": dof / t_seq, "iters": it64}}
-    for name, prec in (("batched_fp64", FP64), ("batched_mixed", MIXED)):
+    policy = [request_policy(s_.ctx) for s_ in systems]
+    for name, prec in (("batched_fp64", FP64), ("batched_mixed", MIXED),
+                       ("batched_policy", None)):
+        for s_, (pp, pc) in zip(systems, policy):
+            s_.ctx.set_option("batch_precond", pc if prec is None else 2)
         a = time.perf_counter()
         done, itb = {}, 0
         with Batch(0, max_slots=64, iters_per_step=16) as b:
             queue = list(range(len(systems)))
             while len(done) < len(systems):
-                while queue and b.submit(systems[queue[0]].ctx, prec, TOL, 50000, queue[0]):
+                while queue and b.submit(systems[queue[0]].ctx,
+                                         policy[queue[0]][0] if prec is None else prec, TOL,
+                                         50000, queue[0]):
                     queue.pop(0)
                 b.step()
                 for tag, st in b.poll():

@@ -86,6 +86,7 @@ def lib():
     L.elastic_autotune.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                    C.POINTER(C.c_double)]
     L.elastic_get_bdiag.argtypes = [vp, F64P]
+    L.elastic_get_block_precond.argtypes = [vp, C.c_int32, F64P]
     L.elastic_spmv_csrc_noalloc.argtypes = [C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
     L.elastic_set_mesh.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                    C.c_int32, I32P, F64P, I32P, I32P, F64P, I32P, C.c_int32, I32P,

@@ -16,7 +16,9 @@
 // tiles, its chunks and the order of its reductions (per-chunk partials summed
 // in chunk order) are functions of its own size, so a result is bit for bit
 // the same alone and in any mix.  The method per instance is the PCG of
-// pcg.hip (Iters.f:252-524, Jacobi, x0 = 0, M-norm stopping rule), in FP64
+// pcg.hip (Iters.f:252-524, x0 = 0, M-norm stopping rule) with Jacobi or,
+// when its context asks for it ("batch_precond" 6), the node-block Jacobi
+// of block_precond / op_block_precond (2x2 in 2-D, 3x3 in 3-D), in FP64
 // or with the FP32 operator (FP32 values; staging and sums in FP64, or in
 // FP32 when the context asks for it) and FP64 true-residual replacement.
 #include <algorithm>
@@ -29,6 +31,8 @@
 #include "spmv_block.h"
 
 namespace elastic {
+
+int ensure_bdiag(elastic_ctx* x, int bsz);  // krylov.hip
 
 constexpr int kChunk = 2048;         // rows per vector work item at most (256 lanes x 8)
 constexpr int kMaxSlots = 255;       // slot index lives in the top 8 bits of a work item
@@ -51,6 +55,9 @@ struct BInst {
   // only happens there) continues from current FP64 vectors.
   float *p32, *r32, *m32;
   const double* m;
+  // node-block Jacobi (null: Jacobi with m): per node the nb(nb+1)/2 values
+  // of block_precond's inverse (2x2: a11 a22 a21, 3x3: c11 c22 c33 c21 c31 c32)
+  const double* bm;
   double* part;   // 5 * nch partials: (unused), r.Mr, true-residual r.Mr, p.r, (Mr).q
   double* dslot;  // 2 ping-pong slots of r.Mr
   SolveScalars* sc;
@@ -59,6 +66,23 @@ struct BInst {
 };
 
 // ---------------------------------------------------------------- kernels
+// z = B_J^-1 r of node J for the node-block Jacobi (op_block_precond's
+// products, Precond.f:419-540)
+template <int NB>
+__device__ inline void node_apply(const double* __restrict__ bm, int J, const double* r,
+                                  double* z) {
+  if constexpr (NB == 2) {
+    const double* a = bm + 3 * (int64_t)J;
+    z[0] = a[0] * r[0] + a[2] * r[1];
+    z[1] = a[2] * r[0] + a[1] * r[1];
+  } else {
+    const double* a = bm + 6 * (int64_t)J;
+    z[0] = a[0] * r[0] + a[3] * r[1] + a[4] * r[2];
+    z[1] = a[3] * r[0] + a[1] * r[1] + a[5] * r[2];
+    z[2] = a[4] * r[0] + a[5] * r[1] + a[2] * r[2];
+  }
+}
+
 __device__ inline int wl_slot(uint32_t e) { return (int)(e >> 24); }
 __device__ inline int wl_index(uint32_t e) { return (int)(e & 0xFFFFFFu); }
 
@@ -222,15 +246,20 @@ __device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, doub
 #pragma unroll
       for (int r = 0; r < NB; ++r) qv[r] += (double)e[r];
     }
+    double rn[NB], zv[NB];
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
       I.x[i0 + r] = xv[r] + alpha * pv[r];
       const VT rs = (VT)(rv[r] - alpha * qv[r]);
       R[i0 + r] = rs;
-      const double ri = (double)rs;
-      const double zi = ri * mv[r];
-      acc += ri * zi;
-      azq += zi * qv[r];
+      rn[r] = (double)rs;
+      zv[r] = rn[r] * mv[r];
+    }
+    if (I.bm) node_apply<NB>(I.bm, J, rn, zv);  // node-block Jacobi (FP64 vectors)
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      acc += rn[r] * zv[r];
+      azq += zv[r] * qv[r];
     }
   }
   const double s = block_sum(acc, sh);
@@ -301,6 +330,19 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
         I.p32[i] = pi;
         apr += (double)pi * ri;
       }
+    } else if (I.bm) {  // node-block Jacobi: one lane per node of the chunk
+      const int nb = I.nb;
+      for (int J = i0 / nb + threadIdx.x; J < i1 / nb; J += kThreads) {
+        double rv[3], zv[3];
+        for (int r = 0; r < nb; ++r) rv[r] = I.r[nb * J + r];
+        if (nb == 3) node_apply<3>(I.bm, J, rv, zv);
+        else node_apply<2>(I.bm, J, rv, zv);
+        for (int r = 0; r < nb; ++r) {
+          const double pi = zv[r] + beta * I.p[nb * J + r];
+          I.p[nb * J + r] = pi;
+          apr += pi * rv[r];
+        }
+      }
     } else {
       for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
         const double ri = I.r[i];
@@ -356,7 +398,22 @@ __global__ __launch_bounds__(kThreads) void k_bt_rel_resid(const BInst* __restri
     const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
     const bool v32 = I.p32 && !I.sc->op64;
     double acc = 0.0, apr = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    if (I.bm) {  // node-block Jacobi (FP64 vectors): one lane per node
+      const int nb = I.nb;
+      for (int J = i0 / nb + threadIdx.x; J < i1 / nb; J += kThreads) {
+        double rv[3], zv[3];
+        for (int r = 0; r < nb; ++r) {
+          const int i = nb * J + r;
+          rv[r] = I.b[i] - merge_row(I.mg, i, I.t[i]);
+          I.r[i] = rv[r];
+          apr += I.p[i] * rv[r];
+        }
+        if (nb == 3) node_apply<3>(I.bm, J, rv, zv);
+        else node_apply<2>(I.bm, J, rv, zv);
+        for (int r = 0; r < nb; ++r) acc += rv[r] * zv[r];
+      }
+    }
+    for (int i = i0 + threadIdx.x; i < (I.bm ? i0 : i1); i += kThreads) {
       double s = I.t[i];
       s = merge_row(I.mg, i, s);
       const double ri = I.b[i] - s;
@@ -418,7 +475,23 @@ __global__ __launch_bounds__(kThreads) void k_bt_init(const BInst* __restrict__ 
   const int c = blockIdx.x;
   const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
   double acc = 0.0, apr = 0.0;
-  for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
+  if (I.bm) {  // node-block Jacobi (FP64 vectors): p = B^-1 b, one lane per node
+    const int nb = I.nb;
+    for (int J = i0 / nb + threadIdx.x; J < i1 / nb; J += kThreads) {
+      double bv[3], zv[3];
+      for (int r = 0; r < nb; ++r) bv[r] = I.b[nb * J + r];
+      if (nb == 3) node_apply<3>(I.bm, J, bv, zv);
+      else node_apply<2>(I.bm, J, bv, zv);
+      for (int r = 0; r < nb; ++r) {
+        const int i = nb * J + r;
+        I.x[i] = 0.0;
+        I.r[i] = bv[r];
+        I.p[i] = zv[r];
+        acc += bv[r] * zv[r];
+      }
+    }
+  }
+  for (int i = i0 + threadIdx.x; i < (I.bm ? i0 : i1); i += kThreads) {
     const double bi = I.b[i], zi = bi * I.m[i];
     I.x[i] = 0.0;
     I.r[i] = bi;
@@ -781,12 +854,20 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   I.q = c->v64[VQ];
   I.b = c->v64[VB];
   I.t = c->v64[VT];
+  // node-block Jacobi when the context asks for it (FP64 vectors then)
+  const double* bm = nullptr;
+  if (c->batch_precond == ELASTIC_PRECOND_BDIAG3) {
+    int rc = ensure_bdiag(x, c->layout_nb);
+    if (rc) return berr(b, rc, c->err);
+    bm = c->A64.bm;
+  }
   // FP32 vectors only for instances whose products sum in FP32
-  const bool v32 = precision == ELASTIC_MIXED && c->accum32;
+  const bool v32 = precision == ELASTIC_MIXED && c->accum32 && !bm;
   I.p32 = v32 ? c->p32 : nullptr;
   I.r32 = v32 ? c->p32 + vec32_stride(c->neq) : nullptr;
   I.m32 = v32 ? c->p32 + 2 * vec32_stride(c->neq) : nullptr;
   I.m = c->A64.m;
+  I.bm = bm;
   I.part = c->red64;
   I.dslot = c->red64 + 8 * kRedBlocks + 8;
   I.sc = c->scal;

@@ -317,6 +317,8 @@ int elastic_create(elastic_ctx** out, int32_t device) {
   }
   if (const char* tb = getenv("ELASTIC_TILE_BLOCKS"))
     c->tile_blocks = std::max(32, std::min(kBlkTileMax, atoi(tb)));
+  if (const char* l = getenv("ELASTIC_LANES64")) c->lanes[0] = atoi(l) == 512 ? 512 : 256;
+  if (const char* l = getenv("ELASTIC_LANES32")) c->lanes[1] = atoi(l) == 512 ? 512 : 256;
   const char* ce = getenv("ELASTIC_CHECK_EVERY");
   if (ce) c->check_every = std::max(1, atoi(ce));
   *out = x;
@@ -475,6 +477,11 @@ int elastic_set_option(elastic_ctx* x, const char* name, int64_t value) {
   if (n == "accum32") {
     c->accum32 = value != 0;
     c->accum32_set = 1;
+  } else if (n == "batch_precond") {
+    // preconditioner of the context's batched solves: 2 Jacobi, 6 block
+    // Jacobi with the layout's node blocks (2x2 in 2-D, 3x3 in 3-D)
+    if (value != ELASTIC_PRECOND_DIAG && value != ELASTIC_PRECOND_BDIAG3) return ELASTIC_BAD_ARG;
+    c->batch_precond = (int)value;
   } else if (n == "check_every") {
     if (value < 1 || value > 4096) return ELASTIC_BAD_ARG;
     c->check_every = (int)value;
@@ -638,7 +645,7 @@ int elastic_download_x(elastic_ctx* x, double* xv) {
 // ------------------------------------------------------------------ solves
 namespace elastic {
 
-int ensure_bdiag(elastic_ctx* x);
+int ensure_bdiag(elastic_ctx* x, int bsz);
 
 // statistics of Iters.f:435-457 on the resident x (FP64 operator)
 // Post-solve figures of Iters.f:435-457 on the FP64 operator: x.Kx, ||x||,
@@ -781,7 +788,7 @@ int solve_resident(elastic_ctx* x, int solver, int precond, int precision, doubl
   if (tol <= 0.0) tol = 2.220446049250313e-16;  // tol 0 -> machine epsilon (Rdata.f:2056)
   if (maxit <= 0) return set_error(c, ELASTIC_BAD_ARG, "maxit must be positive");
   if (precond == ELASTIC_PRECOND_BDIAG3) {
-    int rc = ensure_bdiag(x);
+    int rc = ensure_bdiag(x, 3);
     if (rc) return rc;
   } else if (precond != ELASTIC_PRECOND_DIAG) {
     return set_error(c, ELASTIC_BAD_ARG, "preconditioner must be 2 (diag) or 6 (3x3 blocks)");

@@ -176,6 +176,9 @@ struct Context {
   int64_t nad = 0;
   int tile_nnz = 2048;
   int tile_blocks = 0;         // node blocks per tile; 0: by node size (kTileBlocks3/2)
+  // lanes per tile of the node-blocked product (256 or 512), FP64 [0] and
+  // FP32-value [1] operators
+  int lanes[2] = {256, 256};
   int layout_nb = 1;           // 1: scalar tiles; 2 or 3: node-blocked tiles
   bool accum32 = false;        // FP32 operator: FP32 staging and sums (else FP64)
   // -1: accum32 follows the system (on for 3-dof layouts of at least
@@ -194,6 +197,8 @@ struct Context {
   DevMatrix<double> A64;
   DevMatrix<float> A32;
   bool values_set = false, values32 = false, bdiag_set = false;
+  int bdiag_nb = 0;         // block size of the inverse in A64.bm (2 or 3)
+  int batch_precond = 2;    // preconditioner of batched solves (2 Jacobi, 6 node blocks)
   // workspace
   double *epart64 = nullptr, *red64 = nullptr;
   double* tdot = nullptr;  // per-tile shares of p.(A p) (node-blocked layouts)

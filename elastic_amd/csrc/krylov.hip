@@ -42,63 +42,98 @@ __global__ void k_bdiag(int nb, const T* __restrict__ ad, const T* __restrict__ 
   }
 }
 
-int ensure_bdiag(elastic_ctx* x) {
+// 2x2 inverses of consecutive equation pairs (Precond.f:163-358, nblock = 2):
+// (a22, a11, -a21) / det per pair; slot holds the al position of a21 (-1
+// when absent).
+__global__ void k_bdiag2(int nb, const double* __restrict__ ad, const double* __restrict__ al,
+                         const int64_t* __restrict__ slots, double* __restrict__ bm) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
+    const double a11 = ad[2 * i], a22 = ad[2 * i + 1];
+    const double a21 = slots[i] >= 0 ? al[slots[i]] : 0.0;
+    const double idet = 1.0 / (a11 * a22 - a21 * a21);
+    bm[3 * i] = idet * a22;
+    bm[3 * i + 1] = idet * a11;
+    bm[3 * i + 2] = -idet * a21;
+  }
+}
+
+// The block-Jacobi inverse of block size bsz (2 or 3) in c->A64.bm: 3 values
+// per pair or 6 per triple, the reference's m(., i) of block_precond.
+int ensure_bdiag(elastic_ctx* x, int bsz) {
   Context* c = &x->c;
-  if (c->bdiag_set) return ELASTIC_OK;
-  const int nb = c->neq / 3;
-  if (nb == 0) return set_error(c, ELASTIC_BAD_ARG, "block preconditioner needs neq >= 3");
-  std::vector<int64_t> slots(3 * (size_t)nb, -1);
+  if (bsz != 2 && bsz != 3) return set_error(c, ELASTIC_BAD_ARG, "block size must be 2 or 3");
+  if (c->bdiag_set && c->bdiag_nb == bsz) return ELASTIC_OK;
+  const int nb = c->neq / bsz;
+  if (nb == 0) return set_error(c, ELASTIC_BAD_ARG, "block preconditioner needs neq >= block size");
+  const int per = bsz == 3 ? 3 : 1;  // a21 (a31, a32) positions per block
+  std::vector<int64_t> slots((size_t)per * nb, -1);
   const int32_t* ia = x->ia.data();
   const int32_t* ja = x->ja.data();
   for (int i = 0; i < nb; ++i) {
-    const int ii = 3 * i + 1;  // 1-based first equation of the triple
+    const int ii = bsz * i + 1;  // 1-based first equation of the block
     for (int64_t k = ia[ii] - 1; k < ia[ii + 1] - 1; ++k)
-      if (ja[k] == ii) slots[3 * i] = k;
-    for (int64_t k = ia[ii + 1] - 1; k < ia[ii + 2] - 1; ++k) {
-      if (ja[k] == ii) slots[3 * i + 1] = k;
-      else if (ja[k] == ii + 1) slots[3 * i + 2] = k;
-    }
+      if (ja[k] == ii) slots[(size_t)per * i] = k;
+    if (bsz == 3)
+      for (int64_t k = ia[ii + 1] - 1; k < ia[ii + 2] - 1; ++k) {
+        if (ja[k] == ii) slots[3 * (size_t)i + 1] = k;
+        else if (ja[k] == ii + 1) slots[3 * (size_t)i + 2] = k;
+      }
   }
   int64_t* dslots = nullptr;
   ELASTIC_HIP_CHECK(c, hipMalloc((void**)&dslots, sizeof(int64_t) * slots.size()));
   ELASTIC_HIP_CHECK(c, hipMemcpyAsync(dslots, slots.data(), sizeof(int64_t) * slots.size(),
                                       hipMemcpyHostToDevice, c->stream));
-  if (!c->A64.bm) ELASTIC_HIP_CHECK(c, hipMalloc((void**)&c->A64.bm, sizeof(double) * 6 * nb));
+  // room for either size: 6 per triple or 3 per pair
+  if (!c->A64.bm)
+    ELASTIC_HIP_CHECK(c, hipMalloc((void**)&c->A64.bm,
+                                   sizeof(double) * std::max(6 * (c->neq / 3), 3 * (c->neq / 2))));
   int g = std::min((nb + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_bdiag<double>, dim3(g), dim3(256), 0, c->stream, nb, c->A64.ad, c->A64.al,
-                     dslots, c->A64.bm);
+  if (bsz == 3)
+    hipLaunchKernelGGL(k_bdiag<double>, dim3(g), dim3(256), 0, c->stream, nb, c->A64.ad,
+                       c->A64.al, dslots, c->A64.bm);
+  else
+    hipLaunchKernelGGL(k_bdiag2, dim3(g), dim3(256), 0, c->stream, nb, c->A64.ad, c->A64.al,
+                       dslots, c->A64.bm);
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   hipFree(dslots);
   c->bdiag_set = true;
+  c->bdiag_nb = bsz;
   return ELASTIC_OK;
 }
 
 }  // namespace elastic
 
-// m(6, neq/3 + mod(neq,3)) of block_precond (Precond.f:163-358): the device's
-// block inverse, then one 1/ad entry per remaining equation.
-extern "C" int elastic_get_bdiag(elastic_ctx* x, double* m) {
+// m(w, neq/bsz + mod(neq,bsz)) of block_precond (Precond.f:163-358), w = 6
+// (bsz 3) or 3 (bsz 2): the device's block inverse, then one 1/ad entry per
+// remaining equation.
+extern "C" int elastic_get_block_precond(elastic_ctx* x, int32_t bsz, double* m) {
   using namespace elastic;
   if (!x || !m) return ELASTIC_BAD_ARG;
   Context* c = &x->c;
   if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
   hipSetDevice(c->device);
-  int rc = ensure_bdiag(x);
+  int rc = ensure_bdiag(x, bsz);
   if (rc) return rc;
-  const int nb = c->neq / 3, rest = c->neq % 3;
+  const int w = bsz == 3 ? 6 : 3;
+  const int nb = c->neq / bsz, rest = c->neq % bsz;
   std::vector<double> inv(rest > 0 ? rest : 1);
-  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(m, c->A64.bm, sizeof(double) * 6 * (size_t)nb,
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(m, c->A64.bm, sizeof(double) * w * (size_t)nb,
                                       hipMemcpyDeviceToHost, c->stream));
   if (rest)
-    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(inv.data(), c->A64.m + 3 * (size_t)nb,
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(inv.data(), c->A64.m + (size_t)bsz * nb,
                                         sizeof(double) * rest, hipMemcpyDeviceToHost, c->stream));
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < rest; ++i) {
-    double* o = m + 6 * ((size_t)nb + i);
+    double* o = m + (size_t)w * ((size_t)nb + i);
     o[0] = inv[i];
-    o[1] = o[2] = o[3] = o[4] = o[5] = 0.0;
+    for (int k = 1; k < w; ++k) o[k] = 0.0;
   }
   return ELASTIC_OK;
+}
+
+extern "C" int elastic_get_bdiag(elastic_ctx* x, double* m) {
+  return elastic_get_block_precond(x, 3, m);
 }
 
 namespace elastic {

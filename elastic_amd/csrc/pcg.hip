@@ -169,14 +169,17 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
                                                                 double* part_zq, SolveScalars* sc) {
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
-  const double pq = sc->pq_tiles;
-  const double num = part_pr ? grid_sum(part_pr, sh) : *dslot;
-  const double alpha = num / pq;
-  double acc = 0.0, azq = 0.0;
-  GRID_LOOP(J, nn) {
-    const int64_t i0 = (int64_t)NB * J;
-    const int k0 = mg.ptr[J], k1 = mg.ptr[J + 1];
-    double qv[NB], xv[NB], pv[NB], rv[NB], mv[NB];
+  // the loads of each lane's first node are issued before alpha's
+  // block-wide sum, and every node's loads before the previous node's
+  // stores, so that their latency overlaps
+  const int stride = gridDim.x * kThreads;
+  int J = blockIdx.x * kThreads + threadIdx.x;
+  double qv[NB], xv[NB], pv[NB], rv[NB], mv[NB];
+  int k0 = 0, k1 = 0;
+  auto load_node = [&](int Jn) {
+    const int64_t i0 = (int64_t)NB * Jn;
+    k0 = mg.ptr[Jn];
+    k1 = mg.ptr[Jn + 1];
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
       qv[c] = (double)reinterpret_cast<const QT*>(q)[i0 + c];
@@ -185,21 +188,38 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
       rv[c] = (double)r[i0 + c];
       mv[c] = (double)m[i0 + c];
     }
+  };
+  if (J < nn) load_node(J);
+  const double pq = sc->pq_tiles;
+  const double num = part_pr ? grid_sum(part_pr, sh) : *dslot;
+  const double alpha = num / pq;
+  double acc = 0.0, azq = 0.0;
+  for (; J < nn; J += stride) {
+    const int64_t i0 = (int64_t)NB * J;
     for (int k = k0; k < k1; ++k) {
       const EP* e = reinterpret_cast<const EP*>(mg.epart) + (int64_t)NB * mg.idx[k];
 #pragma unroll
       for (int c = 0; c < NB; ++c) qv[c] += (double)e[c];
     }
+    double xo[NB], ro[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
-      if (sizeof(QT) == 8 && keep_q) q[i0 + c] = qv[c];
-      x[i0 + c] = xv[c] + alpha * pv[c];
+      xo[c] = xv[c] + alpha * pv[c];
       const VT rs = (VT)(rv[c] - alpha * qv[c]);
-      r[i0 + c] = rs;
-      const double ri = (double)rs;
-      const double zi = ri * mv[c];
-      acc += ri * zi;
+      ro[c] = (double)rs;
+      const double zi = ro[c] * mv[c];
+      acc += ro[c] * zi;
       azq += zi * qv[c];
+    }
+    double qo[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) qo[c] = qv[c];
+    if (J + stride < nn) load_node(J + stride);
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      if (sizeof(QT) == 8 && keep_q) q[i0 + c] = qo[c];
+      x[i0 + c] = xo[c];
+      r[i0 + c] = (VT)ro[c];
     }
   }
   if (jacobi) {
@@ -255,24 +275,49 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
                                                            SolveScalars* sc) {
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
+  // FP32 vectors: the first kPre grid-stride groups of r, m and p are loaded
+  // before beta's block-wide sums, so that their latency overlaps
+  constexpr int kPre = 4;
+  const int n4 = r32 ? (n >> 2) : 0;
+  const int stride = gridDim.x * kThreads;
+  int k = blockIdx.x * kThreads + threadIdx.x;
+  float4 rf[kPre], mf[kPre], pf[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; ++u)
+    if (k + u * stride < n4) {
+      rf[u] = reinterpret_cast<const float4*>(r32)[k + u * stride];
+      mf[u] = reinterpret_cast<const float4*>(m32)[k + u * stride];
+      pf[u] = reinterpret_cast<const float4*>(p32)[k + u * stride];
+    }
   const double di = grid_sum(part_rz, sh);
   const double beta = part_zq ? -grid_sum(part_zq, sh) / sc->pq : di / *dold;
   double apr = 0.0;
   if (r32) {
     // the iteration's vectors are FP32 (Jacobi): p = fl32(m r + beta p), in
     // FP64 arithmetic, and p.r of the stored values; four rows per lane
-    const int n4 = n >> 2;
-    for (int k = blockIdx.x * kThreads + threadIdx.x; k < n4; k += gridDim.x * kThreads) {
-      const float4 rf = reinterpret_cast<const float4*>(r32)[k];
-      const float4 mf = reinterpret_cast<const float4*>(m32)[k];
-      const float4 pf = reinterpret_cast<const float4*>(p32)[k];
-      const float4 o = make_float4((float)((double)rf.x * (double)mf.x + beta * (double)pf.x),
-                                   (float)((double)rf.y * (double)mf.y + beta * (double)pf.y),
-                                   (float)((double)rf.z * (double)mf.z + beta * (double)pf.z),
-                                   (float)((double)rf.w * (double)mf.w + beta * (double)pf.w));
-      reinterpret_cast<float4*>(p32)[k] = o;
-      apr += (double)o.x * (double)rf.x + (double)o.y * (double)rf.y +
-             (double)o.z * (double)rf.z + (double)o.w * (double)rf.w;
+    while (k < n4) {
+#pragma unroll
+      for (int u = 0; u < kPre; ++u) {
+        const int kk = k + u * stride;
+        if (kk < n4) {
+          const float4 o =
+              make_float4((float)((double)rf[u].x * (double)mf[u].x + beta * (double)pf[u].x),
+                          (float)((double)rf[u].y * (double)mf[u].y + beta * (double)pf[u].y),
+                          (float)((double)rf[u].z * (double)mf[u].z + beta * (double)pf[u].z),
+                          (float)((double)rf[u].w * (double)mf[u].w + beta * (double)pf[u].w));
+          reinterpret_cast<float4*>(p32)[kk] = o;
+          apr += (double)o.x * (double)rf[u].x + (double)o.y * (double)rf[u].y +
+                 (double)o.z * (double)rf[u].z + (double)o.w * (double)rf[u].w;
+        }
+      }
+      k += kPre * stride;
+#pragma unroll
+      for (int u = 0; u < kPre; ++u)
+        if (k + u * stride < n4) {
+          rf[u] = reinterpret_cast<const float4*>(r32)[k + u * stride];
+          mf[u] = reinterpret_cast<const float4*>(m32)[k + u * stride];
+          pf[u] = reinterpret_cast<const float4*>(p32)[k + u * stride];
+        }
     }
     for (int i = 4 * n4 + blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
       const float pi = (float)((double)r32[i] * (double)m32[i] + beta * (double)p32[i]);
@@ -487,10 +532,13 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
       ELASTIC_HIP_CHECK(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   }
   // Gated replacement passes cost their launches even when they skip.  On
-  // large systems the host enqueues them only in chunks where, at the decay
-  // rate of the recursive residual seen between polls, the trigger can come
-  // within two chunks (a missed one only waits for the next chunk); small
-  // systems enqueue them every second iteration.
+  // large systems a replacement that the iteration asked for (the recursive
+  // residual fell by delta, or kForceReplace iterations passed) runs right
+  // after the poll that sees the flag, and the gated pass is enqueued every
+  // second iteration only in chunks where, at the decay rate of the
+  // recursive residual seen between polls, the stopping threshold can be
+  // reached within two chunks, so that convergence is confirmed on the true
+  // residual without delay.  Small systems enqueue it every second iteration.
   const bool predict = reliable && n >= kPredictRows;
   bool gate = true;
   double rn_prev = -1.0;
@@ -577,15 +625,14 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
                          pr_next, sc);
       replacing = false;
     }
-    if (predict) {
+    if (predict && replacing) {
       const SolveScalars& h = *c->scal_host;
+      if (h.upd) replace(it);  // asked for during the chunk: runs before the next one
       const double rn = std::sqrt(std::fabs(h.di));
-      const double target = std::max(h.delta * h.rmax, h.conv);
       gate = true;
-      if (!h.upd && rn > target && rn_prev > 0.0 && h.iter > it_prev &&
-          h.iter - h.last_upd + 2 * every < kForceReplace) {
+      if (rn > h.conv && rn_prev > 0.0 && h.iter > it_prev) {
         const double rate = std::pow(rn / rn_prev, 1.0 / (double)(h.iter - it_prev));
-        if (rate < 1.0 && std::log(target / rn) / std::log(rate) > 2.0 * every) gate = false;
+        if (rate < 1.0 && std::log(h.conv / rn) / std::log(rate) > 2.0 * every) gate = false;
       }
       rn_prev = rn;
       it_prev = h.iter;

@@ -77,7 +77,8 @@ struct TileRegs {
 };
 
 // Issues the loads of tile t (values nontemporal: read once per product).
-template <typename TV, int NB, int PER>
+// NT: lanes of the workgroup (256 or 512); lane tid owns blocks tid + NT i.
+template <typename TV, int NB, int PER, int NT = 256>
 __device__ __forceinline__ void load_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
                                           const TV* __restrict__ diag, TileRegs<TV, NB, PER>& R) {
   constexpr int NB2 = NB * NB;
@@ -87,7 +88,7 @@ __device__ __forceinline__ void load_tile(const BlkArgs& a, int t, const TV* __r
   const TV* v = vals + base * NB2;
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const int q = tid + 256 * i;
+    const int q = tid + NT * i;
     if (q < nbk) {
       R.wr[i] = __builtin_nontemporal_load(a.word + base + q);
       Planes<TV, NB>::load(v, nbk, q, R.vr[i]);
@@ -104,7 +105,7 @@ __device__ __forceinline__ void load_tile(const BlkArgs& a, int t, const TV* __r
   }
 }
 
-// The product of tile t from its registers (a workgroup of 256 lanes); smem
+// The product of tile t from its registers (a workgroup of NT lanes); smem
 // holds sizeof(T) * NB * (ncols_max + 2 * kBlkRowsMax + nblk_max) bytes.  T is
 // the type of the staged x, the products and the sums (double, or float for
 // the FP32 operator with FP32 accumulation); x, y and the external partials
@@ -115,14 +116,19 @@ struct NoHook {
 };
 
 // Block sum of the lanes' shares of x.(A x) into part[t] (fixed order).
+template <int NT = 256>
 __device__ __forceinline__ void tile_dot_store(double* part, int t, double v) {
-  __shared__ double red[4];
+  __shared__ double red[NT / 64];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();  // red of a previous tile in this workgroup is free
   if (lane == 0) red[wid] = v;
   __syncthreads();
-  if (threadIdx.x == 0) part[t] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (threadIdx.x == 0) {
+    double s = (red[0] + red[1]) + (red[2] + red[3]);
+    if constexpr (NT == 512) s += (red[4] + red[5]) + (red[6] + red[7]);
+    part[t] = s;
+  }
 }
 
 // DOT: also the tile's share of x.(A x) into tdot[t]: x.y over its own rows
@@ -131,7 +137,8 @@ __device__ __forceinline__ void tile_dot_store(double* part, int t, double v) {
 // products and sums in FP64.  The shares
 // are added in tile order by a kernel of their own (k_sum_shares), so x.(A x)
 // needs no further pass over y and no floating-point atomics.
-template <typename TV, typename T, int NB, int PER, bool DOT = false, typename Hook = NoHook>
+template <typename TV, typename T, int NB, int PER, bool DOT = false, typename Hook = NoHook,
+          int NT = 256>
 __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
                                              const TileRegs<TV, NB, PER>& R,
                                              const double* __restrict__ x, double* __restrict__ y,
@@ -149,7 +156,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   const int32_t* ext = a.ext + ext0;
   const uint32_t* wr = R.wr;
   const auto& vr = R.vr;
-  // x of the tile's local columns: the first 8 * 256 entries are loaded into
+  // x of the tile's local columns: the first 8 * NT entries are loaded into
   // registers, then `hook` runs (a caller may issue more loads there, which
   // stay in flight behind these), then they go to LDS
   constexpr int XR = 8;
@@ -157,7 +164,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   double xv[XR];
 #pragma unroll
   for (int k = 0; k < XR; ++k) {
-    const int c = tid + 256 * k;
+    const int c = tid + NT * k;
     if (c < nx) {
       const int node = c / NB, comp = c - node * NB;
       const int g = node < nr ? I0 + node : ext[node - nr];
@@ -169,10 +176,10 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   __syncthreads();  // the LDS of a previous tile is free
 #pragma unroll
   for (int k = 0; k < XR; ++k) {
-    const int c = tid + 256 * k;
+    const int c = tid + NT * k;
     if (c < nx) xs[c] = (T)xv[k];
   }
-  for (int c = tid + 256 * XR; c < nx; c += 256) {
+  for (int c = tid + NT * XR; c < nx; c += NT) {
     const int node = c / NB, comp = c - node * NB;
     const int g = node < nr ? I0 + node : ext[node - nr];
     xs[c] = x32 ? (T)x32[(int64_t)NB * g + comp] : (T)x[(int64_t)NB * g + comp];
@@ -181,7 +188,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   const uint16_t* rp = a.rowptr + a.rp0[t];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const int q = tid + 256 * i;
+    const int q = tid + NT * i;
     const bool valid = q < nbk;
     const uint32_t w = wr[i];
     const int key = valid ? (int)(w >> 25) : 255;
@@ -232,13 +239,13 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   __syncthreads();
   const uint16_t* sp = a.segptr + a.seg0[t];
   double dacc = 0.0;  // this lane's share of x.(Ax) (DOT)
-  for (int c = tid; c < ncols; c += 256) {
+  for (int c = tid; c < ncols; c += NT) {
     T s[NB];
 #pragma unroll
     for (int r = 0; r < NB; ++r) s[r] = T(0);
     if (c < nr) {
       // node diagonal block (nb diagonal entries, then a21, a31, a32)
-      // c < nr <= 127 < 256, so c == tid: the lane loaded this node's diagonal
+      // c < nr <= 127 < NT, so c == tid: the lane loaded this node's diagonal
       T d[NB * (NB + 1) / 2];
 #pragma unroll
       for (int k = 0; k < NB * (NB + 1) / 2; ++k) d[k] = (T)R.dg[k];
@@ -285,11 +292,11 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
       for (int r = 0; r < NB; ++r) dacc += (double)s[r] * (double)xs[c * NB + r];
     }
   }
-  if constexpr (DOT) tile_dot_store(tdot, t, dacc);
+  if constexpr (DOT) tile_dot_store<NT>(tdot, t, dacc);
 }
 
 // One tile: its loads, then its product.
-template <typename TV, typename T, int NB, int PER, bool DOT = false>
+template <typename TV, typename T, int NB, int PER, bool DOT = false, int NT = 256>
 __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
                                            const TV* __restrict__ diag,
                                            const double* __restrict__ x, double* __restrict__ y,
@@ -297,8 +304,8 @@ __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __
                                            double* __restrict__ tdot = nullptr, int y32 = 0,
                                            const float* __restrict__ x32 = nullptr) {
   TileRegs<TV, NB, PER> R;
-  load_tile<TV, NB, PER>(a, t, vals, diag, R);
-  compute_tile<TV, T, NB, PER, DOT>(a, t, R, x, y, epart, smem, tdot, y32, x32);
+  load_tile<TV, NB, PER, NT>(a, t, vals, diag, R);
+  compute_tile<TV, T, NB, PER, DOT, NoHook, NT>(a, t, R, x, y, epart, smem, tdot, y32, x32);
 }
 
 }  // namespace elastic

@@ -5,9 +5,9 @@
 
 namespace elastic {
 
-// DOT: also the tiles' shares of x.(A x) into tdot
-template <typename TV, typename T, int NB, int PER, bool DOT>
-__global__ __launch_bounds__(256) void k_spmv_blocks(BlkArgs a, const TV* __restrict__ vals,
+// DOT: also the tiles' shares of x.(A x) into tdot; NT lanes per tile
+template <typename TV, typename T, int NB, int PER, bool DOT, int NT>
+__global__ __launch_bounds__(NT) void k_spmv_blocks(BlkArgs a, const TV* __restrict__ vals,
                                                       const TV* __restrict__ diag,
                                                       const double* __restrict__ x,
                                                       double* __restrict__ y,
@@ -17,7 +17,8 @@ __global__ __launch_bounds__(256) void k_spmv_blocks(BlkArgs a, const TV* __rest
                                                       const float* __restrict__ x32) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (skip != nullptr && *skip != 0) return;
-  block_tile<TV, T, NB, PER, DOT>(a, blockIdx.x, vals, diag, x, y, epart, smem, tdot, y32, x32);
+  block_tile<TV, T, NB, PER, DOT, NT>(a, blockIdx.x, vals, diag, x, y, epart, smem, tdot, y32,
+                                      x32);
 }
 
 // *out = the n shares added in a fixed order (one workgroup of 1024 lanes):
@@ -100,25 +101,27 @@ template <typename TV, typename TA, int NB>
 static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* diag,
                                const double* x, double* y, double* epart, hipStream_t s,
                                const int32_t* skip, const DotOut* dot, int y32,
-                               const float* x32) {
+                               const float* x32, int lanes) {
   const size_t lds = sizeof(TA) * NB * ((size_t)b.ncols_max + 2 * kBlkRowsMax + (size_t)b.nblk_max);
-  const int per = (b.nblk_max + 255) / 256;
-#define ELASTIC_BLK_LAUNCH(K, P, D, G, ...)                                                     \
+  const int nt = lanes == 512 && b.nblk_max > 256 ? 512 : 256;
+  const int per = (b.nblk_max + nt - 1) / nt;
+#define ELASTIC_BLK_LAUNCH(K, P, D, T, G, ...)                                                  \
   do {                                                                                          \
     static bool raised = false;                                                                 \
     if (lds > 65536 && !raised) {                                                               \
-      (void)hipFuncSetAttribute((const void*)K<TV, TA, NB, P, D>,                               \
+      (void)hipFuncSetAttribute((const void*)K<TV, TA, NB, P, D, T>,                            \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);        \
       raised = true;                                                                            \
     }                                                                                           \
-    hipLaunchKernelGGL((K<TV, TA, NB, P, D>), dim3(G), dim3(256), lds, s, __VA_ARGS__);         \
+    hipLaunchKernelGGL((K<TV, TA, NB, P, D, T>), dim3(G), dim3(T), lds, s, __VA_ARGS__);        \
   } while (0)
   double* td = dot ? dot->part : nullptr;
 #define ELASTIC_BLK_PER(D)                                                                               \
   do {                                                                                                   \
-    if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 1, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
-    else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
-    else ELASTIC_BLK_LAUNCH(k_spmv_blocks, 4, D, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
+    if (nt == 512) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, D, 512, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
+    else if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 1, D, 256, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
+    else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, D, 256, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
+    else ELASTIC_BLK_LAUNCH(k_spmv_blocks, 4, D, 256, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
   } while (0)
   if (dot) {
     ELASTIC_BLK_PER(true);
@@ -136,14 +139,15 @@ void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epar
                  const int32_t* skip, const DotOut* dot, bool y32, const float* x32) {
   hipStream_t s = c->stream;
   const DevBlocks& b = c->blk;
+  const int l32 = c->lanes[1], l64 = c->lanes[0];
   if (c->layout_nb == 3) {
-    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0, x32);
-    else if (fp32) spmv_blocks_launch<float, double, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0, nullptr);
-    else spmv_blocks_launch<double, double, 3>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr);
+    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0, x32, l32);
+    else if (fp32) spmv_blocks_launch<float, double, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0, nullptr, l32);
+    else spmv_blocks_launch<double, double, 3>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr, l64);
   } else if (c->layout_nb == 2) {
-    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0, x32);
-    else if (fp32) spmv_blocks_launch<float, double, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0, nullptr);
-    else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr);
+    if (fp32 && c->accum32) spmv_blocks_launch<float, float, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0, x32, l32);
+    else if (fp32) spmv_blocks_launch<float, double, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0, nullptr, l32);
+    else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr, l64);
   } else if (c->gather) {
     const DevGather& g = c->gth;
     if (fp32)

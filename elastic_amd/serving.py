@@ -13,7 +13,10 @@ completion (x on the host) minus arrival.
 
 There is no counterpart in the reference: its solver call serves one system
 (Solv.f:13-440).  Each instance runs the reference's PCG (Iters.f:252-524)
-in FP64 or with the mixed-precision operator.
+in FP64 or with the mixed-precision operator, with Jacobi or node-block
+Jacobi (block_precond / op_block_precond, Precond.f:163-540).  The server
+picks both per request (`request_policy`) unless it is given a fixed
+precision.
 """
 from __future__ import annotations
 
@@ -87,6 +90,35 @@ class Batch:
             pass
 
 
+# 3-dof systems of at least this many nodes take the mixed-precision operator
+POLICY_MIXED_NODES = 32768
+
+
+def request_policy(ctx) -> tuple[int, int]:
+    """(precision, preconditioner) for a prepared request, from its layout:
+
+    * 3-dof node blocks and at least POLICY_MIXED_NODES nodes: MIXED (FP32
+      values, and FP32 sums by the context's default) with Jacobi.  On such
+      systems the FP32 operator costs few iterations (hex8 100^3: 1,436
+      against 1,329) and halves the bytes of each product.
+    * 2-dof node blocks (plane problems): FP64 with the 2x2 node-block
+      Jacobi.  FP32 values cost these ill-conditioned systems 1.2-3.6 times
+      the FP64 iterations (condition numbers of 1e6-1e7 amplify the
+      rounding of the values), and the 2x2 blocks save about 10% of the
+      iterations on the tria3 requests (none on quad4).
+    * anything else: FP64 with Jacobi.
+
+    Returns the batch_precond option value as the preconditioner (2 Jacobi,
+    6 node blocks)."""
+    info = ctx.layout_info()
+    nb = info["nb"]
+    if nb == 3 and ctx.neq // 3 >= POLICY_MIXED_NODES:
+        return MIXED, 2
+    if nb == 2:
+        return FP64, 6
+    return FP64, 2
+
+
 def footprint_bytes(mesh) -> int:
     """Device bytes a prepared request holds, from its mesh alone (admission
     happens before anything is allocated): CSRC (ia, ja, al, ad), the blocked
@@ -110,6 +142,8 @@ class Request:
     t_ready: float = 0.0
     t_submit: float = 0.0
     t_done: float = 0.0
+    precision: int = 0
+    precond: int = 0
     stats: dict = field(default_factory=dict)
     x: np.ndarray | None = None
 
@@ -126,13 +160,16 @@ def _prepare(req: Request, device: int):
 class Server:
     """Continuous batching over a request stream (see the module text).
 
-    mode "batched": prepared systems share one Batch (precision per request).
+    mode "batched": prepared systems share one Batch.
     mode "sequential": each prepared system is solved alone with
     elastic_solve_resident, in arrival order (the plain drop-in path).
+    precision None (the default): precision and preconditioner per request
+    by request_policy; FP64 or MIXED: that precision with Jacobi for every
+    request.
     """
 
     def __init__(self, device: int = 0, max_slots: int = 64, iters_per_step: int = 16,
-                 hbm_budget: float = 64e9, prep_workers: int = 8, precision: int = MIXED,
+                 hbm_budget: float = 64e9, prep_workers: int = 8, precision: int | None = None,
                  tol: float = 1e-10, maxit: int = 50000, mode: str = "batched",
                  keep_x: bool = False, progress=None):
         self.device = device
@@ -207,7 +244,8 @@ class Server:
                     if ready:
                         r, s = ready.pop(0)
                         r.t_submit = now()
-                        st = s.ctx.solve_resident(precision=self.precision, tol=self.tol,
+                        prec = self.precision if self.precision is not None else FP64
+                        st = s.ctx.solve_resident(precision=prec, tol=self.tol,
                                                   maxit=self.maxit)
                         finish(r, s, st)
                         reserved -= r.footprint
@@ -215,10 +253,16 @@ class Server:
                 else:
                     while ready:
                         r, s = ready[0]
-                        if not batch.submit(s.ctx, self.precision, self.tol, self.maxit, r.k):
+                        if self.precision is None:
+                            prec, pre = request_policy(s.ctx)
+                            s.ctx.set_option("batch_precond", pre)
+                        else:
+                            prec, pre = self.precision, 2
+                        if not batch.submit(s.ctx, prec, self.tol, self.maxit, r.k):
                             break
                         ready.pop(0)
                         r.t_submit = now()
+                        r.precision, r.precond = prec, pre
                         running[r.k] = (r, s)
                     if running:
                         batch.step()
@@ -240,14 +284,24 @@ class Server:
                 batch.close()
         makespan = now()
         lat = np.array([r.t_done - r.arrival for r in done])
+        svc = np.array([r.t_done - r.t_prep for r in done])
+        slv = np.array([r.t_done - r.t_submit for r in done])
         neq = np.array([r.neq for r in done], dtype=np.float64)
         return {
             "requests": len(done),
             "makespan_s": makespan,
             "solves_per_s": len(done) / makespan,
             "dof_per_s": float(neq.sum()) / makespan,
+            # from arrival (queueing included), from admission (set-up,
+            # assembly, waiting for a slot and the solve), from submission
             "latency_ms_p50": 1e3 * float(np.percentile(lat, 50)),
             "latency_ms_p99": 1e3 * float(np.percentile(lat, 99)),
+            "service_ms_p50": 1e3 * float(np.percentile(svc, 50)),
+            "service_ms_p99": 1e3 * float(np.percentile(svc, 99)),
+            "solve_ms_p50": 1e3 * float(np.percentile(slv, 50)),
+            "solve_ms_p99": 1e3 * float(np.percentile(slv, 99)),
+            "precision_mix": {str(p): int(sum(getattr(r, "precision", None) == p for r in done))
+                              for p in (FP64, MIXED)},
             "iters_mean": float(np.mean([r.stats["iters"] for r in done])),
             "maxit_hits": int(sum(r.stats["status"] != 0 for r in done)),
             "done": done,

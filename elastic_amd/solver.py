@@ -75,6 +75,14 @@ class Context:
         self._chk(self._L.elastic_get_bdiag(self._h, m))
         return m
 
+    def get_block_precond(self, bsz: int) -> np.ndarray:
+        """m(bsz(bsz+1)/2, neq/bsz + mod(neq,bsz)) of block_precond with block
+        size bsz (2 or 3), flattened (Precond.f:163-358)."""
+        w = bsz * (bsz + 1) // 2
+        m = np.zeros(w * (self.neq // bsz + self.neq % bsz))
+        self._chk(self._L.elastic_get_block_precond(self._h, bsz, m))
+        return m
+
     def layout_info(self):
         info = np.zeros(4, np.int64)
         self._chk(self._L.elastic_layout_info(self._h, info))

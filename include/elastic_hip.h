@@ -181,6 +181,11 @@ int elastic_j2_get_state(elastic_ctx* ctx, int32_t trial, double* ep, double* al
  * the last mod(neq,3) equations), bit-identical to the reference's, from
  * the device values that the solves apply. */
 int elastic_get_bdiag(elastic_ctx* ctx, double* m);
+/* block_precond (Precond.f:163-358) with block size bsz = 2 or 3: fills
+ * m(w, neq/bsz + mod(neq,bsz)), w = bsz(bsz+1)/2 (2x2: a22/det, a11/det,
+ * -a21/det of each equation pair; 3x3 as elastic_get_bdiag), then 1/ad for
+ * each remaining equation; bit-identical to the reference's. */
+int elastic_get_block_precond(elastic_ctx* ctx, int32_t bsz, double* m);
 
 /* ----------------------------------------------------------- solving */
 /* Solve A x = b.  Replaces solv -> pcg (Solv.f:13-139, Iters.f:252-524) for
@@ -259,6 +264,9 @@ int elastic_set_tile_blocks(elastic_ctx* ctx, int32_t tile_blocks);
  * FP64; until set, 1 for PCG on 3-dof systems of at least 32,768 nodes,
  * else 0),
  * "check_every" (iterations between convergence polls),
+ * "batch_precond" (2: the context's batched solves use Jacobi, the default;
+ * 6: node-block Jacobi, the inverses of block_precond with the layout's node
+ * size, 2x2 in 2-D and 3x3 in 3-D, applied as op_block_precond does),
  * "time_products" (S > 0: PCG brackets the product of every S-th iteration
  * with HIP events on its stream; 0, the default: off). */
 int elastic_set_option(elastic_ctx* ctx, const char* name, int64_t value);

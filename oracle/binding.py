@@ -73,6 +73,9 @@ def lib():
         L.or_gmres.argtypes = solver_args + [C.c_int, C.c_double, C.c_int, C.POINTER(Stats)]
         L.or_pbicgstab.argtypes = solver_args + [C.c_double, C.c_int, C.POINTER(Stats)]
         L.or_pcg_bdiag3.argtypes = solver_args + [C.c_double, C.c_int, C.POINTER(Stats)]
+        L.or_pcg_block.argtypes = [C.c_int] + solver_args + [C.c_double, C.c_int, C.POINTER(Stats)]
+        L.or_block_precond2.argtypes = [F64P, F64P, I32P, I32P, F64P, C.c_int]
+        L.or_op_block_precond2.argtypes = [F64P, F64P, F64P, C.c_int]
         L.or_colormsh.argtypes = [I32P, C.c_int, C.c_int, C.c_int, C.c_int, I32P]
         L.or_tform.argtypes = [I32P, F64P, F64P, I32P, F64P, C.c_void_p, F64P, C.c_int, C.c_int,
                                C.c_int, C.c_int, C.c_int, C.c_int]
@@ -223,6 +226,29 @@ def block_precond3(ad, al, ia, ja):
     m = np.zeros(6 * (neq // 3 + neq % 3))
     lib().or_block_precond3(ad, al, ia, ja, m, neq)
     return m
+
+
+def block_precond2(ad, al, ia, ja):
+    neq = ad.shape[0]
+    m = np.zeros(3 * (neq // 2 + neq % 2))
+    lib().or_block_precond2(ad, al, ia, ja, m, neq)
+    return m
+
+
+def op_block_precond2(m, x):
+    y = np.zeros_like(x)
+    lib().or_op_block_precond2(m, np.ascontiguousarray(x), y, x.shape[0])
+    return y
+
+
+def pcg_block(nblock, ia, ja, ad, al, b, tol=1e-11, maxit=10000, bm=None):
+    """PCG with the block-Jacobi application of op_block_precond, nblock 2 or 3."""
+    if bm is None:
+        bm = (block_precond2 if nblock == 2 else block_precond3)(ad, al, ia, ja)
+    x = np.zeros_like(b)
+    st = Stats()
+    lib().or_pcg_block(nblock, b.shape[0], ia, ja, ad, al, bm, b, x, tol, maxit, C.byref(st))
+    return x, st.as_dict()
 
 
 def op_block_precond3(m, x):

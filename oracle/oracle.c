@@ -1190,6 +1190,42 @@ void or_op_block_precond3(const double* m, const double* x, double* y, int neq) 
   }
 }
 
+/* Precond.f:163-358 nblock=2: 2x2 inverses of consecutive equation pairs,
+ * m(3, neq/2 + rest) = (a22, a11, -a21)/det, then 1/ad for the last equation. */
+void or_block_precond2(const double* ad, const double* al, const int32_t* ia,
+                       const int32_t* ja, double* m, int neq) {
+  int ndiv = neq / 2, rest = neq % 2;
+  for (int i = 0; i < 3 * (ndiv + rest); i++) m[i] = 0.0;
+  for (int i = 1; i <= ndiv; i++) {
+    int ii = 2 * i - 1;
+    double a11 = A1(ad, ii), a22 = A1(ad, ii + 1), a21 = 0.0;
+    for (int j = A1(ia, ii + 1); j <= A1(ia, ii + 2) - 1; j++)
+      if (A1(ja, j) == ii) a21 = A1(al, j);
+    double idet = 1.0 / (a11 * a22 - a21 * a21);
+    double* mi = m + 3 * (i - 1);
+    mi[0] = idet * a22;
+    mi[1] = idet * a11;
+    mi[2] = -idet * a21;
+  }
+  for (int i = 1; i <= rest; i++) m[3 * (ndiv + i - 1)] = 1.0 / A1(ad, 2 * ndiv + i);
+}
+
+/* Precond.f:419-540 nblock=2 */
+void or_op_block_precond2(const double* m, const double* x, double* y, int neq) {
+  int ndiv = neq / 2, rest = neq % 2;
+  for (int i = 1; i <= ndiv; i++) {
+    int ii = 2 * i - 2;
+    const double* a = m + 3 * (i - 1);
+    double x1 = x[ii], x2 = x[ii + 1];
+    y[ii] = a[0] * x1 + a[2] * x2;
+    y[ii + 1] = a[2] * x1 + a[1] * x2;
+  }
+  for (int i = 1; i <= rest; i++) {
+    int ii = 2 * ndiv + i - 1;
+    y[ii] = m[3 * (ndiv + i - 1)] * x[ii];
+  }
+}
+
 /* ------------------------------------------------------------------ SpMV */
 void or_matvec_csrcsym1(int neq, const int32_t* ia, const int32_t* ja, const double* ad,
                         const double* al, const double* x, double* y) {
@@ -1286,9 +1322,20 @@ void or_pcg(int neq, const int32_t* ia, const int32_t* ja, const double* ad,
  * that block_precond (Precond.f:163-358) builds: the preconditioned CG that
  * the reference's precond code 6 names (its call site is dead).  Stopping
  * rule and post-solve figures as in or_pcg, with M the block inverse. */
+static void op_block(int nblock, const double* bm, const double* x, double* y, int neq) {
+  if (nblock == 2) or_op_block_precond2(bm, x, y, neq);
+  else or_op_block_precond3(bm, x, y, neq);
+}
+
 void or_pcg_bdiag3(int neq, const int32_t* ia, const int32_t* ja, const double* ad,
                    const double* al, const double* bm, const double* b, double* x, double tol,
                    int maxit, or_stats* st) {
+  or_pcg_block(3, neq, ia, ja, ad, al, bm, b, x, tol, maxit, st);
+}
+
+void or_pcg_block(int nblock, int neq, const int32_t* ia, const int32_t* ja, const double* ad,
+                  const double* al, const double* bm, const double* b, double* x, double tol,
+                  int maxit, or_stats* st) {
   memset(st, 0, sizeof(*st));
   for (int i = 0; i < neq; i++)
     if (ad[i] == 0.0) {
@@ -1299,12 +1346,12 @@ void or_pcg_bdiag3(int neq, const int32_t* ia, const int32_t* ja, const double* 
   double* r = (double*)malloc(sizeof(double) * (size_t)neq);
   double* p = (double*)malloc(sizeof(double) * (size_t)neq);
   for (int i = 0; i < neq; i++) x[i] = 0.0;
-  or_op_block_precond3(bm, b, z, neq);
+  op_block(nblock, bm, b, z, neq);
   double d = or_dot(b, z, neq);
   double conv = tol * sqrt(fabs(d));
   or_matvec_csrcsym1(neq, ia, ja, ad, al, x, z);
   for (int i = 0; i < neq; i++) r[i] = b[i] - z[i];
-  or_op_block_precond3(bm, r, z, neq);
+  op_block(nblock, bm, r, z, neq);
   for (int i = 0; i < neq; i++) p[i] = z[i];
   d = or_dot(r, z, neq);
   int j, conv_ok = 0;
@@ -1315,7 +1362,7 @@ void or_pcg_bdiag3(int neq, const int32_t* ia, const int32_t* ja, const double* 
       x[i] = x[i] + alpha * p[i];
       r[i] = r[i] - alpha * z[i];
     }
-    or_op_block_precond3(bm, r, z, neq);
+    op_block(nblock, bm, r, z, neq);
     double di = or_dot(r, z, neq);
     double beta = di / d;
     for (int i = 0; i < neq; i++) p[i] = z[i] + beta * p[i];
@@ -1331,7 +1378,7 @@ void or_pcg_bdiag3(int neq, const int32_t* ia, const int32_t* ja, const double* 
   st->xkx = or_dot(x, z, neq);
   st->norm_x = sqrt(or_dot(x, x, neq));
   for (int i = 0; i < neq; i++) r[i] = b[i] - z[i];
-  or_op_block_precond3(bm, r, z, neq);
+  op_block(nblock, bm, r, z, neq);
   st->norm_m_r = sqrt(fabs(or_dot(r, z, neq)));
   st->norm_r = sqrt(or_dot(r, r, neq));
   st->conv = conv;

@@ -103,8 +103,19 @@ void or_pcg(int neq, const int32_t* ia, const int32_t* ja, const double* ad,
             const double* al, const double* m, const double* b, double* x, double tol,
             int maxit, or_stats* st);
 
+/* Precond.f:163-358 nblock=2: m(3, neq/2 + rest) */
+void or_block_precond2(const double* ad, const double* al, const int32_t* ia,
+                       const int32_t* ja, double* m, int neq);
+/* Precond.f:419-540 nblock=2 */
+void or_op_block_precond2(const double* m, const double* x, double* y, int neq);
+
 /* Iters.f:252-524 with z = M^-1 r applied by op_block_precond (Precond.f:419-540)
- * on bm = m(6, .) of or_block_precond3 (3x3 block Jacobi, precond code 6). */
+ * on bm = m(nblock(nblock+1)/2, .) of or_block_precond2/3 (block Jacobi,
+ * precond code 6 with block size 2 or 3). */
+void or_pcg_block(int nblock, int neq, const int32_t* ia, const int32_t* ja, const double* ad,
+                  const double* al, const double* bm, const double* b, double* x, double tol,
+                  int maxit, or_stats* st);
+/* or_pcg_block with nblock = 3 */
 void or_pcg_bdiag3(int neq, const int32_t* ia, const int32_t* ja, const double* ad,
                    const double* al, const double* bm, const double* b, double* x, double tol,
                    int maxit, or_stats* st);

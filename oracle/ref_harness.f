@@ -19,6 +19,8 @@ c *             result files of write_mesh_res_mec (Write_vtk.f:742):  *
 c *             legacy ASCII, legacy binary and XML ASCII              *
 c *   mode 64 = set-up only: stop after the set-up dumps and the     *
 c *             product (no solve; for the full-size workloads)        *
+c *   mode 256 = + block_precond nblock=2 (2x2 inverses of equation    *
+c *             pairs, Precond.f:163-358) and op_block_precond         *
 c *   mode 128 = solve only: `deck` is a directory holding a CSRC      *
 c *             system (sys.txt: neq nad; ia.i4 ja.i4 ad.f8 al.f8      *
 c *             b.f8), solved by solv (Solv.f:13) with nth threads;    *
@@ -30,7 +32,8 @@ c * Output (raw little-endian stream files in outdir):                 *
 c *   meta.txt, inum.i4, id.i4, ia.i4, ja.i4, ad.f8, al.f8, b.f8,      *
 c *   elcolor.i4, colorg.i4 (coloredmesh, Colormsh.f:9-63),            *
 c *   x.f8, spmv_v.f8, spmv_y.f8, resid.f8, and optionally             *
-c *   x_gmres.f8, x_bicgstab.f8, bdiag.f8, bdiag_y.f8, tx.f8, u.f8,    *
+c *   x_gmres.f8, x_bicgstab.f8, bdiag.f8, bdiag_y.f8, bdiag2.f8,      *
+c *   bdiag2_y.f8, tx.f8, u.f8,                                        *
 c *   and the files                                                    *
 c *   res_istep_0.vtk, resb_istep_0.vtk, res_istep_0.vtu               *
 c **********************************************************************
@@ -329,6 +332,26 @@ c ... one application y = M^-1 v (Precond.f:419), v as for the product
         call wf8(trim(outd)//'/bdiag_y.f8',ia(i_y),neq)
         i_y = dealloc('bdy     ')
         i_v = dealloc('bdv     ')
+      endif
+c ... 2x2 block-Jacobi set-up and one application (Precond.f:163, 419)
+      if (iand(mode,256) .ne. 0) then
+        param(1) = 0
+        param(2) = 0
+        param(3) = 3
+        param(4) = 2
+        mm = neq/2 + mod(neq,2)
+        i_bd  = alloc_8('bdiag2  ',3,mm)
+        i_aux = alloc_8('baux2   ',2,2)
+        call block_precond(ia(i_ad),ia(i_al),ia(i_ia),ia(i_ja),
+     .       ia(i_bd),neq,ia(i_aux),param)
+        nbd = 3*int(mm)
+        call wf8(trim(outd)//'/bdiag2.f8',ia(i_bd),nbd)
+        i_v = alloc_8('bd2v    ',1,neq)
+        i_y = alloc_8('bd2y    ',1,neq)
+        call fillv(ia(i_v),neq)
+        call azero(ia(i_y),neq)
+        call op_block_precond(ia(i_bd),ia(i_v),ia(i_y),ia(i_aux),param)
+        call wf8(trim(outd)//'/bdiag2_y.f8',ia(i_y),neq)
       endif
 c ... nodal stresses and the result files (Main.f:930-963), after the
 c     displacements of the solve went to u (update_mec above)

@@ -4,7 +4,8 @@ The reference (Fortran, /root/reference/mef/src) is compiled by
 oracle/Makefile.ref into oracle/_ref/ref_harness, a test-only driver that runs
 the reference's own mesh/solvm path (RCM, equation numbering, CSRC structure,
 element stiffness + load assembly, PCG through ``solv``) and additionally calls
-``gmres``, ``pbicgstab`` and ``block_precond`` with their own signatures, and
+``gmres``, ``pbicgstab`` and ``block_precond`` (3x3 and 2x2 blocks) with their
+own signatures, and
 ``tform_mec`` + ``write_mesh_res_mec`` for the nodal stresses and result files.
 For every case this script writes a deck with elastic_amd.deck, runs the
 harness, and stores its outputs as a small .npz fixture (large cases keep
@@ -53,7 +54,7 @@ def run_harness(deck_path, tol, maxit, mode, workdir, nth=1, maxmem=400000000):
     for name in ("elcolor", "colorg"):
         out[name] = rd(name + ".i4", np.int32)
     for name in ("x", "spmv_v", "spmv_y", "resid", "x_gmres", "x_bicgstab", "bdiag", "bdiag_y",
-                 "tx", "u"):
+                 "bdiag2", "bdiag2_y", "tx", "u"):
         p = os.path.join(workdir, name + ".f8")
         if os.path.exists(p):
             out[name] = np.fromfile(p, dtype=np.float64)

@@ -162,14 +162,19 @@ def test_batch_slot_full_and_errors(golden):
         c.close()
 
 
-@pytest.mark.parametrize("mode", ["batched", "sequential"])
+@pytest.mark.parametrize("mode", ["batched", "sequential", "policy"])
 def test_server_small_stream(mode):
+    """The request server on a small stream: every x within 1e-8 of an FP64
+    solve; with the per-request policy the plane requests run FP64 with 2x2
+    node-block Jacobi; latencies from arrival >= from admission >= from
+    submission."""
     from elastic_amd.serving import Server, request_stream
     from elastic_amd.system import System
 
     reqs = request_stream(10, rate=2000.0, nmin=8, nmax=48)
-    srv = Server(max_slots=4, iters_per_step=8, prep_workers=3, precision=MIXED if mode == "batched"
-                 else FP64, tol=1e-10, mode=mode, keep_x=True)
+    prec = {"batched": MIXED, "sequential": FP64, "policy": None}[mode]
+    srv = Server(max_slots=4, iters_per_step=8, prep_workers=3, precision=prec, tol=1e-10,
+                 mode="sequential" if mode == "sequential" else "batched", keep_x=True)
     res = srv.run(reqs, realtime=True)
     assert res["requests"] == 10 and res["maxit_hits"] == 0
     for r in res["done"]:
@@ -177,7 +182,10 @@ def test_server_small_stream(mode):
         x64, st = s.ctx.solve(s.b, tol=1e-12)
         assert rel(r.x, x64) <= 1e-8
         s.ctx.close()
+        if mode == "policy":
+            assert (r.precision, r.precond) == (FP64, 6)
     assert res["latency_ms_p99"] >= res["latency_ms_p50"] > 0
+    assert res["latency_ms_p50"] >= res["service_ms_p50"] >= res["solve_ms_p50"] > 0
 
 
 def test_batch_fp32_accumulation_instances(golden):
@@ -225,3 +233,32 @@ def test_batch_hand_over_to_fp64(monkeypatch):
         assert np.array_equal(xa, x) and sa["iters"] == st["iters"]
     for c in ctxs:
         c.close()
+
+
+@pytest.mark.parametrize("name", ["quad4_8", "tria3_8", "hex8_5j"])
+def test_batch_node_block_jacobi(golden, name):
+    """Instances whose context asks for node-block Jacobi ("batch_precond"
+    6: the 2x2 or 3x3 inverses of block_precond, Precond.f:163-540, for the
+    layout's node size) follow the oracle's block-Jacobi PCG: FP64
+    iterations within 2 and x within 1e-10 of its x; mixed-precision x
+    within 1e-8; alone and in a mix with Jacobi instances bit-identical."""
+    from oracle import binding as O
+
+    g = golden(name)
+    nb = 3 if name.startswith("hex") else 2
+    xo, so = O.pcg_block(nb, g["ia"], g["ja"], g["ad"], g["al"], g["b"], tol=1e-10)
+    assert so["status"] == 0
+    c = make_ctx(g)
+    c.set_option("batch_precond", 6)
+    print("batch bdiag", name, flush=True)
+    x, st = run_batch([c], FP64, 1e-10)[0]
+    assert st["status"] == 0 and abs(st["iters"] - so["iters"]) <= 2
+    assert rel(x, xo) <= 1e-10
+    others = [make_ctx(golden(n)) for n in ("hex8_4", "quad4_6ps")]
+    mix = run_batch(others + [c], FP64, 1e-10, stagger=2)
+    xm, sm = mix[2]
+    assert np.array_equal(xm, x) and sm["iters"] == st["iters"]
+    xx, sx = run_batch([c], MIXED, 1e-10)[0]
+    assert sx["status"] == 0 and rel(xx, g["x"]) <= 1e-8
+    for o in others + [c]:
+        o.close()

@@ -2,7 +2,8 @@
 tests only checked final solutions.
 
 * 3x3 block Jacobi (Precond.f:163-358, 1676-1710): the device's block inverse
-  equals the reference's block_precond m(6, .) bit for bit, and PCG with it
+  equals the reference's block_precond m(6, .) bit for bit (and the 2x2
+  inverses of the 2-D meshes its m(3, .) with nblock = 2), and PCG with it
   follows the oracle's block-Jacobi PCG (its application restates
   op_block_precond, Precond.f:419-540, pinned to the reference's output):
   iterations within 2 and x within 1e-10 relative L2 (FP64), 1e-8 (mixed).
@@ -59,6 +60,16 @@ def test_block_inverse_is_the_references(ctx, golden, name):
     m = ctx.get_bdiag()
     assert m.shape == g["bdiag"].shape
     assert np.array_equal(m, g["bdiag"])
+
+
+@pytest.mark.parametrize("name", [n for n in SMALL if CASES[n][2] & 256])
+def test_block_inverse2_is_the_references(ctx, golden, name):
+    """block_precond with nblock = 2 (the 2x2 node blocks of 2-D meshes)."""
+    g = golden(name)
+    load(ctx, g)
+    m = ctx.get_block_precond(2)
+    assert m.shape == g["bdiag2"].shape
+    assert np.array_equal(m, g["bdiag2"])
 
 
 @pytest.mark.parametrize("name", ["hex8_4", "hex8_5j", "tet4_3j", "quad4_8", "tria3_8"])

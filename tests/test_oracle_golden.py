@@ -127,6 +127,15 @@ def test_op_block_precond_bit_exact(name, golden):
     assert np.array_equal(y, g["bdiag_y"])
 
 
+@pytest.mark.parametrize("name", [n for n in SMALL if CASES[n][2] & 256])
+def test_block_jacobi2_bit_exact(name, golden, systems):
+    """block_precond and op_block_precond with nblock = 2 (Precond.f:163-358,
+    419-540) restated: the reference's m(3, .) and its application."""
+    g, s = golden(name), systems(name)
+    assert np.array_equal(O.block_precond2(s.ad, s.al, s.ia, s.ja), g["bdiag2"])
+    assert np.array_equal(O.op_block_precond2(g["bdiag2"], g["spmv_v"]), g["bdiag2_y"])
+
+
 def dense(ia, ja, ad, al):
     n = ad.shape[0]
     A = np.diag(ad.copy())
@@ -153,6 +162,26 @@ def test_block_precond_is_the_block_inverse(name, systems):
         z[sl] = np.linalg.solve(A[sl, sl], v[sl])
     z[3 * nb:] = v[3 * nb:] / s.ad[3 * nb:]
     assert np.linalg.norm(y - z) <= 1e-13 * np.linalg.norm(z)
+
+
+@pytest.mark.parametrize("name", ["quad4_8", "tria3_8", "req1"])
+def test_block_precond2_is_the_block_inverse(name, systems):
+    """The 2x2 set-up and application = the inverse of each diagonal 2x2 block
+    of consecutive equations (the node blocks of these 2-D meshes)."""
+    s = systems(name)
+    A = dense(s.ia, s.ja, s.ad, s.al[: s.nad])
+    n = s.neq
+    v = 1.0 + np.arange(n) % 7 / 8.0 + np.cos(np.arange(n))
+    y = O.op_block_precond2(O.block_precond2(s.ad, s.al, s.ia, s.ja), v)
+    z = np.empty(n)
+    for i in range(n // 2):
+        sl = slice(2 * i, 2 * i + 2)
+        z[sl] = np.linalg.solve(A[sl, sl], v[sl])
+    z[2 * (n // 2):] = v[2 * (n // 2):] / s.ad[2 * (n // 2):]
+    assert np.linalg.norm(y - z) <= 1e-13 * np.linalg.norm(z)
+    x, st = O.pcg_block(2, s.ia, s.ja, s.ad, s.al, s.b, tol=1e-12)
+    assert st["status"] == 0
+    assert np.linalg.norm(x - np.linalg.solve(A, s.b)) <= 1e-9 * np.linalg.norm(x)
 
 
 @pytest.mark.parametrize("name", ["hex8_5j", "tet4_3j", "quad4_8"])
