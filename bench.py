@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="length of the one-core sample of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default layout (1024-block tiles, 256 lanes)")
     ap.add_argument("--serving", type=int, default=512,
                     help="requests of the plane-strain serving stream (0: skip)")
     ap.add_argument("--serving-fixed", action="store_true",
@@ -110,6 +112,17 @@ def main():
     x_ref = ctx.download_x()
 
     phase("reference solve done")
+    # layout autotuner (untimed set-up): tile size and lanes per tile of the
+    # FP32-value operator timed inside short mixed PCG runs, then the lanes of
+    # the FP64 operator on the chosen tiles
+    tuned = None
+    if not args.no_autotune:
+        tile32, ms32 = ctx.autotune(precision=MIXED, reps=32)
+        _, ms64 = ctx.autotune(precision=FP64, reps=32, tiles=False)
+        tuned = {"tile_blocks": tile32, "lanes32": ctx.get_option("lanes32"),
+                 "lanes64": ctx.get_option("lanes64"), "fp32_product_ms": ms32,
+                 "fp64_product_ms": ms64}
+        phase("autotuned: %s" % tuned)
     # plain FP64 GPU PCG (the drop-in), timed.  The product of every 4th
     # iteration is bracketed by HIP events on the solve's stream, so the
     # kernel's roofline comes from the timed solves themselves
@@ -247,13 +260,15 @@ def main():
                             "external_nodes": info["ext_partials"]},
                  "standalone": {"fp32_kernel_ms": ker32, "fp32_with_sum_and_merge_ms": tot32,
                                 "fp64_kernel_ms": ker64, "fp64_with_sum_and_merge_ms": tot64}},
+        "autotune": tuned,
         "assembly_ms": t_asm,
         "setup_s": t_setup,
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs32 / HBM_PEAK_GBS, "bytes_per_launch": m32,
                      "traffic": measured_traffic(neq, nad, m32),
-                     "kernel": ("k_spmv_blocks<float,%s,%d> with tile shares of p.q"
-                                % ("float" if acc32 else "double", nb))
+                     "kernel": ("k_spmv_blocks<float,%s,%d,PER,%d lanes> with tile shares of p.q"
+                                % ("float" if acc32 else "double", nb,
+                                   ctx.get_option("lanes32")))
                                if nb > 1 else "k_spmv_tiles<float,double>"},
         "copy_probe": copy,
         "cpu_baseline": cpu,

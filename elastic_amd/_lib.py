@@ -81,6 +81,7 @@ def lib():
     L.elastic_set_tile_nnz.argtypes = [vp, C.c_int32]
     L.elastic_set_tile_blocks.argtypes = [vp, C.c_int32]
     L.elastic_set_option.argtypes = [vp, C.c_char_p, C.c_int64]
+    L.elastic_get_option.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
     L.elastic_product_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                          C.c_int32]
     L.elastic_autotune.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32),

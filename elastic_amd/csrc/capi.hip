@@ -109,6 +109,12 @@ using namespace elastic;
 // tiles grow to the longest CSRC row; a row of more than
 // kTileColsMax - kTileRowsMax lower entries takes the gather-only product
 // (spmv_gather.hip), which holds any pattern.
+// node blocks per tile the layout is built with (tile_blocks 0: by node size)
+static int effective_tile_blocks(const Context* c, int nb) {
+  if (c->tile_blocks > 0) return c->tile_blocks;
+  return nb == 3 && c->neq / 3 >= kLargeTileNodes ? kTileBlocks3 : kTileBlocks2;
+}
+
 static bool getenv_is(const char* name, const char* value) {
   const char* v = getenv(name);
   return v && std::string(v) == value;
@@ -126,10 +132,7 @@ static int build_layout(elastic_ctx* x) {
   c->layout_note.clear();
   if (nb > 1) {
     try {
-      build_block_tiles(neq, nb, ia, ja,
-                        c->tile_blocks > 0 ? c->tile_blocks
-                        : (nb == 3 && neq / 3 >= kLargeTileNodes ? kTileBlocks3 : kTileBlocks2),
-                        c->blayout);
+      build_block_tiles(neq, nb, ia, ja, effective_tile_blocks(c, nb), c->blayout);
     } catch (const std::exception& e) {
       c->layout_note = std::string("scalar tiles: ") + e.what();
       c->blayout = BlockLayout();
@@ -438,34 +441,107 @@ int elastic_set_structure(elastic_ctx* x, int32_t neq, int64_t nad, const int32_
   return ELASTIC_OK;
 }
 
-// Autotuner: times the product (tile kernel + merge, HIP events) for each
-// candidate tile size on the resident values and keeps the fastest layout.
+namespace elastic {
+int solve_resident(elastic_ctx* x, int solver, int precond, int precision, double tol, int maxit,
+                   int ngram, elastic_stats* st);
+}
+
+// Average device time of the products of a short resident PCG run of
+// `iters` iterations (b = ones; the caller's b is kept and put back; x is
+// overwritten), sampled with HIP events on every iteration.
+static int time_in_solve(elastic_ctx* x, int precision, int iters, double* ms) {
+  Context* c = &x->c;
+  const size_t nb = sizeof(double) * (size_t)c->neq;
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->v64[VW], c->v64[VB], nb, hipMemcpyDeviceToDevice, c->stream));
+  const std::vector<double> ones((size_t)c->neq, 1.0);
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->v64[VB], ones.data(), nb, hipMemcpyHostToDevice, c->stream));
+  const int stride = c->time_stride;
+  const double tsum = c->tprod_ms;
+  const int64_t tn = c->tprod_n;
+  c->time_stride = 1;
+  c->tprod_ms = 0.0;
+  c->tprod_n = 0;
+  elastic_stats st;
+  int rc = solve_resident(x, ELASTIC_SOLVER_PCG, ELASTIC_PRECOND_DIAG, precision, 1e-300, iters,
+                          50, &st);
+  *ms = c->tprod_n ? c->tprod_ms / (double)c->tprod_n : 1e300;
+  c->time_stride = stride;
+  c->tprod_ms = tsum;
+  c->tprod_n = tn;
+  ELASTIC_HIP_CHECK(c, hipMemcpyAsync(c->v64[VB], c->v64[VW], nb, hipMemcpyDeviceToDevice, c->stream));
+  ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+// Autotuner: for each candidate layout (node-blocked: tile size x lanes per
+// tile; scalar rows: tile size) runs a short resident PCG of the given
+// precision and keeps the candidate whose products are fastest on average.
+// The tile size is shared by both operators (one layout); the lanes are
+// kept per precision.  With the option "tune_tiles" = 0 only the lanes are
+// tuned.
 int elastic_autotune(elastic_ctx* x, int32_t precision, int32_t reps, int32_t* best,
                      double* best_ms) {
   if (!x || reps <= 0) return ELASTIC_BAD_ARG;
   Context* c = &x->c;
   if (!c->values_set) return set_error(c, ELASTIC_NOT_READY, "values not set");
+  if (c->gather) return set_error(c, ELASTIC_BAD_ARG, "the gather product has no tiles to tune");
   hipSetDevice(c->device);
   const bool blocked = c->layout_nb > 1;
-  const std::vector<int> cands = blocked ? std::vector<int>{128, 256, 384, 512, 768, 1024}
-                                         : std::vector<int>{1024, 2048, 3072, 3840};
-  int arg = -1;
+  const int pi = precision == ELASTIC_MIXED ? 1 : 0;
+  auto current = [&]() { return blocked ? effective_tile_blocks(c, c->layout_nb) : c->tile_nnz; };
+  // the current layout is timed first and a candidate replaces the best
+  // only when faster by more than 2%: timing noise does not move the layout
+  std::vector<int> tiles = {current()};
+  if (c->tune_tiles)
+    for (int v : blocked ? std::vector<int>{256, 512, 768, 1024}
+                         : std::vector<int>{1024, 2048, 3072, 3840})
+      if (v != tiles[0]) tiles.push_back(v);
+  const int iters = std::max(16, (int)reps);
+  const bool verbose = getenv("ELASTIC_VERBOSE") != nullptr;
+  int bt = -1, bl = 256;
   double tmin = 1e300;
-  auto apply = [&](int v) -> int {
-    return blocked ? relayout(x, v, c->tile_nnz) : relayout(x, c->tile_blocks, v);
-  };
-  for (int v : cands) {
-    int rc = apply(v);
-    if (rc) return rc;
-    double tot = 0, ker = 0;
-    rc = elastic_time_spmv(x, precision, reps, &tot, &ker);
-    if (rc) return rc;
-    if (tot < tmin) tmin = tot, arg = v;
+  for (int v : tiles) {
+    if (v != current()) {
+      int rc = blocked ? relayout(x, v, c->tile_nnz) : relayout(x, c->tile_blocks, v);
+      if (rc) return rc;
+    }
+    for (int lanes : {256, 512}) {
+      if (lanes == 512 && (!blocked || c->blk.nblk_max <= 256)) continue;
+      c->lanes[pi] = lanes;
+      double ms = 0.0;
+      int rc = time_in_solve(x, precision, iters, &ms);
+      if (rc && rc != ELASTIC_MAXIT) return rc;
+      if (verbose) fprintf(stderr, "elastic_autotune: tile %d lanes %d product %.4f ms\n", v, lanes, ms);
+      if (ms < 0.98 * tmin) tmin = ms, bt = v, bl = lanes;
+    }
   }
-  int rc = apply(arg);
-  if (rc) return rc;
-  if (best) *best = arg;
+  c->lanes[pi] = bl;
+  if (bt > 0 && bt != current()) {
+    int rc = blocked ? relayout(x, bt, c->tile_nnz) : relayout(x, c->tile_blocks, bt);
+    if (rc) return rc;
+  }
+  if (best) *best = current();
   if (best_ms) *best_ms = tmin;
+  return ELASTIC_OK;
+}
+
+// Current value of a tuning knob (names of elastic_set_option, plus
+// "lanes64"/"lanes32" and "layout_nb").
+int elastic_get_option(elastic_ctx* x, const char* name, int64_t* value) {
+  if (!x || !name || !value) return ELASTIC_BAD_ARG;
+  const Context* c = &x->c;
+  const std::string n(name);
+  if (n == "accum32") *value = c->accum32 ? 1 : 0;
+  else if (n == "batch_precond") *value = c->batch_precond;
+  else if (n == "check_every") *value = c->check_every;
+  else if (n == "time_products") *value = c->time_stride;
+  else if (n == "tile_blocks") *value = effective_tile_blocks(c, c->layout_nb > 1 ? c->layout_nb : 3);
+  else if (n == "tile_nnz") *value = c->tile_nnz;
+  else if (n == "lanes64") *value = c->lanes[0];
+  else if (n == "lanes32") *value = c->lanes[1];
+  else if (n == "tune_tiles") *value = c->tune_tiles ? 1 : 0;
+  else if (n == "layout_nb") *value = c->gather ? 0 : c->layout_nb;
+  else return set_error((Context*)c, ELASTIC_BAD_ARG, "unknown option " + n);
   return ELASTIC_OK;
 }
 
@@ -482,6 +558,11 @@ int elastic_set_option(elastic_ctx* x, const char* name, int64_t value) {
     // Jacobi with the layout's node blocks (2x2 in 2-D, 3x3 in 3-D)
     if (value != ELASTIC_PRECOND_DIAG && value != ELASTIC_PRECOND_BDIAG3) return ELASTIC_BAD_ARG;
     c->batch_precond = (int)value;
+  } else if (n == "lanes64" || n == "lanes32") {
+    if (value != 256 && value != 512) return ELASTIC_BAD_ARG;
+    c->lanes[n == "lanes32" ? 1 : 0] = (int)value;
+  } else if (n == "tune_tiles") {
+    c->tune_tiles = value != 0;
   } else if (n == "check_every") {
     if (value < 1 || value > 4096) return ELASTIC_BAD_ARG;
     c->check_every = (int)value;

@@ -179,6 +179,7 @@ struct Context {
   // lanes per tile of the node-blocked product (256 or 512), FP64 [0] and
   // FP32-value [1] operators
   int lanes[2] = {256, 256};
+  bool tune_tiles = true;      // elastic_autotune also tries tile sizes (else lanes only)
   int layout_nb = 1;           // 1: scalar tiles; 2 or 3: node-blocked tiles
   bool accum32 = false;        // FP32 operator: FP32 staging and sums (else FP64)
   // -1: accum32 follows the system (on for 3-dof layouts of at least

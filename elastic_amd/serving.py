@@ -144,6 +144,7 @@ class Request:
     t_done: float = 0.0
     precision: int = 0
     precond: int = 0
+    tuned: bool = False
     stats: dict = field(default_factory=dict)
     x: np.ndarray | None = None
 
@@ -166,12 +167,15 @@ class Server:
     precision None (the default): precision and preconditioner per request
     by request_policy; FP64 or MIXED: that precision with Jacobi for every
     request.
+    autotune True: the first request of each size class (node block size,
+    precision and the bit length of neq) runs elastic_autotune before its
+    solve; later requests of the class take the tile size and lanes it chose.
     """
 
     def __init__(self, device: int = 0, max_slots: int = 64, iters_per_step: int = 16,
                  hbm_budget: float = 64e9, prep_workers: int = 8, precision: int | None = None,
                  tol: float = 1e-10, maxit: int = 50000, mode: str = "batched",
-                 keep_x: bool = False, progress=None):
+                 keep_x: bool = False, progress=None, autotune: bool = False):
         self.device = device
         self.max_slots = max_slots
         self.iters_per_step = iters_per_step
@@ -183,6 +187,26 @@ class Server:
         self.mode = mode
         self.keep_x = keep_x
         self.progress = progress  # a file (e.g. sys.stderr) for a status line every 5 s
+        self.autotune = autotune
+        self.tuned = {}           # size class -> (tile size, lanes)
+
+    def _tune(self, ctx, prec: int):
+        """Layout of the request's size class (see the class text)."""
+        nb = ctx.layout_info()["nb"]
+        if not self.autotune or nb == 0:    # the gather product has no tiles
+            return
+        tile_opt = "tile_blocks" if nb > 1 else "tile_nnz"
+        lanes_opt = "lanes32" if prec == MIXED else "lanes64"
+        key = (nb, prec, int(ctx.neq).bit_length())
+        hit = self.tuned.get(key)
+        if hit is None:
+            tile, _ = ctx.autotune(precision=prec, reps=16)
+            self.tuned[key] = (tile, ctx.get_option(lanes_opt))
+            return
+        tile, lanes = hit
+        if ctx.get_option(tile_opt) != tile:
+            ctx.set_option(tile_opt, tile)
+        ctx.set_option(lanes_opt, lanes)
 
     def run(self, requests: list[Request], realtime: bool = True) -> dict:
         """Serves the stream; realtime=False makes every request arrive at 0."""
@@ -245,6 +269,7 @@ class Server:
                         r, s = ready.pop(0)
                         r.t_submit = now()
                         prec = self.precision if self.precision is not None else FP64
+                        self._tune(s.ctx, prec)
                         st = s.ctx.solve_resident(precision=prec, tol=self.tol,
                                                   maxit=self.maxit)
                         finish(r, s, st)
@@ -258,6 +283,9 @@ class Server:
                             s.ctx.set_option("batch_precond", pre)
                         else:
                             prec, pre = self.precision, 2
+                        if not r.tuned:
+                            self._tune(s.ctx, prec)
+                            r.tuned = True
                         if not batch.submit(s.ctx, prec, self.tol, self.maxit, r.k):
                             break
                         ready.pop(0)

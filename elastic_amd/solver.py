@@ -94,6 +94,11 @@ class Context:
         """Tuning knob of the context (see elastic_set_option)."""
         self._chk(self._L.elastic_set_option(self._h, name.encode(), int(value)))
 
+    def get_option(self, name: str) -> int:
+        v = C.c_int64()
+        self._chk(self._L.elastic_get_option(self._h, name.encode(), C.byref(v)))
+        return v.value
+
     def product_timing(self, reset: bool = True):
         """(average ms, count) of the PCG iteration products sampled under
         the "time_products" option since the last reset."""
@@ -104,9 +109,12 @@ class Context:
     def set_tile_blocks(self, tile_blocks: int):
         self._chk(self._L.elastic_set_tile_blocks(self._h, tile_blocks))
 
-    def autotune(self, precision: int = FP64, reps: int = 20):
-        """Picks the fastest tile size for the resident values (rebuilds the
-        layout); returns (tile size, product time in ms)."""
+    def autotune(self, precision: int = FP64, reps: int = 32, tiles: bool = True):
+        """Picks the fastest tile size (unless tiles=False) and lanes per tile
+        for the resident values, timed inside a short PCG run (rebuilds the
+        layout; the resident x is overwritten); returns (tile size, product
+        time in ms)."""
+        self.set_option("tune_tiles", 1 if tiles else 0)
         best, ms = C.c_int32(), C.c_double()
         self._chk(self._L.elastic_autotune(self._h, precision, reps, C.byref(best), C.byref(ms)))
         return best.value, ms.value

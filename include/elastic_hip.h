@@ -263,6 +263,9 @@ int elastic_set_tile_blocks(elastic_ctx* ctx, int32_t tile_blocks);
  * less accurate, which ill-conditioned systems may not tolerate; 0: in
  * FP64; until set, 1 for PCG on 3-dof systems of at least 32,768 nodes,
  * else 0),
+ * "lanes64" / "lanes32" (256 or 512 lanes per tile of the node-blocked FP64 /
+ * FP32-value product, 4 or 2 blocks per lane; default 256), "tune_tiles"
+ * (0: elastic_autotune keeps the tile size),
  * "check_every" (iterations between convergence polls),
  * "batch_precond" (2: the context's batched solves use Jacobi, the default;
  * 6: node-block Jacobi, the inverses of block_precond with the layout's node
@@ -276,12 +279,23 @@ int elastic_set_option(elastic_ctx* ctx, const char* name, int64_t value);
  * counterpart in the reference (its matvectime timer, Iters.f:383-386, is
  * the closest). */
 int elastic_product_timing(elastic_ctx* ctx, double* avg_ms, int64_t* count, int32_t reset);
-/* Autotuner (no counterpart in the reference): with values resident, times
- * the product of the given precision for each candidate tile size (HIP
- * events, reps launches each), rebuilds the layout with the fastest and
- * returns its size and time.  Not while the context is in a batch. */
+/* Autotuner (no counterpart in the reference): with values resident, runs
+ * a short resident PCG of the given precision (max(16, reps) iterations, b
+ * = ones; the caller's resident b is put back, the resident x is
+ * overwritten) for each candidate layout -- node-blocked: 256, 512, 768 and
+ * 1024 blocks per tile, each with 256 and 512 lanes per tile (4 or 2 blocks
+ * per lane); scalar rows: 1024 to 3840 nonzeros per tile -- and keeps the
+ * candidate whose products are fastest on average (HIP events around every
+ * product of the run).  The tile size is shared by both operators; the lanes
+ * are kept for the given precision.  With the option "tune_tiles" = 0 only
+ * the lanes are tuned.  Returns the tile size and the product time.  Not
+ * while the context is in a batch. */
 int elastic_autotune(elastic_ctx* ctx, int32_t precision, int32_t reps, int32_t* best,
                      double* best_ms);
+/* Current value of a knob of elastic_set_option, or "lanes64" / "lanes32"
+ * (lanes per tile of the FP64 / FP32-value product) or "layout_nb" (node
+ * block size of the layout, 1 scalar rows, 0 the gather product). */
+int elastic_get_option(elastic_ctx* ctx, const char* name, int64_t* value);
 
 /* ------------------------------------------------------ batched serving */
 /* No counterpart in the reference (its solver serves one system per call).

@@ -162,20 +162,28 @@ def test_batch_slot_full_and_errors(golden):
         c.close()
 
 
-@pytest.mark.parametrize("mode", ["batched", "sequential", "policy"])
+@pytest.mark.parametrize("mode", ["batched", "sequential", "policy", "tuned", "tuned_seq"])
 def test_server_small_stream(mode):
     """The request server on a small stream: every x within 1e-8 of an FP64
     solve; with the per-request policy the plane requests run FP64 with 2x2
-    node-block Jacobi; latencies from arrival >= from admission >= from
+    node-block Jacobi; with the size-class autotuner every class is tuned once
+    and its layout kept; latencies from arrival >= from admission >= from
     submission."""
     from elastic_amd.serving import Server, request_stream
     from elastic_amd.system import System
 
     reqs = request_stream(10, rate=2000.0, nmin=8, nmax=48)
-    prec = {"batched": MIXED, "sequential": FP64, "policy": None}[mode]
+    prec = {"batched": MIXED, "sequential": FP64, "policy": None, "tuned": None,
+            "tuned_seq": FP64}[mode]
     srv = Server(max_slots=4, iters_per_step=8, prep_workers=3, precision=prec, tol=1e-10,
-                 mode="sequential" if mode == "sequential" else "batched", keep_x=True)
+                 mode="sequential" if mode in ("sequential", "tuned_seq") else "batched",
+                 keep_x=True, autotune=mode.startswith("tuned"))
     res = srv.run(reqs, realtime=True)
+    if mode.startswith("tuned"):
+        assert srv.tuned and len(srv.tuned) <= len(reqs)
+        for (nb, p, _), (tile, lanes) in srv.tuned.items():
+            assert lanes in (256, 512) and tile in ((256, 512, 768, 1024) if nb > 1
+                                                   else (1024, 2048, 3072, 3840))
     assert res["requests"] == 10 and res["maxit_hits"] == 0
     for r in res["done"]:
         s = System(r.make_mesh())

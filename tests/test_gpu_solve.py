@@ -215,19 +215,28 @@ def test_partially_restrained_nodes_use_scalar_tiles():
 
 @pytest.mark.parametrize("name", ["hex8_5j", "quad4_8"])
 def test_autotune_keeps_results(golden, name):
-    """The autotuner rebuilds the layout with the fastest tile size; the
-    product and the solve stay within the same bars."""
-    from elastic_amd.solver import MIXED, Context
+    """The autotuner rebuilds the layout with the fastest tile size and lanes
+    per tile (timed inside a short PCG); the product and the solves stay
+    within the same bars."""
+    from elastic_amd.solver import FP64, MIXED, Context
 
     g = golden(name)
     with Context(0) as c:
         load_system(c, g)
-        best, ms = c.autotune(precision=MIXED, reps=5)
-        assert best in (128, 256, 384, 512, 768, 1024) and ms > 0
+        best, ms = c.autotune(precision=MIXED, reps=16)
+        assert best in (256, 512, 768, 1024) and ms > 0
+        assert best == c.get_option("tile_blocks")
+        assert c.get_option("lanes32") in (256, 512)
         assert rel(c.spmv(g["spmv_v"]), g["spmv_y"]) <= 1e-13
         x, st = c.solve(g["b"], tol=float(g["tol"]))
         assert abs(st["iters"] - int(g["pcg_iters"])) <= 2
         assert rel(x, g["x"]) <= 1e-10
+        # lanes only: the tile size stays, the FP64 product still agrees
+        best64, _ = c.autotune(precision=FP64, reps=16, tiles=False)
+        assert best64 == best and c.get_option("lanes64") in (256, 512)
+        assert rel(c.spmv(g["spmv_v"]), g["spmv_y"]) <= 1e-13
+        xm, sm = c.solve(g["b"], precision=MIXED, tol=float(g["tol"]))
+        assert rel(xm, g["x"]) <= 1e-8
 
 
 @pytest.mark.parametrize("name", ["hex8_5j", "quad4_8", "tria3_8", "tet4_3j"])
