@@ -195,6 +195,7 @@ def main():
     gbs64 = m64 / (live64_ms * 1e-3) / 1e9
     t_asm = sysm.time_assembly(reps=3)
     copy = copy_probe(torch)
+    lanes32 = ctx.get_option("lanes32")  # the context is closed before the serving runs
 
     phase("product, assembly and copy timed")
     cpu, x_or = None, None
@@ -268,7 +269,7 @@ def main():
                      "traffic": measured_traffic(neq, nad, m32),
                      "kernel": ("k_spmv_blocks<float,%s,%d,PER,%d lanes> with tile shares of p.q"
                                 % ("float" if acc32 else "double", nb,
-                                   ctx.get_option("lanes32")))
+                                   lanes32))
                                if nb > 1 else "k_spmv_tiles<float,double>"},
         "copy_probe": copy,
         "cpu_baseline": cpu,
