@@ -364,13 +364,17 @@ def serving_bench(nreq: int, ncompare: int, fixed: bool = False):
     return out
 
 
-def plastic_compare(npaths: int, n: int = 60):
-    """The J2 workload's systems (hex8 60^3 between grips, E = 1, nu = 0.3,
-    H = 0.05, yield stresses U[5e-4, 2e-3], load in 10 increments): the paths
-    run increment 1 (batched mixed-precision Newton), then every path
-    assembles its tangent system of increment 2's first Newton iteration.
-    Those systems are solved to 1e-10 by plain FP64 PCG one after the other,
-    and together in a batch in FP64 and in mixed precision."""
+def plastic_compare(npaths: int, n: int = 60, increments: int = 10):
+    """The J2 workload (hex8 60^3 between grips, E = 1, nu = 0.3, H = 0.05,
+    yield stresses U[5e-4, 2e-3], load in 10 increments), run end to end: the
+    paths' batched mixed-precision Newton iterations (GPU re-assembly of K_t
+    and R, batched solves to 1e-11) through all increments, timed.  After
+    increment 1 the run pauses (outside the timed spans) for a solve-only
+    comparison: every path assembles its tangent system of increment 2's
+    first Newton iteration, and those systems are solved to 1e-10 by plain
+    FP64 PCG one after the other, and together in a batch in FP64 and in
+    mixed precision.  The run then resumes from the committed increment 1
+    (the comparison leaves u and the committed state untouched)."""
     from elastic_amd.plasticity import compare_solves, load_paths, run_paths
     from elastic_amd.serving import Batch
     from elastic_amd.solver import MIXED
@@ -378,22 +382,47 @@ def plastic_compare(npaths: int, n: int = 60):
     a = time.perf_counter()
     paths, _ = load_paths(n, npaths)
     t_setup = time.perf_counter() - a
-    with Batch(0, max_slots=max(1, min(npaths, 255)), iters_per_step=16) as b:
-        run_paths(paths, increments=10, precision=MIXED, batch=b, last=1)
+    neq = paths[0].sys.neq
+    counts1, counts2 = {}, {}
+    slots = max(1, min(npaths, 255))
+    with Batch(0, max_slots=slots, iters_per_step=16) as b:
+        a = time.perf_counter()
+        run_paths(paths, increments=increments, precision=MIXED, batch=b, last=1,
+                  counts=counts1, progress=sys.stderr)
+        t_inc1 = time.perf_counter() - a
     # plastic Gauss points of the committed state (committing the converged
     # state again changes nothing and returns the count)
     plastic = [int(p.commit()[1]) for p in paths]
     for p in paths:
-        p.increment(0.2)
+        p.increment(2.0 / increments)
         p.assemble(lhs=True)
-    print("plastic compare: %d systems of %d equations" % (npaths, paths[0].sys.neq),
+    print("plastic compare: %d systems of %d equations" % (npaths, neq),
           file=sys.stderr, flush=True)
     out = compare_solves(paths, tol=TOL)
     out["workload"] = ("J2 hex8 %d^3 tangent systems of increment 2 (first Newton iteration), "
-                       "%d load paths, %d equations each" % (n, npaths, paths[0].sys.neq))
+                       "%d load paths, %d equations each" % (n, npaths, neq))
     out["setup_s"] = t_setup
     out["plastic_points_after_increment_1"] = {"min": min(plastic), "max": max(plastic),
                                                "of": 8 * paths[0].mesh.numel}
+    with Batch(0, max_slots=slots, iters_per_step=16) as b:
+        a = time.perf_counter()
+        run_paths(paths, increments=increments, precision=MIXED, batch=b, first=2,
+                  counts=counts2, progress=sys.stderr)
+        t_rest = time.perf_counter() - a
+    wall = t_inc1 + t_rest
+    solves = counts1["linear_solves"] + counts2["linear_solves"]
+    plastic_end = [int(p.commit()[1]) for p in paths]
+    out["end_to_end"] = {
+        "workload": "J2 hex8 %d^3 pulled 5e-3 between grips, %d load paths, sy U[5e-4, 2e-3], "
+                    "H 0.05, %d increments, Newton to 1e-8, linear solves to 1e-11"
+                    % (n, npaths, increments),
+        "method": "batched mixed-precision PCG (FP32 operator, FP64 residual replacement), "
+                  "GPU re-assembly of K_t and R every Newton iteration",
+        "wall_s": wall, "linear_solves": solves,
+        "line_search_halvings": counts1["halvings"] + counts2["halvings"],
+        "solved_dof_per_s": neq * solves / wall,
+        "plastic_points_at_end": {"min": min(plastic_end), "max": max(plastic_end),
+                                  "of": 8 * paths[0].mesh.numel}}
     for p in paths:
         p.close()
     return out

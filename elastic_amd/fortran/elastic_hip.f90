@@ -150,5 +150,181 @@ module elastic_hip
       integer(c_int64_t) :: i_fmapi, i_xfi, i_rcvsi, i_dspli
       real(c_double) :: ad(*), au(*), al(*), m(*), b(*), x(*), tol
     end function
+    integer(c_int) function elastic_time_assembly(ctx, reps, ms) bind(C, name="elastic_time_assembly")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: reps
+      real(c_double), intent(out) :: ms
+    end function
+
+    ! J2 plasticity on hex8 meshes (state on the device; after elastic_set_mesh)
+    integer(c_int) function elastic_j2_setup(ctx, sy, h, ubar) bind(C, name="elastic_j2_setup")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: sy, h
+      real(c_double), intent(in) :: ubar(*)
+    end function
+    integer(c_int) function elastic_j2_increment(ctx, lambda) bind(C, name="elastic_j2_increment")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: lambda
+    end function
+    integer(c_int) function elastic_j2_assemble(ctx, lhs, rnorm) bind(C, name="elastic_j2_assemble")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: lhs
+      real(c_double), intent(out) :: rnorm
+    end function
+    integer(c_int) function elastic_j2_correct(ctx) bind(C, name="elastic_j2_correct")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function elastic_j2_step(ctx, s) bind(C, name="elastic_j2_step")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: s
+    end function
+    integer(c_int) function elastic_j2_commit(ctx, alpha_max, nplastic) bind(C, name="elastic_j2_commit")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: alpha_max
+      integer(c_int64_t), intent(out) :: nplastic
+    end function
+    integer(c_int) function elastic_j2_get_u(ctx, u) bind(C, name="elastic_j2_get_u")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: u(*)
+    end function
+    integer(c_int) function elastic_j2_get_residual(ctx, r) bind(C, name="elastic_j2_get_residual")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: r(*)
+    end function
+    ! ep and alpha are c_loc of the caller's arrays, or c_null_ptr
+    integer(c_int) function elastic_j2_get_state(ctx, trial, ep, alpha) bind(C, name="elastic_j2_get_state")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: trial
+      type(c_ptr), value :: ep, alpha
+    end function
+
+    ! block_precond (Precond.f:163-358): m(6, neq/3 + mod(neq,3)) or, for
+    ! bsz = 2, m(3, neq/2 + mod(neq,2))
+    integer(c_int) function elastic_get_bdiag(ctx, m) bind(C, name="elastic_get_bdiag")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: m(*)
+    end function
+    integer(c_int) function elastic_get_block_precond(ctx, bsz, m) &
+         bind(C, name="elastic_get_block_precond")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: bsz
+      real(c_double), intent(out) :: m(*)
+    end function
+
+    ! device pointers and a hipStream_t (c_null_ptr: the null stream)
+    integer(c_int) function elastic_spmv_csrc_noalloc(neq, nad, ia, ja, ad, al, x, y, stream) &
+         bind(C, name="elastic_spmv_csrc_noalloc")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      integer(c_int32_t), value :: neq
+      integer(c_int64_t), value :: nad
+      type(c_ptr), value :: ia, ja, ad, al, x, y, stream
+    end function
+
+    ! measurement and tuning (names are C strings: 'accum32'//c_null_char)
+    integer(c_int) function elastic_time_spmv(ctx, precision, reps, total_ms, kernel_ms) &
+         bind(C, name="elastic_time_spmv")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: precision, reps
+      real(c_double), intent(out) :: total_ms, kernel_ms
+    end function
+    integer(c_int) function elastic_layout_info(ctx, info4) bind(C, name="elastic_layout_info")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: info4(4)
+    end function
+    integer(c_int) function elastic_set_tile_nnz(ctx, tile_nnz) bind(C, name="elastic_set_tile_nnz")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tile_nnz
+    end function
+    integer(c_int) function elastic_set_tile_blocks(ctx, tile_blocks) &
+         bind(C, name="elastic_set_tile_blocks")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tile_blocks
+    end function
+    integer(c_int) function elastic_set_option(ctx, name, val) bind(C, name="elastic_set_option")
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int64_t), value :: val
+    end function
+    integer(c_int) function elastic_get_option(ctx, name, val) bind(C, name="elastic_get_option")
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int64_t), intent(out) :: val
+    end function
+    integer(c_int) function elastic_product_timing(ctx, avg_ms, count, reset) &
+         bind(C, name="elastic_product_timing")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: avg_ms
+      integer(c_int64_t), intent(out) :: count
+      integer(c_int32_t), value :: reset
+    end function
+    integer(c_int) function elastic_autotune(ctx, precision, reps, best, best_ms) &
+         bind(C, name="elastic_autotune")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: precision, reps
+      integer(c_int32_t), intent(out) :: best
+      real(c_double), intent(out) :: best_ms
+    end function
+
+    ! batched serving: many contexts iterated together (one host thread)
+    integer(c_int) function elastic_batch_create(b, device, max_slots, iters_per_step, use_graph) &
+         bind(C, name="elastic_batch_create")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), intent(out) :: b
+      integer(c_int32_t), value :: device, max_slots, iters_per_step, use_graph
+    end function
+    integer(c_int) function elastic_batch_destroy(b) bind(C, name="elastic_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+    type(c_ptr) function elastic_batch_last_error(b) bind(C, name="elastic_batch_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: b
+    end function
+    integer(c_int) function elastic_batch_submit(b, ctx, precision, tol, maxit, tag) &
+         bind(C, name="elastic_batch_submit")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: b, ctx
+      integer(c_int32_t), value :: precision, maxit
+      real(c_double), value :: tol
+      integer(c_int64_t), value :: tag
+    end function
+    integer(c_int) function elastic_batch_step(b) bind(C, name="elastic_batch_step")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+    integer(c_int) function elastic_batch_poll(b, tags, stats, cap, ndone) &
+         bind(C, name="elastic_batch_poll")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, elastic_stats
+      type(c_ptr), value :: b
+      integer(c_int64_t), intent(out) :: tags(*)
+      type(elastic_stats), intent(out) :: stats(*)
+      integer(c_int32_t), value :: cap
+      integer(c_int32_t), intent(out) :: ndone
+    end function
+    integer(c_int) function elastic_batch_active(b, nactive) bind(C, name="elastic_batch_active")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: b
+      integer(c_int32_t), intent(out) :: nactive
+    end function
   end interface
 end module elastic_hip

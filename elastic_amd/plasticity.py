@@ -78,7 +78,7 @@ class PlasticPath:
 def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int = 60,
               precision: int = MIXED, tol: float = 1e-11, batch=None, maxit: int = 50000,
               counts: dict | None = None, max_halvings: int = 6, last: int | None = None,
-              progress=None):
+              progress=None, first: int = 1):
     """Drives every path through `increments` equal load steps.  Per Newton
     iteration each unconverged path re-assembles K_t and R = -f_int on the
     GPU; the systems K_t du = R are solved together in `batch` (a
@@ -92,11 +92,12 @@ def run_paths(paths, increments: int = 10, rtol: float = 1e-8, max_newton: int =
     next under a large load step; the halving restores descent.
     Returns per path the list of (increment, assembly, ||R||); `counts`
     (optional dict) receives the numbers of linear solves and of halvings.
-    `last` stops after that increment (the load still split in `increments`).
+    `last` stops after that increment (the load still split in `increments`);
+    `first` resumes a run whose increments before it are committed.
     `progress` (a file, optional) receives one line per finished increment."""
     hist = [[] for _ in paths]
     nsolve = nhalf = 0
-    for inc in range(1, (last or increments) + 1):
+    for inc in range(first, (last or increments) + 1):
         lam = inc / increments
         for p in paths:
             p.increment(lam)
@@ -161,11 +162,12 @@ def compare_solves(paths, tol: float = 1e-10, maxit: int = 50000, fp32_sums: boo
     mixed precision (FP64 sums in the FP32 operator, and with `fp32_sums`
     also FP32 sums).  Returns wall times, solved DOF/s, iteration totals and
     the largest relative L2 difference of du against the sequential FP64
-    solutions."""
+    solutions.  Each context's FP32-sum option is put back afterwards."""
     import time
 
     from .serving import Batch
 
+    saved = [p.ctx.get_option("accum32") for p in paths]
     dof = float(sum(p.sys.neq for p in paths))
     a = time.perf_counter()  # the assemblies before this returned on completion
     it64 = 0
@@ -199,8 +201,8 @@ def compare_solves(paths, tol: float = 1e-10, maxit: int = 50000, fp32_sums: boo
                   for i, p in enumerate(paths))
         out[name] = {"s": t_bat, "dof_per_s": dof / t_bat, "iters": itb,
                      "speedup_vs_sequential_fp64": t_seq / t_bat, "max_rel_err_vs_fp64": err}
-    for p in paths:
-        p.ctx.set_option("accum32", 0)
+    for p, a32 in zip(paths, saved):
+        p.ctx.set_option("accum32", a32)
     return out
 
 
