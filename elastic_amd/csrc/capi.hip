@@ -243,7 +243,7 @@ static int build_layout(elastic_ctx* x) {
     c->inc_idx = b.inc_idx;  // node-granular lists
     nepart = (int64_t)nb * L.next_nodes;
     rc = rc ? rc : dalloc(c, &c->tdot, (size_t)L.ntiles);
-    rc = rc ? rc : dalloc(c, &c->p32, 3 * vec32_stride(neq));
+    rc = rc ? rc : dalloc(c, &c->p32, 4 * vec32_stride(neq));
   }
   if (nb == 1) {  // FP32 operator of the scalar tiles (the blocked layout has its planes)
     rc = rc ? rc : dalloc(c, &c->A32.ad, neq);

@@ -204,7 +204,8 @@ struct Context {
   double *epart64 = nullptr, *red64 = nullptr;
   double* tdot = nullptr;  // per-tile shares of p.(A p) (node-blocked layouts)
   // FP32 vectors of the mixed PCG with the FP32-sum operator (node-blocked):
-  // p at p32, r at p32 + vec32_stride(neq), the Jacobi scale after that
+  // p at p32, r at p32 + vec32_stride(neq), the Jacobi scale after that,
+  // then the FP32 increments of x since the last true-residual replacement
   float* p32 = nullptr;
   double *v64[8] = {nullptr};
   SolveScalars* scal = nullptr;
@@ -328,10 +329,12 @@ struct DotOut {
 };
 void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epart,
                  const int32_t* skip = nullptr, const DotOut* dot = nullptr, bool y32 = false,
-                 const float* x32 = nullptr);
+                 const float* x32 = nullptr, const float* dx = nullptr);
 // (y32: a product with FP32 sums on the node-blocked layout stores its own
 // rows of y as FP32 too, exactly; the caller then reads y as float.  x32: an
-// FP32 copy of x that such a product stages from instead of x)
+// FP32 copy of x that such a product stages from instead of x.  dx: FP32
+// increments that the FP64 product of the node-blocked layout adds to x when
+// it stages it, i.e. the product of x + dx)
 // gathers the CSRC values into the node-blocked planes (after every assembly)
 void launch_block_pack(Context* c);
 template <typename T>

@@ -38,12 +38,13 @@ __global__ __launch_bounds__(kThreads) void k_pcg_init(int n, const double* __re
                                                         const double* __restrict__ bm, double* x,
                                                         double* r, double* p, double* part,
                                                         double* part_pr, float* p32,
-                                                        float* r32) {
+                                                        float* r32, float* dx32) {
   __shared__ double sh[8];
   double acc = 0.0, apr = 0.0;
   GRID_LOOP(i, n) {
     const double zi = apply_m(i, n, b, m, bm);
     x[i] = 0.0;
+    if (dx32) dx32[i] = 0.0f;
     r[i] = b[i];
     p[i] = zi;
     if (p32) p32[i] = (float)zi;
@@ -156,12 +157,14 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int n, const double* __
 // asked for FP32 rows; q is then not written back).  VT: type of p, r and the
 // Jacobi scale m (FP32 when the mixed iteration keeps its vectors in FP32: p
 // is then the vector the product staged, so x += alpha p moves along exactly
-// that vector; x stays FP64).
-template <int NB, typename EP, typename QT, typename VT>
+// that vector).  XT: type of the stored x; float when the iteration adds its
+// steps to FP32 increments dx (x = x64 + dx), which every true-residual
+// replacement folds into the FP64 x (k_rel_resid) and clears.
+template <int NB, typename EP, typename QT, typename VT, typename XT = double>
 __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs mg,
                                                                 const double* __restrict__ part_pr,
                                                                 const double* __restrict__ dslot,
-                                                                double* x,
+                                                                XT* x,
                                                                 const VT* __restrict__ p,
                                                                 VT* r, double* q, int keep_q,
                                                                 const VT* __restrict__ m,
@@ -183,7 +186,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
       qv[c] = (double)reinterpret_cast<const QT*>(q)[i0 + c];
-      xv[c] = x[i0 + c];
+      xv[c] = (double)x[i0 + c];
       pv[c] = (double)p[i0 + c];
       rv[c] = (double)r[i0 + c];
       mv[c] = (double)m[i0 + c];
@@ -218,7 +221,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
       if (sizeof(QT) == 8 && keep_q) q[i0 + c] = qo[c];
-      x[i0 + c] = xo[c];
+      x[i0 + c] = (XT)xo[c];
       r[i0 + c] = (VT)ro[c];
     }
   }
@@ -388,7 +391,9 @@ __global__ void k_rel_force(SolveScalars* sc) {
 // r = b - (t + merged partials); with Jacobi also partial r.(m r); with
 // part_pr also the partial p.r of the replaced residual.  With r32 the
 // iteration's vectors are FP32: the residual is stored there, rounded, and p
-// is read from p32; the norm is formed from the FP64 residual.
+// is read from p32; the norm is formed from the FP64 residual.  With dx32 the
+// product t was formed on x + dx32 (the FP32 increments of x since the last
+// replacement): they are added to x here and cleared.
 __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
                                                          const double* __restrict__ t,
                                                          const double* __restrict__ b,
@@ -397,11 +402,16 @@ __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
                                                          const double* __restrict__ p,
                                                          const float* __restrict__ p32,
                                                          float* r32, double* part_pr,
+                                                         double* x, float* dx32,
                                                          const SolveScalars* sc) {
   __shared__ double sh[8];
   if (!block_flag(&sc->upd, sh) || block_flag(&sc->done, sh)) return;
   double acc = 0.0, apr = 0.0;
   GRID_LOOP(i, n) {
+    if (dx32) {
+      x[i] = x[i] + (double)dx32[i];
+      dx32[i] = 0.0f;
+    }
     double s = t[i];
     s = merge_row(mg, i, s);
     const double ri = b[i] - s;
@@ -437,6 +447,7 @@ __global__ void k_rel_finish(const double* __restrict__ part, double* dslot, Sol
     progress_guard(sc, sqrt(fabs(rz)));
     if (sqrt(fabs(rz)) < sc->conv) {
       sc->done = 1;
+      sc->skip64 = 1;  // gated products later in the chunk return at once
     } else {
       *dslot = rz;
       sc->rmax = sqrt(fabs(rz));
@@ -500,17 +511,25 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   bool v32 = p32 != nullptr && jac && reliable && epart_f32(c, true);
   float* r32 = v32 ? p32 + vec32_stride(n) : nullptr;
   float* m32 = v32 ? p32 + 2 * vec32_stride(n) : nullptr;
+  // On large systems the FP32 iteration also keeps its steps of x as FP32
+  // increments dx32 (x = V.x + dx32): the update then moves 4 bytes of x per
+  // row instead of 8 each way.  The increments since the last replacement
+  // are a small fraction of x (the residual fell by at most delta since),
+  // and every replacement forms its product on V.x + dx32 and folds dx32
+  // into V.x, so x, the true residuals and the stop test stay FP64.
+  float* dx32 = (v32 && n >= kPredictRows) ? p32 + 3 * vec32_stride(n) : nullptr;
   if (v32) launch_cast<double, float>(n, V.m, m32, s);
   hipLaunchKernelGGL(k_pcg_init, G, TB, 0, s, n, V.b, V.m, V.bm, V.x, V.r, V.p, part_a, pr_next,
-                     p32, r32);
+                     p32, r32, dx32);
   hipLaunchKernelGGL(k_pcg_init2, dim3(1), TB, 0, s, part_a, tol, delta, reliable ? 1 : 0, dslots,
                      sc);
   auto replace = [&](int iters_so_far) {
     double* dnext = dslots + (iters_so_far & 1);
-    launch_spmv(c, false, V.x, V.t, V.epart, &sc->skip64);
+    launch_spmv(c, false, V.x, V.t, V.epart, &sc->skip64, nullptr, false, nullptr, dx32);
     hipLaunchKernelGGL(k_rel_resid, G, TB, 0, s, n, merge_of(c, V.epart),
                        V.t, V.b, V.m, V.r, jac, part_b, (const double*)V.p,
-                       (const float*)(v32 ? p32 : nullptr), v32 ? r32 : nullptr, pr_next, sc);
+                       (const float*)(v32 ? p32 : nullptr), v32 ? r32 : nullptr, pr_next, V.x,
+                       dx32, sc);
     if (!jac)
       hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)nullptr,
                          (double*)nullptr, &sc->upd, sc);
@@ -535,14 +554,18 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   // large systems a replacement that the iteration asked for (the recursive
   // residual fell by delta, or kForceReplace iterations passed) runs right
   // after the poll that sees the flag, and the gated pass is enqueued every
-  // second iteration only in chunks where, at the decay rate of the
-  // recursive residual seen between polls, the stopping threshold can be
+  // second iteration only in chunks where the stopping threshold can be
   // reached within two chunks, so that convergence is confirmed on the true
-  // residual without delay.  Small systems enqueue it every second iteration.
+  // residual without delay.  The decay rate of the recursive residual is
+  // taken over the last kRateWindow polls: CG's residual is not monotone, so
+  // a rate over one chunk often shows growth and would keep the gate open.
+  // Small systems enqueue the gated pass every second iteration.
   const bool predict = reliable && n >= kPredictRows;
   bool gate = true;
-  double rn_prev = -1.0;
-  int it_prev = 0;
+  constexpr int kRateWindow = 4;
+  double rn_hist[kRateWindow + 1];
+  int it_hist[kRateWindow + 1];
+  int nhist = 0;
   while (it < maxit) {
     const int chunk = std::min(every, maxit - it);
     const int it0 = it;
@@ -564,18 +587,25 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   hipLaunchKernelGGL((k_pcg_update_nodes<NB, EP, QT, VT>), G, TB, 0, s, n / NB, mg,            \
                      (const double*)part_pr, dcur, V.x, (const VT*)(P), (VT*)(R), V.q, keep_q,  \
                      (const VT*)(M), jac, part_b, part_zq, sc)
+#define ELASTIC_UPD_DX(NB)                                                                     \
+  hipLaunchKernelGGL((k_pcg_update_nodes<NB, float, float, float, float>), G, TB, 0, s, n / NB, \
+                     mg, (const double*)part_pr, dcur, dx32, (const float*)p32, r32, V.q,      \
+                     keep_q, (const float*)m32, jac, part_b, part_zq, sc)
         if (c->layout_nb == 3) {
-          if (q32 && v32) ELASTIC_UPD(3, float, float, float, p32, r32, m32);
+          if (q32 && v32 && dx32) ELASTIC_UPD_DX(3);
+          else if (q32 && v32) ELASTIC_UPD(3, float, float, float, p32, r32, m32);
           else if (q32) ELASTIC_UPD(3, float, float, double, V.p, V.r, V.m);
           else if (e32) ELASTIC_UPD(3, float, double, double, V.p, V.r, V.m);
           else ELASTIC_UPD(3, double, double, double, V.p, V.r, V.m);
         } else {
-          if (q32 && v32) ELASTIC_UPD(2, float, float, float, p32, r32, m32);
+          if (q32 && v32 && dx32) ELASTIC_UPD_DX(2);
+          else if (q32 && v32) ELASTIC_UPD(2, float, float, float, p32, r32, m32);
           else if (q32) ELASTIC_UPD(2, float, float, double, V.p, V.r, V.m);
           else if (e32) ELASTIC_UPD(2, float, double, double, V.p, V.r, V.m);
           else ELASTIC_UPD(2, double, double, double, V.p, V.r, V.m);
         }
 #undef ELASTIC_UPD
+#undef ELASTIC_UPD_DX
       } else {
         launch_spmv(c, op32, V.p, V.q, V.epart, &sc->done);
         if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j + 1], s));
@@ -618,9 +648,11 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
         v32 = false;
         r32 = m32 = nullptr;
       }
-      // restart: r = b - A64 x, p = M^-1 r, FP64 PCG from here on
+      // restart: r = b - A64 x, p = M^-1 r, FP64 PCG from here on (the
+      // replacement folds the FP32 increments of x, which end there)
       hipLaunchKernelGGL(k_rel_force, dim3(1), dim3(64), 0, s, sc);
       replace(it);
+      dx32 = nullptr;
       hipLaunchKernelGGL(k_pcg_restart, G, TB, 0, s, n, (const double*)V.r, V.m, V.bm, V.p,
                          pr_next, sc);
       replacing = false;
@@ -629,13 +661,24 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
       const SolveScalars& h = *c->scal_host;
       if (h.upd) replace(it);  // asked for during the chunk: runs before the next one
       const double rn = std::sqrt(std::fabs(h.di));
-      gate = true;
-      if (rn > h.conv && rn_prev > 0.0 && h.iter > it_prev) {
-        const double rate = std::pow(rn / rn_prev, 1.0 / (double)(h.iter - it_prev));
-        if (rate < 1.0 && std::log(h.conv / rn) / std::log(rate) > 2.0 * every) gate = false;
+      if (nhist == kRateWindow + 1) {
+        for (int k = 0; k < kRateWindow; ++k) {
+          rn_hist[k] = rn_hist[k + 1];
+          it_hist[k] = it_hist[k + 1];
+        }
+        --nhist;
       }
-      rn_prev = rn;
-      it_prev = h.iter;
+      rn_hist[nhist] = rn;
+      it_hist[nhist] = h.iter;
+      ++nhist;
+      gate = true;
+      if (rn > h.conv && nhist >= 2 && h.iter > it_hist[0] && rn_hist[0] > 0.0) {
+        const double rate = std::pow(rn / rn_hist[0], 1.0 / (double)(h.iter - it_hist[0]));
+        if (rate < 1.0)
+          gate = std::log(h.conv / rn) / std::log(rate) <= 2.0 * every;
+        else  // no decay over the window: gate only close to the threshold
+          gate = rn < 10.0 * h.conv;
+      }
     }
   }
   if (!done && replacing) {  // maxit: leave the true residual of the final x

@@ -138,14 +138,15 @@ __device__ __forceinline__ void tile_dot_store(double* part, int t, double v) {
 // are added in tile order by a kernel of their own (k_sum_shares), so x.(A x)
 // needs no further pass over y and no floating-point atomics.
 template <typename TV, typename T, int NB, int PER, bool DOT = false, typename Hook = NoHook,
-          int NT = 256>
+          int NT = 256, bool XD = false>
 __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
                                              const TileRegs<TV, NB, PER>& R,
                                              const double* __restrict__ x, double* __restrict__ y,
                                              double* __restrict__ epart, unsigned char* smem,
                                              double* __restrict__ tdot = nullptr,
                                              int y32 = 0, const float* __restrict__ x32 = nullptr,
-                                             Hook hook = Hook()) {
+                                             Hook hook = Hook(),
+                                             const float* __restrict__ dx = nullptr) {
   constexpr int NB2 = NB * NB;
   T* xs = reinterpret_cast<T*>(smem);       // ncols_max * NB
   T* pp = xs + a.ncols_max * NB;            // 2 * kBlkRowsMax * NB (row pieces)
@@ -158,7 +159,10 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   const auto& vr = R.vr;
   // x of the tile's local columns: the first 8 * NT entries are loaded into
   // registers, then `hook` runs (a caller may issue more loads there, which
-  // stay in flight behind these), then they go to LDS
+  // stay in flight behind these), then they go to LDS.  XD (FP64 staging
+  // only): x is held as x + dx, an FP64 base plus FP32 increments, and the
+  // product is formed on their FP64 sum (a separate instance, so the other
+  // products keep their registers)
   constexpr int XR = 8;
   const int nx = ncols * NB;
   double xv[XR];
@@ -170,6 +174,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
       const int g = node < nr ? I0 + node : ext[node - nr];
       // x32: an FP32 copy of x (FP32 staging takes the same values from it)
       xv[k] = x32 ? (double)x32[(int64_t)NB * g + comp] : x[(int64_t)NB * g + comp];
+      if constexpr (XD) xv[k] += (double)dx[(int64_t)NB * g + comp];
     }
   }
   hook();
@@ -182,7 +187,9 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   for (int c = tid + NT * XR; c < nx; c += NT) {
     const int node = c / NB, comp = c - node * NB;
     const int g = node < nr ? I0 + node : ext[node - nr];
-    xs[c] = x32 ? (T)x32[(int64_t)NB * g + comp] : (T)x[(int64_t)NB * g + comp];
+    double v = x32 ? (double)x32[(int64_t)NB * g + comp] : x[(int64_t)NB * g + comp];
+    if constexpr (XD) v += (double)dx[(int64_t)NB * g + comp];
+    xs[c] = (T)v;
   }
   __syncthreads();
   const uint16_t* rp = a.rowptr + a.rp0[t];
@@ -296,16 +303,19 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
 }
 
 // One tile: its loads, then its product.
-template <typename TV, typename T, int NB, int PER, bool DOT = false, int NT = 256>
+template <typename TV, typename T, int NB, int PER, bool DOT = false, int NT = 256,
+          bool XD = false>
 __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
                                            const TV* __restrict__ diag,
                                            const double* __restrict__ x, double* __restrict__ y,
                                            double* __restrict__ epart, unsigned char* smem,
                                            double* __restrict__ tdot = nullptr, int y32 = 0,
-                                           const float* __restrict__ x32 = nullptr) {
+                                           const float* __restrict__ x32 = nullptr,
+                                           const float* __restrict__ dx = nullptr) {
   TileRegs<TV, NB, PER> R;
   load_tile<TV, NB, PER, NT>(a, t, vals, diag, R);
-  compute_tile<TV, T, NB, PER, DOT, NoHook, NT>(a, t, R, x, y, epart, smem, tdot, y32, x32);
+  compute_tile<TV, T, NB, PER, DOT, NoHook, NT, XD>(a, t, R, x, y, epart, smem, tdot, y32,
+                                                    x32, NoHook(), dx);
 }
 
 }  // namespace elastic

@@ -5,8 +5,9 @@
 
 namespace elastic {
 
-// DOT: also the tiles' shares of x.(A x) into tdot; NT lanes per tile
-template <typename TV, typename T, int NB, int PER, bool DOT, int NT>
+// DOT: also the tiles' shares of x.(A x) into tdot; NT lanes per tile; XD:
+// the product of x + dx (FP32 increments dx)
+template <typename TV, typename T, int NB, int PER, bool DOT, int NT, bool XD = false>
 __global__ __launch_bounds__(NT) void k_spmv_blocks(BlkArgs a, const TV* __restrict__ vals,
                                                       const TV* __restrict__ diag,
                                                       const double* __restrict__ x,
@@ -14,11 +15,12 @@ __global__ __launch_bounds__(NT) void k_spmv_blocks(BlkArgs a, const TV* __restr
                                                       double* __restrict__ epart,
                                                       const int32_t* __restrict__ skip,
                                                       double* __restrict__ tdot, int y32,
-                                                      const float* __restrict__ x32) {
+                                                      const float* __restrict__ x32,
+                                                      const float* __restrict__ dx) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (skip != nullptr && *skip != 0) return;
-  block_tile<TV, T, NB, PER, DOT, NT>(a, blockIdx.x, vals, diag, x, y, epart, smem, tdot, y32,
-                                      x32);
+  block_tile<TV, T, NB, PER, DOT, NT, XD>(a, blockIdx.x, vals, diag, x, y, epart, smem, tdot,
+                                          y32, x32, dx);
 }
 
 // *out = the n shares added in a fixed order (one workgroup of 1024 lanes):
@@ -101,7 +103,7 @@ template <typename TV, typename TA, int NB>
 static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* diag,
                                const double* x, double* y, double* epart, hipStream_t s,
                                const int32_t* skip, const DotOut* dot, int y32,
-                               const float* x32, int lanes) {
+                               const float* x32, int lanes, const float* dx = nullptr) {
   const size_t lds = sizeof(TA) * NB * ((size_t)b.ncols_max + 2 * kBlkRowsMax + (size_t)b.nblk_max);
   const int nt = lanes == 512 && b.nblk_max > 256 ? 512 : 256;
   const int per = (b.nblk_max + nt - 1) / nt;
@@ -118,11 +120,25 @@ static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* dia
   double* td = dot ? dot->part : nullptr;
 #define ELASTIC_BLK_PER(D)                                                                               \
   do {                                                                                                   \
-    if (nt == 512) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, D, 512, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
-    else if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 1, D, 256, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
-    else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, D, 256, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
-    else ELASTIC_BLK_LAUNCH(k_spmv_blocks, 4, D, 256, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32); \
+    if (nt == 512) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, D, 512, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32, dx); \
+    else if (per <= 1) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 1, D, 256, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32, dx); \
+    else if (per <= 2) ELASTIC_BLK_LAUNCH(k_spmv_blocks, 2, D, 256, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32, dx); \
+    else ELASTIC_BLK_LAUNCH(k_spmv_blocks, 4, D, 256, b.ntiles, blk_args(b), vals, diag, x, y, epart, skip, td, y32, x32, dx); \
   } while (0)
+  if constexpr (sizeof(TV) == 8 && sizeof(TA) == 8) {
+    if (dx != nullptr && dot == nullptr) {  // the product of x + dx (residual replacement)
+      const int per256 = (b.nblk_max + 255) / 256;  // 256 lanes whatever the lanes option
+      const auto kx = per256 <= 1 ? k_spmv_blocks<TV, TA, NB, 1, false, 256, true>
+                    : per256 <= 2 ? k_spmv_blocks<TV, TA, NB, 2, false, 256, true>
+                                  : k_spmv_blocks<TV, TA, NB, 4, false, 256, true>;
+      if (lds > 65536)
+        (void)hipFuncSetAttribute((const void*)kx, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+      hipLaunchKernelGGL(kx, dim3(b.ntiles), dim3(256), lds, s, blk_args(b), vals, diag, x, y,
+                         epart, skip, (double*)nullptr, 0, (const float*)nullptr, dx);
+      return;
+    }
+  }
   if (dot) {
     ELASTIC_BLK_PER(true);
     if (dot->result)  // (null: the shares only)
@@ -136,18 +152,19 @@ static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* dia
 }
 
 void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epart,
-                 const int32_t* skip, const DotOut* dot, bool y32, const float* x32) {
+                 const int32_t* skip, const DotOut* dot, bool y32, const float* x32,
+                 const float* dx) {
   hipStream_t s = c->stream;
   const DevBlocks& b = c->blk;
   const int l32 = c->lanes[1], l64 = c->lanes[0];
   if (c->layout_nb == 3) {
     if (fp32 && c->accum32) spmv_blocks_launch<float, float, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0, x32, l32);
     else if (fp32) spmv_blocks_launch<float, double, 3>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0, nullptr, l32);
-    else spmv_blocks_launch<double, double, 3>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr, l64);
+    else spmv_blocks_launch<double, double, 3>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr, l64, dx);
   } else if (c->layout_nb == 2) {
     if (fp32 && c->accum32) spmv_blocks_launch<float, float, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, y32 ? 1 : 0, x32, l32);
     else if (fp32) spmv_blocks_launch<float, double, 2>(b, b.v32, b.d32, x, y, epart, s, skip, dot, 0, nullptr, l32);
-    else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr, l64);
+    else spmv_blocks_launch<double, double, 2>(b, b.v64, b.d64, x, y, epart, s, skip, dot, 0, nullptr, l64, dx);
   } else if (c->gather) {
     const DevGather& g = c->gth;
     if (fp32)
