@@ -794,6 +794,13 @@ int elastic_batch_destroy(elastic_batch* b) {
   if (!b) return ELASTIC_OK;
   hipSetDevice(b->device);
   if (b->stream) hipStreamSynchronize(b->stream);
+  // contexts of instances never retired go back to the CSRC order (with the
+  // x their iteration reached)
+  for (auto& S : b->slots)
+    if (S.used && S.ctx) {
+      (void)leave_layout_order(&S.ctx->c);
+      hipStreamSynchronize(S.ctx->c.stream);
+    }
   if (b->gexec) hipGraphExecDestroy(b->gexec);
   if (b->graph) hipGraphDestroy(b->graph);
   for (void* p : {(void*)b->d_insts, (void*)b->d_vwork, (void*)b->d_slist, (void*)b->d_counts,
@@ -836,6 +843,18 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
     }
   if (k < 0) return berr(b, ELASTIC_BUSY, "no free slot");
   hipSetDevice(b->device);
+  // node-block Jacobi when the context asks for it (FP64 vectors then)
+  const double* bm = nullptr;
+  if (c->batch_precond == ELASTIC_PRECOND_BDIAG3) {
+    int rc = ensure_bdiag(x, c->layout_nb);
+    if (rc) return berr(b, rc, c->err);
+    bm = solver_bm(c);
+  }
+  // a compact layout iterates in its node order until the instance retires
+  {
+    int rc = enter_layout_order(c);
+    if (rc) return berr(b, rc, c->err);
+  }
   // the context's own work (uploads, assembly) must be complete
   BATCH_HIP(b, hipStreamSynchronize(c->stream));
   BInst I;
@@ -854,13 +873,6 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   I.q = c->v64[VQ];
   I.b = c->v64[VB];
   I.t = c->v64[VT];
-  // node-block Jacobi when the context asks for it (FP64 vectors then)
-  const double* bm = nullptr;
-  if (c->batch_precond == ELASTIC_PRECOND_BDIAG3) {
-    int rc = ensure_bdiag(x, c->layout_nb);
-    if (rc) return berr(b, rc, c->err);
-    bm = c->A64.bm;
-  }
   // FP32 vectors only for instances whose products sum in FP32
   const bool v32 = precision == ELASTIC_MIXED && c->accum32 && !bm;
   I.p32 = v32 ? c->p32 : nullptr;
@@ -966,6 +978,12 @@ int elastic_batch_poll(elastic_batch* b, int64_t* tags, elastic_stats* stats, in
       o.conv = c->scal_host->conv;
       int rc = post_solve_stats(c, &o);
       if (rc) return berr(b, rc, c->err);
+    }
+    {  // x back into the CSRC order (compact layouts)
+      Context* c = &S.ctx->c;
+      int rc = leave_layout_order(c);
+      if (rc) return berr(b, rc, c->err);
+      BATCH_HIP(b, hipStreamSynchronize(c->stream));
     }
     // the slot holds no instance any more: kernels skip it even if a list
     // still named it (the lists are rebuilt before the next step)

@@ -68,17 +68,28 @@ constexpr int kBlkTileMax = 1024;
 struct BlockLayout {
   int nb = 1, nn = 0, ntiles = 0, nblk_max = 0, ncols_max = 0;
   int64_t nblocks = 0, next_nodes = 0;
+  bool compact = false;  // nodes in the compact tile order (d2r), else the CSRC order
   std::vector<int32_t> row0, nrows, nblk, ncols, ext0, seg0, rp0;
   std::vector<int64_t> blk0;
   std::vector<uint32_t> word;
   std::vector<uint16_t> rowptr, segptr;
   std::vector<int32_t> ext, inc_ptr, inc_idx;
+  // layout node d is CSRC node d2r[d] (compact order only)
+  std::vector<int32_t> d2r;
+  // source of block q in the CSRC values: the lower blocks of CSRC node row
+  // src_row[q], entry src_k[q] & 0x7FFFFFFF of that row; bit 31 of src_k: the
+  // layout block is the transpose of that CSRC block
+  std::vector<int32_t> src_row, src_k;
 };
 
 // 3 (or 2) when the CSRC pattern is a pattern of node blocks, else 1
 int detect_node_blocks(int neq, const int32_t* ia1, const int32_t* ja1);
+// compact: order the nodes in compact tiles grown breadth-first over the node
+// graph (a tile's lower neighbours outside it are then few) instead of the
+// CSRC order; falls back to the CSRC order when a node would exceed 64 lower
+// blocks in the compact order
 void build_block_tiles(int neq, int nb, const int32_t* ia1, const int32_t* ja1, int tile_blocks,
-                       BlockLayout& L);
+                       BlockLayout& L, bool compact = false);
 
 struct DevBlocks {
   int nb = 1, nn = 0, ntiles = 0, nblk_max = 0, ncols_max = 0;
@@ -90,6 +101,9 @@ struct DevBlocks {
   uint16_t *rowptr = nullptr, *segptr = nullptr;
   int32_t *ext = nullptr, *inc_ptr = nullptr, *inc_idx = nullptr;
   int32_t* ia = nullptr;  // device copy of ia (value pack)
+  // compact order: layout node d is CSRC node d2r[d] (null: the same order);
+  // per block the CSRC source of its values (BlockLayout::src_row / src_k)
+  int32_t *d2r = nullptr, *src_row = nullptr, *src_k = nullptr;
   // values: plane p of block q of tile t at blk0[t]*nb*nb + p*nblk[t] + q;
   // node diagonal blocks: entry c of node I at c*nn + I (nb diagonal entries,
   // then the in-node lower entries a21, a31, a32)
@@ -229,6 +243,18 @@ struct Context {
   double tprod_ms = 0.0;
   int64_t tprod_n = 0;
   int zero_diag_row = -1;  // first row with ad == 0 (1-based), -1 if none
+  // Node order of the node-blocked layout: -1 follows the system (compact
+  // tiles for 3-dof layouts of at least kLargeTileNodes nodes), 0 the CSRC
+  // order, 1 compact tiles (3-dof layouts).  With
+  // compact tiles every vector inside a solve is in the layout's node order:
+  // b and x cross between orders when a solve starts and ends (enter/leave
+  // of the layout order), so v64[VB] and v64[VX] are in the CSRC order
+  // outside a solve, and the Jacobi scale A64.m and bm_dev are in the layout
+  // order always.
+  int node_order = -1;
+  double *xd = nullptr, *bd = nullptr;  // x and b in the layout order (compact only)
+  double* bm_dev = nullptr;             // 3x3 block inverse in the layout order
+  bool in_layout_order = false;         // v64[VX] / v64[VB] hold xd / bd (inside a solve)
 };
 
 // How the external partials of a product reach their rows.
@@ -337,6 +363,18 @@ void launch_spmv(Context* c, bool fp32, const double* x, double* y, double* epar
 // it stages it, i.e. the product of x + dx)
 // gathers the CSRC values into the node-blocked planes (after every assembly)
 void launch_block_pack(Context* c);
+// the Jacobi scale 1/ad in the compact layout order (into A64.m)
+void launch_recip_layout(Context* c);
+// vectors between the CSRC order and the compact layout order:
+//   to_layout: dst[nb*d + r] = src[nb*d2r[d] + r];  else the inverse
+void launch_permute(int nn, int nb, const int32_t* d2r, const double* src, double* dst,
+                    bool to_layout, hipStream_t s);
+// v64[VB] -> layout order and v64[VX] / v64[VB] swapped with xd / bd (enter),
+// and back with x -> CSRC order (leave); no-ops for layouts in the CSRC order
+int enter_layout_order(Context* c);
+int leave_layout_order(Context* c, bool keep_x = true);
+// the block inverse that solves apply: bm_dev (compact order) or A64.bm
+inline const double* solver_bm(const Context* c) { return c->bm_dev ? c->bm_dev : c->A64.bm; }
 template <typename T>
 void launch_recip(int n, const T* ad, T* m, hipStream_t s);
 void launch_post(const MergeArgs& mg, int n, const double* x, double* y, const double* b,

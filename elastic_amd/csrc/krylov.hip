@@ -95,6 +95,14 @@ int ensure_bdiag(elastic_ctx* x, int bsz) {
   else
     hipLaunchKernelGGL(k_bdiag2, dim3(g), dim3(256), 0, c->stream, nb, c->A64.ad, c->A64.al,
                        dslots, c->A64.bm);
+  // compact layouts apply the 3x3 inverse in their node order (node blocks
+  // of 6 values move with their nodes); A64.bm keeps the CSRC order
+  if (c->bm_dev) hipFree(c->bm_dev);
+  c->bm_dev = nullptr;
+  if (bsz == 3 && c->blk.d2r && c->blk.nb == 3) {
+    ELASTIC_HIP_CHECK(c, hipMalloc((void**)&c->bm_dev, sizeof(double) * 6 * (size_t)c->blk.nn));
+    launch_permute(c->blk.nn, 6, c->blk.d2r, c->A64.bm, c->bm_dev, true, c->stream);
+  }
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   hipFree(dslots);
   c->bdiag_set = true;
@@ -120,13 +128,15 @@ extern "C" int elastic_get_block_precond(elastic_ctx* x, int32_t bsz, double* m)
   std::vector<double> inv(rest > 0 ? rest : 1);
   ELASTIC_HIP_CHECK(c, hipMemcpyAsync(m, c->A64.bm, sizeof(double) * w * (size_t)nb,
                                       hipMemcpyDeviceToHost, c->stream));
+  // 1/ad of the remaining equations from the CSRC-order diagonal (A64.m may
+  // be in a compact layout's order); IEEE division, as the device's
   if (rest)
-    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(inv.data(), c->A64.m + (size_t)bsz * nb,
+    ELASTIC_HIP_CHECK(c, hipMemcpyAsync(inv.data(), c->A64.ad + (size_t)bsz * nb,
                                         sizeof(double) * rest, hipMemcpyDeviceToHost, c->stream));
   ELASTIC_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < rest; ++i) {
     double* o = m + (size_t)w * ((size_t)nb + i);
-    o[0] = inv[i];
+    o[0] = 1.0 / inv[i];
     for (int k = 1; k < w; ++k) o[k] = 0.0;
   }
   return ELASTIC_OK;
@@ -398,7 +408,7 @@ int bicgstab_run(Context* c, bool bdiag, bool fp32_operator, double tol, int max
   double *x = c->v64[VX], *r = c->v64[VR], *p = c->v64[VP], *v = c->v64[VQ], *b = c->v64[VB],
          *t = c->v64[VT], *z = c->v64[VU], *r0 = c->v64[VW];
   const double* m = c->A64.m;
-  const double* bm = bdiag ? c->A64.bm : nullptr;
+  const double* bm = bdiag ? solver_bm(c) : nullptr;
   const int jac = bm == nullptr;
   const bool reliable = fp32_operator;
   SolveScalars* sc = c->scal;
@@ -640,7 +650,7 @@ int gmres_run(Context* c, bool bdiag, bool fp32_operator, double tol, int maxit,
   double* P[2] = {c->red64, c->red64 + kRedBlocks};
   double *x = c->v64[VX], *b = c->v64[VB], *t = c->v64[VT], *u = c->v64[VU];
   const double* m = c->A64.m;
-  const double* bm = bdiag ? c->A64.bm : nullptr;
+  const double* bm = bdiag ? solver_bm(c) : nullptr;
   auto g = [&](int j) { return c->gm_basis + (size_t)(j - 1) * n; };  // 1-based
   SolveScalars* sc = c->scal;
   SolveScalars* hs = c->scal_host;

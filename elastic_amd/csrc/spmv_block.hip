@@ -53,49 +53,68 @@ __global__ __launch_bounds__(1024) void k_sum_shares(const double* __restrict__ 
 }
 
 // Gathers the CSRC al/ad into the block planes (one workgroup per tile) and
-// the node diagonal blocks (grid over nodes).
+// the node diagonal blocks (grid over nodes).  Block q takes the values of
+// entry k of CSRC node row R's lower blocks (src_row / src_k), transposed
+// when bit 31 of src_k is set (compact node order); row r of that CSRC block
+// starts at ia[NB R + r] - 1 + NB k.
 template <typename TV, int NB>
 __global__ __launch_bounds__(256) void k_pack_blocks(BlkArgs a, const int32_t* __restrict__ ia,
+                                                      const int32_t* __restrict__ src_row,
+                                                      const int32_t* __restrict__ src_k,
                                                       const double* __restrict__ al,
                                                       TV* __restrict__ vals) {
-  constexpr int NB2 = NB * NB;
   const int t = blockIdx.x;
-  const int I0 = a.row0[t], nbk = a.nblk[t];
+  const int nbk = a.nblk[t];
   const int64_t base = a.blk0[t];
-  TV* v = vals + base * NB2;
-  // block index inside the node row comes from the rowptr of the tile
-  const uint16_t* rp = a.rowptr + a.rp0[t];
+  TV* v = vals + base * (NB * NB);
   for (int q = threadIdx.x; q < nbk; q += 256) {
-    const uint32_t w = a.word[base + q];
-    const int lr = (int)(w >> 25);
-    const int I = I0 + lr;
-    const int k = q - rp[lr];
+    const int R = src_row[base + q];
+    const int kk = src_k[base + q];
+    const int k = kk & 0x7FFFFFFF;
+    const bool tr = kk < 0;
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
-      const int64_t row_start = (int64_t)ia[NB * I + r] - 1 + (int64_t)NB * k;
+      const int64_t row_start = (int64_t)ia[NB * R + r] - 1 + (int64_t)NB * k;
 #pragma unroll
-      for (int cc = 0; cc < NB; ++cc)
-        v[Planes<TV, NB>::index(nbk, q, r * NB + cc)] = (TV)al[row_start + cc];
+      for (int cc = 0; cc < NB; ++cc) {
+        // CSRC entry (r, cc) is layout entry (r, cc), or (cc, r) when transposed
+        const int p = tr ? cc * NB + r : r * NB + cc;
+        v[Planes<TV, NB>::index(nbk, q, p)] = (TV)al[row_start + cc];
+      }
     }
   }
 }
 
+// d2r: layout node -> CSRC node (null: the same order)
 template <typename TV, int NB>
 __global__ __launch_bounds__(256) void k_pack_diag(int nn, const int32_t* __restrict__ ia,
+                                                    const int32_t* __restrict__ d2r,
                                                     const double* __restrict__ ad,
                                                     const double* __restrict__ al,
                                                     TV* __restrict__ diag) {
-  for (int I = blockIdx.x * 256 + threadIdx.x; I < nn; I += gridDim.x * 256) {
+  for (int d = blockIdx.x * 256 + threadIdx.x; d < nn; d += gridDim.x * 256) {
+    const int I = d2r ? d2r[d] : d;
     const int nbr = (ia[NB * I + 1] - ia[NB * I]) / NB;  // lower blocks of the node row
 #pragma unroll
-    for (int r = 0; r < NB; ++r) diag[(int64_t)r * nn + I] = (TV)ad[NB * I + r];
+    for (int r = 0; r < NB; ++r) diag[(int64_t)r * nn + d] = (TV)ad[NB * I + r];
     const int64_t p1 = (int64_t)ia[NB * I + 1] - 1 + (int64_t)NB * nbr;  // row 1 own entries
-    diag[(int64_t)NB * nn + I] = (TV)al[p1];                               // a21
+    diag[(int64_t)NB * nn + d] = (TV)al[p1];                               // a21
     if constexpr (NB == 3) {
       const int64_t p2 = (int64_t)ia[NB * I + 2] - 1 + (int64_t)NB * nbr;
-      diag[(int64_t)4 * nn + I] = (TV)al[p2];      // a31
-      diag[(int64_t)5 * nn + I] = (TV)al[p2 + 1];  // a32
+      diag[(int64_t)4 * nn + d] = (TV)al[p2];      // a31
+      diag[(int64_t)5 * nn + d] = (TV)al[p2 + 1];  // a32
     }
+  }
+}
+
+// Jacobi scale in the layout order: m[NB d + r] = 1 / ad[NB d2r[d] + r]
+__global__ __launch_bounds__(256) void k_recip_perm(int nn, int nb, const int32_t* __restrict__ d2r,
+                                                     const double* __restrict__ ad,
+                                                     double* __restrict__ m) {
+  const int64_t n = (int64_t)nn * nb;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int d = (int)(i / nb), r = (int)(i - (int64_t)d * nb);
+    m[i] = 1.0 / ad[(int64_t)nb * d2r[d] + r];
   }
 }
 
@@ -186,25 +205,28 @@ void launch_block_pack(Context* c) {
   hipStream_t s = c->stream;
   const DevBlocks& b = c->blk;
   const int g = std::max(1, std::min((b.nn + 255) / 256, 8192));
-  if (b.nb == 3) {
-    hipLaunchKernelGGL((k_pack_blocks<double, 3>), dim3(b.ntiles), dim3(256), 0, s, blk_args(b),
-                       b.ia, c->A64.al, b.v64);
-    hipLaunchKernelGGL((k_pack_blocks<float, 3>), dim3(b.ntiles), dim3(256), 0, s, blk_args(b),
-                       b.ia, c->A64.al, b.v32);
-    hipLaunchKernelGGL((k_pack_diag<double, 3>), dim3(g), dim3(256), 0, s, b.nn, b.ia, c->A64.ad,
-                       c->A64.al, b.d64);
-    hipLaunchKernelGGL((k_pack_diag<float, 3>), dim3(g), dim3(256), 0, s, b.nn, b.ia, c->A64.ad,
-                       c->A64.al, b.d32);
-  } else {
-    hipLaunchKernelGGL((k_pack_blocks<double, 2>), dim3(b.ntiles), dim3(256), 0, s, blk_args(b),
-                       b.ia, c->A64.al, b.v64);
-    hipLaunchKernelGGL((k_pack_blocks<float, 2>), dim3(b.ntiles), dim3(256), 0, s, blk_args(b),
-                       b.ia, c->A64.al, b.v32);
-    hipLaunchKernelGGL((k_pack_diag<double, 2>), dim3(g), dim3(256), 0, s, b.nn, b.ia, c->A64.ad,
-                       c->A64.al, b.d64);
-    hipLaunchKernelGGL((k_pack_diag<float, 2>), dim3(g), dim3(256), 0, s, b.nn, b.ia, c->A64.ad,
-                       c->A64.al, b.d32);
-  }
+#define ELASTIC_PACK(NB)                                                                       \
+  do {                                                                                         \
+    hipLaunchKernelGGL((k_pack_blocks<double, NB>), dim3(b.ntiles), dim3(256), 0, s,           \
+                       blk_args(b), b.ia, b.src_row, b.src_k, c->A64.al, b.v64);               \
+    hipLaunchKernelGGL((k_pack_blocks<float, NB>), dim3(b.ntiles), dim3(256), 0, s,            \
+                       blk_args(b), b.ia, b.src_row, b.src_k, c->A64.al, b.v32);               \
+    hipLaunchKernelGGL((k_pack_diag<double, NB>), dim3(g), dim3(256), 0, s, b.nn, b.ia, b.d2r, \
+                       c->A64.ad, c->A64.al, b.d64);                                           \
+    hipLaunchKernelGGL((k_pack_diag<float, NB>), dim3(g), dim3(256), 0, s, b.nn, b.ia, b.d2r,  \
+                       c->A64.ad, c->A64.al, b.d32);                                           \
+  } while (0)
+  if (b.nb == 3) ELASTIC_PACK(3);
+  else ELASTIC_PACK(2);
+#undef ELASTIC_PACK
+}
+
+void launch_recip_layout(Context* c) {
+  const DevBlocks& b = c->blk;
+  const int64_t n = (int64_t)b.nn * b.nb;
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_recip_perm, dim3(g), dim3(256), 0, c->stream, b.nn, b.nb, b.d2r,
+                     c->A64.ad, c->A64.m);
 }
 
 }  // namespace elastic

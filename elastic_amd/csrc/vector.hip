@@ -55,6 +55,20 @@ __global__ __launch_bounds__(kT) void k_post(int n, MergeArgs mg,
   }
 }
 
+// vectors between the CSRC node order and the compact layout order (one lane
+// per layout node and component)
+__global__ __launch_bounds__(kT) void k_permute(int nn, int nb, const int32_t* __restrict__ d2r,
+                                                 const double* __restrict__ src,
+                                                 double* __restrict__ dst, int to_layout) {
+  const int64_t n = (int64_t)nn * nb;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const int d = (int)(i / nb), r = (int)(i - (int64_t)d * nb);
+    const int64_t j = (int64_t)nb * d2r[d] + r;
+    if (to_layout) dst[i] = src[j];
+    else dst[j] = src[i];
+  }
+}
+
 // out[j] = sum of part[j*kRedBlocks ...] for j < nsum (one block)
 __global__ __launch_bounds__(kT) void k_sum_parts(const double* __restrict__ part, int nsum,
                                                    double* out) {
@@ -93,6 +107,12 @@ void launch_post(const MergeArgs& mg, int n, const double* x, double* y, const d
                  const double* m, double* part, double* out6, hipStream_t s) {
   hipLaunchKernelGGL(k_post, dim3(kRedBlocks), dim3(kT), 0, s, n, mg, x, y, b, m, part);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, s, part, 6, out6);
+}
+
+void launch_permute(int nn, int nb, const int32_t* d2r, const double* src, double* dst,
+                    bool to_layout, hipStream_t s) {
+  hipLaunchKernelGGL(k_permute, dim3(grid_for((int64_t)nn * nb)), dim3(kT), 0, s, nn, nb, d2r, src,
+                     dst, to_layout ? 1 : 0);
 }
 
 void launch_sum_parts(const double* part, int nsum, double* out, hipStream_t s) {

@@ -271,7 +271,13 @@ int elastic_set_tile_blocks(elastic_ctx* ctx, int32_t tile_blocks);
  * 6: node-block Jacobi, the inverses of block_precond with the layout's node
  * size, 2x2 in 2-D and 3x3 in 3-D, applied as op_block_precond does),
  * "time_products" (S > 0: PCG brackets the product of every S-th iteration
- * with HIP events on its stream; 0, the default: off). */
+ * with HIP events on its stream; 0, the default: off),
+ * "node_order" (1: the node-blocked tiles of a 3-dof layout are compact
+ * breadth-first clusters of the node graph, in whose node order the solves
+ * iterate -- b and x are exchanged with the caller in the CSRC order as
+ * always; 0: tiles of consecutive CSRC node rows; until set, 1 for 3-dof
+ * systems of at least 32,768 nodes, else 0; rebuilds the layout at once when
+ * a structure is set). */
 int elastic_set_option(elastic_ctx* ctx, const char* name, int64_t value);
 /* Average device time of the PCG iteration products sampled under the
  * "time_products" option, and how many were sampled, since the last reset
@@ -294,7 +300,8 @@ int elastic_autotune(elastic_ctx* ctx, int32_t precision, int32_t reps, int32_t*
                      double* best_ms);
 /* Current value of a knob of elastic_set_option, or "lanes64" / "lanes32"
  * (lanes per tile of the FP64 / FP32-value product) or "layout_nb" (node
- * block size of the layout, 1 scalar rows, 0 the gather product). */
+ * block size of the layout, 1 scalar rows, 0 the gather product);
+ * "node_order" reads the order of the layout in use (1 compact). */
 int elastic_get_option(elastic_ctx* ctx, const char* name, int64_t* value);
 
 /* ------------------------------------------------------ batched serving */
