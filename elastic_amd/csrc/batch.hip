@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "device_util.h"
@@ -53,7 +54,7 @@ struct BInst {
   // all sums stay FP64); null otherwise.  Every true-residual replacement also
   // refreshes the FP64 r and p, so a hand-over to the FP64 operator (which
   // only happens there) continues from current FP64 vectors.
-  float *p32, *r32, *m32;
+  float *p32, *r32, *m32, *dx32;
   const double* m;
   // node-block Jacobi (null: Jacobi with m): per node the nb(nb+1)/2 values
   // of block_precond's inverse (2x2: a11 a22 a21, 3x3: c11 c22 c33 c21 c31 c32)
@@ -103,21 +104,17 @@ __device__ inline double inst_sum(const double* __restrict__ part, int nch, doub
 // keep per-item values live across iterations: 120 instead of 80 VGPRs for
 // the FP64 3-D variant), so each variant compiles with the registers of the
 // single-system kernel; the grid is a size class of the list (workgroups
-// beyond the list's count return at once).  mode 0: q = A p and the tiles' shares of p.q; the
-// FP64 list holds every instance
-// and serves FP64 instances and mixed ones the progress guard handed over to
-// FP64, the FP32 lists hold the mixed instances that are still on FP32.
-// mode 1 (FP64 lists): t = A64 x for mixed instances that asked for a true
-// residual (reliable replacement).
+// beyond the list's count return at once): q = A p and the tiles' shares of
+// p.q.  The FP64 lists hold the FP64 instances, the FP32 lists the mixed
+// instances, which run here while they are on the FP32 operator.  The FP64
+// work of mixed instances (after the progress guard's hand-over, and the
+// true-residual products) goes through device-built lists (k_bt_dyn_*).
 //
 // Every lane reads the instance's flags itself (uniform addresses that no
-// kernel writes while this one runs, so all lanes see the same values).
-// Where the item is almost always served (FP64 instances on the FP64 list,
-// FP32 instances on an FP32 list) the tile's loads are issued first and the
-// flags are checked while they are in flight; the other cases (mixed
-// instances on the FP64 list, true-residual passes) check before loading, so
-// the items they skip cost no value traffic.
-template <typename TV, typename TA, int NB, int MODE>
+// kernel writes while this one runs, so all lanes see the same values); the
+// tile's loads are issued first and the flags are checked while they are in
+// flight.
+template <typename TV, typename TA, int NB>
 __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts,
                                                  const uint32_t* __restrict__ work,
                                                  const int32_t* __restrict__ count) {
@@ -131,54 +128,158 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
   const int t = wl_index(e);
   if (!I.live || t >= I.ntiles) return;  // not a work item of an admitted instance
   const SolveScalars* sc = I.sc;
-  const bool early = MODE == 1 || (f64 && I.fp32);
-  if (early) {
-    const bool run = MODE == 1 ? (I.fp32 && sc->upd && !sc->done) : (!sc->done && sc->op64);
-    if (!run) return;
-  }
   const TV* vals = f64 ? (const TV*)I.v64 : (const TV*)I.v32;
   const TV* diag = f64 ? (const TV*)I.d64 : (const TV*)I.d32;
   TileRegs<TV, NB, PER> R;
   load_tile<TV, NB, PER>(I.a, t, vals, diag, R);
-  if (!early) {
-    const bool run = f64 ? !sc->done : (!sc->done && !sc->op64);
-    if (!run) return;
-  }
-  if (MODE == 0)  // the FP32-sum variant stages x from the FP32 copy of p
-    compute_tile<TV, TA, NB, PER, true>(I.a, t, R, I.p, I.q, I.epart, smem, I.tdot, 0,
-                                        sizeof(TA) == 4 ? I.p32 : nullptr);
-  else
-    compute_tile<TV, TA, NB, PER>(I.a, t, R, I.x, I.t, I.epart, smem);
+  const bool run = f64 ? !sc->done : (!sc->done && !sc->op64);
+  if (!run) return;
+  // the FP32-sum variant stages x from the FP32 copy of p and, for instances
+  // with FP32 vectors, stores its own rows of q in FP32
+  compute_tile<TV, TA, NB, PER, true>(I.a, t, R, I.p, I.q, I.epart, smem, I.tdot,
+                                      (sizeof(TA) == 4 && I.p32) ? 1 : 0,
+                                      sizeof(TA) == 4 ? I.p32 : nullptr);
 }
 
-// operator variants: v = 3 * (nb == 3) + {0: FP64, 1: FP32 values FP64 sums,
-// 2: FP32 values and sums}
+// Device-built lists of mixed instances with FP64 work, so that a captured
+// graph serves them without a grid per instance: list 2 * mode + (nb == 3),
+// mode 0: instances the progress guard handed over to the FP64 operator
+// (q = A64 p), mode 1: instances that asked for a true residual (t = A64 x,
+// then r = b - t).  Built by one workgroup in slot-list order: the slots and
+// the running totals of their tiles and chunks.
+struct DynList {
+  int32_t n;
+  int32_t slot[kMaxSlots];
+  int32_t tile_end[kMaxSlots];   // inclusive running total of ntiles
+  int32_t chunk_end[kMaxSlots];  // of nch
+};
+constexpr int kDynLists = 4;
+constexpr int kDynGrid = 1024;   // workgroups of the list kernels (loop over items)
+
+__device__ inline bool dyn_wants(const BInst& I, int mode) {
+  if (!I.live || !I.fp32) return false;
+  const SolveScalars* sc = I.sc;
+  return mode == 0 ? (sc->op64 && !sc->done) : (sc->upd && !sc->done);
+}
+
+__global__ __launch_bounds__(256) void k_bt_dyn_list(const BInst* __restrict__ insts,
+                                                     const int32_t* __restrict__ slots,
+                                                     const int32_t* __restrict__ count,
+                                                     DynList* lists, int mode) {
+  __shared__ int32_t f[2][256], tl[2][256], ch[2][256];
+  const int ns = *count;  // <= kMaxSlots < 256
+  const int w = threadIdx.x;
+  int flag[2] = {0, 0}, nt = 0, nc = 0, k = -1;
+  if (w < ns) {
+    k = slots[w];
+    const BInst& I = insts[k];
+    if (dyn_wants(I, mode)) {
+      flag[I.nb == 3 ? 1 : 0] = 1;
+      nt = I.ntiles;
+      nc = I.nch;
+    }
+  }
+  // inclusive scans over the slot list (Hillis-Steele in LDS, fixed order)
+  for (int l = 0; l < 2; ++l) {
+    f[l][w] = flag[l];
+    tl[l][w] = flag[l] ? nt : 0;
+    ch[l][w] = flag[l] ? nc : 0;
+  }
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    int a[2][3];
+    for (int l = 0; l < 2; ++l) {
+      a[l][0] = w >= d ? f[l][w - d] : 0;
+      a[l][1] = w >= d ? tl[l][w - d] : 0;
+      a[l][2] = w >= d ? ch[l][w - d] : 0;
+    }
+    __syncthreads();
+    for (int l = 0; l < 2; ++l) {
+      f[l][w] += a[l][0];
+      tl[l][w] += a[l][1];
+      ch[l][w] += a[l][2];
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < 2; ++l) {
+    DynList& L = lists[2 * mode + l];
+    if (flag[l]) {
+      const int pos = f[l][w] - 1;
+      L.slot[pos] = k;
+      L.tile_end[pos] = tl[l][w];
+      L.chunk_end[pos] = ch[l][w];
+    }
+    if (w == 255) L.n = f[l][255];
+  }
+}
+
+// Item w of a list (tiles or chunks): the entry whose running total first
+// exceeds w (entries in LDS; every lane walks the same path).
+__device__ inline int dyn_find(const int32_t* end, int n, int w, int k) {
+  while (k < n && end[k] <= w) ++k;
+  return k;
+}
+
+// FP64 products of the listed instances, each workgroup looping over the
+// list's tiles (mode 0: q = A64 p and the tiles' shares of p.q, mode 1:
+// t = A64 x).  The flags are checked again per item: a list built before an
+// instance finished (or before its slot was reused) skips it.
+template <int NB>
+__global__ __launch_bounds__(256) void k_bt_dyn_spmv(const BInst* __restrict__ insts,
+                                                     const DynList* __restrict__ lists, int mode) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int32_t s_slot[kMaxSlots], s_end[kMaxSlots];
+  constexpr int PER = batch_per<NB>();
+  const DynList& L = lists[2 * mode + (NB == 3 ? 1 : 0)];
+  const int n = L.n;
+  if (n == 0) return;
+  const int total = L.tile_end[n - 1];
+  if ((int)blockIdx.x >= total) return;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    s_slot[k] = L.slot[k];
+    s_end[k] = L.tile_end[k];
+  }
+  __syncthreads();
+  int k = 0;
+  for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    k = dyn_find(s_end, n, w, k);
+    const BInst& I = insts[s_slot[k]];
+    const int t = w - (k ? s_end[k - 1] : 0);
+    if (t >= I.ntiles || !dyn_wants(I, mode)) continue;
+    TileRegs<double, NB, PER> R;
+    load_tile<double, NB, PER>(I.a, t, I.v64, I.d64, R);
+    if (mode == 0)
+      compute_tile<double, double, NB, PER, true>(I.a, t, R, I.p, I.q, I.epart, smem, I.tdot);
+    else if (I.dx32)  // the product of x + dx32 (the FP32 increments since the last fold)
+      compute_tile<double, double, NB, PER, false, NoHook, 256, true>(
+          I.a, t, R, I.x, I.t, I.epart, smem, nullptr, 0, nullptr, NoHook(), I.dx32);
+    else
+      compute_tile<double, double, NB, PER>(I.a, t, R, I.x, I.t, I.epart, smem);
+  }
+}
+
+// operator variants of the static lists: v = 3 * (nb == 3) + {0: FP64,
+// 1: FP32 values FP64 sums, 2: FP32 values and sums}
 constexpr int kVariants = 6;
-template <int MODE>
 static const void* variant_kernel(int v) {
   switch (v) {
-    case 0: return (const void*)k_bt_spmv<double, double, 2, MODE>;
-    case 1: return (const void*)k_bt_spmv<float, double, 2, MODE>;
-    case 2: return (const void*)k_bt_spmv<float, float, 2, MODE>;
-    case 3: return (const void*)k_bt_spmv<double, double, 3, MODE>;
-    case 4: return (const void*)k_bt_spmv<float, double, 3, MODE>;
-    default: return (const void*)k_bt_spmv<float, float, 3, MODE>;
+    case 0: return (const void*)k_bt_spmv<double, double, 2>;
+    case 1: return (const void*)k_bt_spmv<float, double, 2>;
+    case 2: return (const void*)k_bt_spmv<float, float, 2>;
+    case 3: return (const void*)k_bt_spmv<double, double, 3>;
+    case 4: return (const void*)k_bt_spmv<float, double, 3>;
+    default: return (const void*)k_bt_spmv<float, float, 3>;
   }
 }
 static void launch_variant(int v, dim3 G, dim3 TB, size_t lds, hipStream_t s, const BInst* insts,
-                           const uint32_t* work, const int32_t* count, int mode) {
-  if (mode == 1) {  // only the FP64 lists serve the true residuals
-    if (v == 0) hipLaunchKernelGGL((k_bt_spmv<double, double, 2, 1>), G, TB, lds, s, insts, work, count);
-    else hipLaunchKernelGGL((k_bt_spmv<double, double, 3, 1>), G, TB, lds, s, insts, work, count);
-    return;
-  }
+                           const uint32_t* work, const int32_t* count) {
   switch (v) {
-    case 0: hipLaunchKernelGGL((k_bt_spmv<double, double, 2, 0>), G, TB, lds, s, insts, work, count); break;
-    case 1: hipLaunchKernelGGL((k_bt_spmv<float, double, 2, 0>), G, TB, lds, s, insts, work, count); break;
-    case 2: hipLaunchKernelGGL((k_bt_spmv<float, float, 2, 0>), G, TB, lds, s, insts, work, count); break;
-    case 3: hipLaunchKernelGGL((k_bt_spmv<double, double, 3, 0>), G, TB, lds, s, insts, work, count); break;
-    case 4: hipLaunchKernelGGL((k_bt_spmv<float, double, 3, 0>), G, TB, lds, s, insts, work, count); break;
-    default: hipLaunchKernelGGL((k_bt_spmv<float, float, 3, 0>), G, TB, lds, s, insts, work, count); break;
+    case 0: hipLaunchKernelGGL((k_bt_spmv<double, double, 2>), G, TB, lds, s, insts, work, count); break;
+    case 1: hipLaunchKernelGGL((k_bt_spmv<float, double, 2>), G, TB, lds, s, insts, work, count); break;
+    case 2: hipLaunchKernelGGL((k_bt_spmv<float, float, 2>), G, TB, lds, s, insts, work, count); break;
+    case 3: hipLaunchKernelGGL((k_bt_spmv<double, double, 3>), G, TB, lds, s, insts, work, count); break;
+    case 4: hipLaunchKernelGGL((k_bt_spmv<float, double, 3>), G, TB, lds, s, insts, work, count); break;
+    default: hipLaunchKernelGGL((k_bt_spmv<float, float, 3>), G, TB, lds, s, insts, work, count); break;
   }
 }
 
@@ -209,8 +310,18 @@ __global__ __launch_bounds__(kThreads) void k_bt_sum_shares(const BInst* __restr
   for (int w = blockIdx.x; w < ns; w += gridDim.x) {
     const BInst& I = insts[slots[w]];
     if (uniform_done(I, 0, sh)) continue;
+    // lane l adds shares l, l + 256, ... in index order, eight loads in flight
     double v = 0.0;
-    for (int k = threadIdx.x; k < I.ntiles; k += kThreads) v += I.tdot[k];
+    for (int k0 = threadIdx.x; k0 < I.ntiles; k0 += 8 * kThreads) {
+      double t[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = k0 + j * kThreads;
+        t[j] = k < I.ntiles ? I.tdot[k] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v += t[j];
+    }
     const double pq = block_sum(v, sh);
     if (threadIdx.x == 0) I.sc->pq_tiles = pq;
   }
@@ -219,15 +330,20 @@ __global__ __launch_bounds__(kThreads) void k_bt_sum_shares(const BInst* __restr
 // alpha = d / p.q (mixed: p.r / p.q) ; q = own rows + merged external
 // partials (node by node, not stored) ; x += alpha p ; r -= alpha q ;
 // partials r.(m r) and (mixed) (m r).q.  EP: type of the stored partials
-// (FP32 after a product with FP32 sums).
-template <int NB, typename EP>
-__device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, double* sh) {
-  // after a product with FP32 sums the iteration's p, r and m are FP32 too
-  typedef EP VT;
-  const VT* P = sizeof(VT) == 4 ? (const VT*)I.p32 : (const VT*)I.p;
-  VT* R = sizeof(VT) == 4 ? (VT*)I.r32 : (VT*)I.r;
-  const VT* M = sizeof(VT) == 4 ? (const VT*)I.m32 : (const VT*)I.m;
+// (FP32 after a product with FP32 sums).  V32: the instance keeps its
+// iteration vectors in FP32 (p, r, the Jacobi scale, its product's own rows
+// of q, and the steps of x as FP32 increments dx32 on the FP64 x, which
+// every true-residual replacement folds into x).
+template <int NB, typename EP, bool V32, typename AlphaFn>
+__device__ inline double bt_update_chunk(const BInst& I, int c, double* sh, AlphaFn alpha_fn) {
+  typedef typename std::conditional<V32, float, double>::type VT;
+  const VT* P = V32 ? (const VT*)I.p32 : (const VT*)I.p;
+  VT* R = V32 ? (VT*)I.r32 : (VT*)I.r;
+  const VT* M = V32 ? (const VT*)I.m32 : (const VT*)I.m;
+  const VT* Q = (const VT*)I.q;  // own rows as the product stored them
+  VT* X = V32 ? (VT*)I.dx32 : (VT*)I.x;
   const int J0 = c * (I.cr / NB), J1 = min(I.n / NB, J0 + I.cr / NB);
+  const double alpha = alpha_fn();
   double acc = 0.0, azq = 0.0;
   for (int J = J0 + threadIdx.x; J < J1; J += kThreads) {
     const int64_t i0 = (int64_t)NB * J;
@@ -235,8 +351,8 @@ __device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, doub
     double qv[NB], xv[NB], pv[NB], rv[NB], mv[NB];
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
-      qv[r] = I.q[i0 + r];
-      xv[r] = I.x[i0 + r];
+      qv[r] = (double)Q[i0 + r];
+      xv[r] = (double)X[i0 + r];
       pv[r] = (double)P[i0 + r];
       rv[r] = (double)R[i0 + r];
       mv[r] = (double)M[i0 + r];
@@ -249,7 +365,7 @@ __device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, doub
     double rn[NB], zv[NB];
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
-      I.x[i0 + r] = xv[r] + alpha * pv[r];
+      X[i0 + r] = (VT)(xv[r] + alpha * pv[r]);
       const VT rs = (VT)(rv[r] - alpha * qv[r]);
       R[i0 + r] = rs;
       rn[r] = (double)rs;
@@ -268,6 +384,7 @@ __device__ inline void bt_update_chunk(const BInst& I, int c, double alpha, doub
     const double t = block_sum(azq, sh);
     if (threadIdx.x == 0) I.part[4 * I.nch + c] = t;
   }
+  return alpha;
 }
 
 __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict__ insts,
@@ -282,21 +399,27 @@ __global__ __launch_bounds__(kThreads) void k_bt_update(const BInst* __restrict_
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
     if (uniform_done(I, c, sh)) return;
-    const double pq = I.sc->pq_tiles;
-    const double num = I.fp32 ? inst_sum(I.part + 3 * I.nch, I.nch, sh) : I.dslot[cur];
-    const double alpha = num / pq;
-    // the FP32-sum variant served this instance's product (see k_bt_spmv)
+    auto alpha_fn = [&]() {
+      const double num = I.fp32 ? inst_sum(I.part + 3 * I.nch, I.nch, sh) : I.dslot[cur];
+      return num / I.sc->pq_tiles;
+    };
+    // the FP32-sum variant served this instance's product (see k_bt_spmv);
+    // FP32 vectors only with it (p32 is set for such instances only)
     const bool e32 = I.fp32 && I.acc32 && !I.sc->op64;
+    const bool v32 = e32 && I.p32 != nullptr;
+    double alpha;
     if (I.nb == 3) {
-      if (e32) bt_update_chunk<3, float>(I, c, alpha, sh);
-      else bt_update_chunk<3, double>(I, c, alpha, sh);
+      if (v32) alpha = bt_update_chunk<3, float, true>(I, c, sh, alpha_fn);
+      else if (e32) alpha = bt_update_chunk<3, float, false>(I, c, sh, alpha_fn);
+      else alpha = bt_update_chunk<3, double, false>(I, c, sh, alpha_fn);
     } else {
-      if (e32) bt_update_chunk<2, float>(I, c, alpha, sh);
-      else bt_update_chunk<2, double>(I, c, alpha, sh);
+      if (v32) alpha = bt_update_chunk<2, float, true>(I, c, sh, alpha_fn);
+      else if (e32) alpha = bt_update_chunk<2, float, false>(I, c, sh, alpha_fn);
+      else alpha = bt_update_chunk<2, double, false>(I, c, sh, alpha_fn);
     }
     if (c == 0 && threadIdx.x == 0) {
       I.sc->alpha = alpha;
-      I.sc->pq = pq;
+      I.sc->pq = I.sc->pq_tiles;
     }
   }
 }
@@ -315,20 +438,40 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
     const BInst& I = insts[wl_slot(e)];
     const int c = wl_index(e);
     if (uniform_done(I, c, sh)) return;
+    const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
+    const bool v32 = I.p32 && !I.sc->op64;
+    // FP32 vectors: the lane's rows (at most kChunk / kThreads) are loaded
+    // before beta's sums, so that their latency overlaps
+    constexpr int kRows = kChunk / kThreads;
+    float rf[kRows], mf[kRows], pf[kRows];
+    if (v32) {
+#pragma unroll
+      for (int u = 0; u < kRows; ++u) {
+        const int i = i0 + threadIdx.x + u * kThreads;
+        if (i < i1) {
+          rf[u] = I.r32[i];
+          mf[u] = I.m32[i];
+          pf[u] = I.p32[i];
+        }
+      }
+    }
     const double di = inst_sum(I.part + I.nch, I.nch, sh);
     // restart (beta = 0) right after the hand-over to the FP64 operator
     const bool restart = I.sc->restart[cur] != 0;
     const double beta =
         restart ? 0.0
         : I.fp32 ? -inst_sum(I.part + 4 * I.nch, I.nch, sh) / I.sc->pq : di / I.dslot[cur];
-    const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
     double apr = 0.0;
-    if (I.p32 && !I.sc->op64) {  // FP32 vectors (see BInst)
-      for (int i = i0 + threadIdx.x; i < i1; i += kThreads) {
-        const double ri = (double)I.r32[i];
-        const float pi = (float)(ri * (double)I.m32[i] + beta * (double)I.p32[i]);
-        I.p32[i] = pi;
-        apr += (double)pi * ri;
+    if (v32) {  // FP32 vectors (see BInst)
+#pragma unroll
+      for (int u = 0; u < kRows; ++u) {
+        const int i = i0 + threadIdx.x + u * kThreads;
+        if (i < i1) {
+          const double ri = (double)rf[u];
+          const float pi = (float)(ri * (double)mf[u] + beta * (double)pf[u]);
+          I.p32[i] = pi;
+          apr += (double)pi * ri;
+        }
       }
     } else if (I.bm) {  // node-block Jacobi: one lane per node of the chunk
       const int nb = I.nb;
@@ -382,57 +525,78 @@ __global__ __launch_bounds__(kThreads) void k_bt_pupdate(const BInst* __restrict
   }
 }
 
-// true residual r = b - (t + merged partials) of instances that asked for it;
-// partials r.(m r) and p.r
-__global__ __launch_bounds__(kThreads) void k_bt_rel_resid(const BInst* __restrict__ insts,
-                                                           const uint32_t* __restrict__ work,
-                                                           const int32_t* __restrict__ count) {
+// true residual r = b - (t + merged partials) of chunk c of an instance that
+// asked for it; partials r.(m r) and p.r.  Called by every lane of the
+// workgroup (block sums inside).
+__device__ inline void rel_resid_chunk(const BInst& I, int c, double* sh) {
+  const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
+  const bool v32 = I.p32 && !I.sc->op64;
+  double acc = 0.0, apr = 0.0;
+  if (I.bm) {  // node-block Jacobi (FP64 vectors): one lane per node
+    const int nb = I.nb;
+    for (int J = i0 / nb + threadIdx.x; J < i1 / nb; J += kThreads) {
+      double rv[3], zv[3];
+      for (int r = 0; r < nb; ++r) {
+        const int i = nb * J + r;
+        rv[r] = I.b[i] - merge_row(I.mg, i, I.t[i]);
+        I.r[i] = rv[r];
+        apr += I.p[i] * rv[r];
+      }
+      if (nb == 3) node_apply<3>(I.bm, J, rv, zv);
+      else node_apply<2>(I.bm, J, rv, zv);
+      for (int r = 0; r < nb; ++r) acc += rv[r] * zv[r];
+    }
+  }
+  for (int i = i0 + threadIdx.x; i < (I.bm ? i0 : i1); i += kThreads) {
+    double s = I.t[i];
+    s = merge_row(I.mg, i, s);
+    const double ri = I.b[i] - s;
+    I.r[i] = ri;
+    acc += ri * (ri * I.m[i]);
+    if (v32) {  // FP32 vectors: store the rounded residual; refresh the FP64 p
+      if (I.dx32) {  // the product was formed on x + dx32: fold the increments
+        I.x[i] = I.x[i] + (double)I.dx32[i];
+        I.dx32[i] = 0.0f;
+      }
+      const float rs = (float)ri;
+      I.r32[i] = rs;
+      const double pi = (double)I.p32[i];
+      I.p[i] = pi;
+      apr += pi * (double)rs;
+    } else {
+      apr += I.p[i] * ri;
+    }
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) I.part[2 * I.nch + c] = s;
+  const double t = block_sum(apr, sh);  // only mixed instances ask for true residuals
+  if (threadIdx.x == 0) I.part[3 * I.nch + c] = t;
+}
+
+// the true-residual chunks of the instances on list 2 + nb3 (mode 1), each
+// workgroup looping over the list's chunks
+__global__ __launch_bounds__(kThreads) void k_bt_dyn_resid(const BInst* __restrict__ insts,
+                                                           const DynList* __restrict__ lists,
+                                                           int nb3) {
   __shared__ double sh[8];
-  const int w = blockIdx.x;  // one work item per workgroup
-  if (w >= *count) return;
-  {
-    const uint32_t e = work[w];
-    const BInst& I = insts[wl_slot(e)];
-    const int c = wl_index(e);
-    if (uniform_done(I, c, sh, true)) return;
-    const int i0 = c * I.cr, i1 = min(I.n, i0 + I.cr);
-    const bool v32 = I.p32 && !I.sc->op64;
-    double acc = 0.0, apr = 0.0;
-    if (I.bm) {  // node-block Jacobi (FP64 vectors): one lane per node
-      const int nb = I.nb;
-      for (int J = i0 / nb + threadIdx.x; J < i1 / nb; J += kThreads) {
-        double rv[3], zv[3];
-        for (int r = 0; r < nb; ++r) {
-          const int i = nb * J + r;
-          rv[r] = I.b[i] - merge_row(I.mg, i, I.t[i]);
-          I.r[i] = rv[r];
-          apr += I.p[i] * rv[r];
-        }
-        if (nb == 3) node_apply<3>(I.bm, J, rv, zv);
-        else node_apply<2>(I.bm, J, rv, zv);
-        for (int r = 0; r < nb; ++r) acc += rv[r] * zv[r];
-      }
-    }
-    for (int i = i0 + threadIdx.x; i < (I.bm ? i0 : i1); i += kThreads) {
-      double s = I.t[i];
-      s = merge_row(I.mg, i, s);
-      const double ri = I.b[i] - s;
-      I.r[i] = ri;
-      acc += ri * (ri * I.m[i]);
-      if (v32) {  // FP32 vectors: store the rounded residual; refresh the FP64 p
-        const float rs = (float)ri;
-        I.r32[i] = rs;
-        const double pi = (double)I.p32[i];
-        I.p[i] = pi;
-        apr += pi * (double)rs;
-      } else {
-        apr += I.p[i] * ri;
-      }
-    }
-    const double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) I.part[2 * I.nch + c] = s;
-    const double t = block_sum(apr, sh);  // only mixed instances ask for true residuals
-    if (threadIdx.x == 0) I.part[3 * I.nch + c] = t;
+  __shared__ int32_t s_slot[kMaxSlots], s_end[kMaxSlots];
+  const DynList& L = lists[2 + nb3];
+  const int n = L.n;
+  if (n == 0) return;
+  const int total = L.chunk_end[n - 1];
+  if ((int)blockIdx.x >= total) return;
+  for (int k = threadIdx.x; k < n; k += kThreads) {
+    s_slot[k] = L.slot[k];
+    s_end[k] = L.chunk_end[k];
+  }
+  __syncthreads();
+  int k = 0;
+  for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    k = dyn_find(s_end, n, w, k);
+    const BInst& I = insts[s_slot[k]];
+    const int c = w - (k ? s_end[k - 1] : 0);
+    if (c >= I.nch || !dyn_wants(I, 1)) continue;
+    rel_resid_chunk(I, c, sh);
   }
 }
 
@@ -499,6 +663,7 @@ __global__ __launch_bounds__(kThreads) void k_bt_init(const BInst* __restrict__ 
     acc += bi * zi;
     if (I.p32) {  // FP32 vectors (see BInst)
       I.p32[i] = (float)zi;
+      I.dx32[i] = 0.0f;
       I.r32[i] = (float)bi;
       I.m32[i] = (float)I.m[i];
       apr += (double)(float)bi * (double)(float)zi;
@@ -537,6 +702,15 @@ __global__ void k_bt_init2(const BInst* __restrict__ insts, int slot, double tol
     sc->pq_tiles = 0.0;
     progress_init(sc, sqrt(fabs(d)));
     sc->done = (d == 0.0) ? 1 : 0;
+  }
+}
+
+// x += dx32, dx32 = 0: the FP32 increments of an instance that stopped
+// without a final replacement (maxit, or a batch closed under it)
+__global__ __launch_bounds__(kThreads) void k_bt_fold(int n, double* x, float* dx32) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    x[i] = x[i] + (double)dx32[i];
+    dx32[i] = 0.0f;
   }
 }
 
@@ -591,7 +765,10 @@ struct elastic_batch {
   size_t vwork_cap = 0;
   std::vector<uint32_t> h_sw[kVariants], h_vwork;
   std::vector<int32_t> h_slist, h_counts;
-  unsigned mask = 0, graph_mask = ~0u;  // non-empty variant lists (+ bit 8: mixed present)
+  // non-empty variant lists, + bit 8: mixed instances present, bits 9 / 10:
+  // 2-D / 3-D mixed instances present (their device-built lists are served)
+  unsigned mask = 0, graph_mask = ~0u;
+  DynList* d_dyn = nullptr;  // kDynLists device-built lists (see DynList)
   // grid of each product variant: a power-of-two size class of its list
   // (one workgroup per item); a captured graph holds the classes it was
   // captured with
@@ -616,8 +793,23 @@ static int berr(elastic_batch* b, int code, const std::string& m) {
                                                                    hipGetErrorString(e_));   \
   } while (0)
 
+// LDS of one tile's product with FP64 sums (FP32 sums take half)
 static size_t tile_lds(int nb, int ncols_max, int nblk_max) {
   return sizeof(double) * nb * ((size_t)ncols_max + 2 * kBlkRowsMax + (size_t)nblk_max);
+}
+// the dynamic LDS of variant v's launch: the batch's largest tile with the
+// variant's sum type
+static size_t variant_lds(const elastic_batch* b, int v) { return v % 3 == 2 ? b->lds / 2 : b->lds; }
+// static LDS of the list kernels (slots and running totals)
+constexpr size_t kDynStaticLds = 2 * kMaxSlots * sizeof(int32_t) + 64;
+
+static void launch_dyn_spmv(elastic_batch* b, int nb3, int mode) {
+  if (nb3)
+    hipLaunchKernelGGL((k_bt_dyn_spmv<3>), dim3(kDynGrid), dim3(256), b->lds, b->stream,
+                       b->d_insts, b->d_dyn, mode);
+  else
+    hipLaunchKernelGGL((k_bt_dyn_spmv<2>), dim3(kDynGrid), dim3(256), b->lds, b->stream,
+                       b->d_insts, b->d_dyn, mode);
 }
 
 // one PCG iteration of every active instance; `it` is the iteration index in
@@ -627,34 +819,51 @@ static void enqueue_iteration(elastic_batch* b, int it) {
   const dim3 TB(kThreads);
   const int cur = it & 1;
   const int32_t* vcount = b->d_counts + kVariants;
+  const int32_t* scount = vcount + 1;
+  // mixed instances: FP64 products of those the progress guard handed over.
+  // A hand-over happens only in the true-residual pass that ends odd
+  // iterations, and a step starts at an even one, so the list is rebuilt on
+  // the device at every even iteration: a hand-over inside a captured step is
+  // served from the next iteration on.
+  const bool mixed = (b->mask & 0x100u) != 0;
+  if (mixed) {
+    if (cur == 0)
+      hipLaunchKernelGGL(k_bt_dyn_list, dim3(1), dim3(256), 0, s, b->d_insts, b->d_slist, scount,
+                         b->d_dyn, 0);
+    for (int nb3 = 0; nb3 < 2; ++nb3)
+      if (b->mask & (nb3 ? 0x400u : 0x200u))
+        launch_dyn_spmv(b, nb3, 0);
+  }
   for (int v = 0; v < kVariants; ++v)
     if (b->mask & (1u << v))
-      launch_variant(v, dim3(b->sgrid[v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
-                     b->d_counts + v, 0);
-  hipLaunchKernelGGL(k_bt_sum_shares, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist,
-                     vcount + 1);
+      launch_variant(v, dim3(b->sgrid[v]), TB, variant_lds(b, v), s, b->d_insts, b->d_sw[v],
+                     b->d_counts + v);
+  hipLaunchKernelGGL(k_bt_sum_shares, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist, scount);
   hipLaunchKernelGGL(k_bt_update, dim3(b->vgrid), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
   hipLaunchKernelGGL(k_bt_pupdate, dim3(b->vgrid), TB, 0, s, b->d_insts, b->d_vwork, vcount, cur);
-  if (cur == 1 && (b->mask & 0x100u)) {  // every second iteration: true residuals where asked
-    for (int v = 0; v < kVariants; v += 3)  // the FP64 lists
-      if (b->mask & (1u << v))
-        launch_variant(v, dim3(b->sgrid[v]), TB, b->lds, s, b->d_insts, b->d_sw[v],
-                       b->d_counts + v, 1);
-    hipLaunchKernelGGL(k_bt_rel_resid, dim3(b->vgrid), TB, 0, s, b->d_insts, b->d_vwork, vcount);
+  if (cur == 1 && mixed) {  // every second iteration: true residuals where asked
+    hipLaunchKernelGGL(k_bt_dyn_list, dim3(1), dim3(256), 0, s, b->d_insts, b->d_slist, scount,
+                       b->d_dyn, 1);
+    for (int nb3 = 0; nb3 < 2; ++nb3)
+      if (b->mask & (nb3 ? 0x400u : 0x200u)) {
+        launch_dyn_spmv(b, nb3, 1);
+        hipLaunchKernelGGL(k_bt_dyn_resid, dim3(kDynGrid), TB, 0, s, b->d_insts, b->d_dyn, nb3);
+      }
     hipLaunchKernelGGL(k_bt_rel_finish, dim3(kMaxSlots), TB, 0, s, b->d_insts, b->d_slist,
-                       vcount + 1, 0);
+                       scount, 0);
   }
 }
 
 static int set_lds_limit(elastic_batch* b) {
-  if (b->lds > 64 * 1024)
-    for (int v = 0; v < kVariants; ++v) {
-      BATCH_HIP(b, hipFuncSetAttribute(variant_kernel<0>(v),
+  if (b->lds + kDynStaticLds > 64 * 1024) {
+    for (int v = 0; v < kVariants; ++v)
+      BATCH_HIP(b, hipFuncSetAttribute(variant_kernel(v),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-      if (v % 3 == 0)
-        BATCH_HIP(b, hipFuncSetAttribute(variant_kernel<1>(v),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    }
+    BATCH_HIP(b, hipFuncSetAttribute((const void*)k_bt_dyn_spmv<2>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024));
+    BATCH_HIP(b, hipFuncSetAttribute((const void*)k_bt_dyn_spmv<3>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024));
+  }
   return ELASTIC_OK;
 }
 
@@ -664,30 +873,37 @@ static int upload_lists(elastic_batch* b) {
   b->h_slist.clear();
   size_t lds = 0;
   bool mixed = false;
+  unsigned mixed_nb = 0;
   for (int k = 0; k < b->max_slots; ++k) {
     const auto& S = b->slots[k];
     if (!S.used) continue;
     const BInst& I = b->h_insts[k];
     b->h_slist.push_back(k);
     const int v0 = I.nb == 3 ? 3 : 0;
-    for (int t = 0; t < S.ntiles; ++t) b->h_sw[v0].push_back(((uint32_t)k << 24) | (uint32_t)t);
+    // FP64 instances on the FP64 list; mixed ones on an FP32 list (their FP64
+    // work goes through the device-built lists)
+    const int v = I.fp32 ? v0 + (I.acc32 ? 2 : 1) : v0;
+    for (int t = 0; t < S.ntiles; ++t) b->h_sw[v].push_back(((uint32_t)k << 24) | (uint32_t)t);
     if (I.fp32) {
       mixed = true;
-      const int v = v0 + (I.acc32 ? 2 : 1);
-      for (int t = 0; t < S.ntiles; ++t) b->h_sw[v].push_back(((uint32_t)k << 24) | (uint32_t)t);
+      mixed_nb |= I.nb == 3 ? 0x400u : 0x200u;
     }
     for (int c = 0; c < S.nch; ++c) b->h_vwork.push_back(((uint32_t)k << 24) | (uint32_t)c);
     lds = std::max(lds, S.lds);
   }
-  unsigned mask = mixed ? 0x100u : 0u;
+  unsigned mask = mixed ? 0x100u | mixed_nb : 0u;
   for (int v = 0; v < kVariants; ++v)
     if (!b->h_sw[v].empty()) mask |= 1u << v;
-  // grid size classes: the next power of two of the count, kept while the
-  // count stays above half of it (fewer re-captures as instances leave)
+  // grid size classes: the count rounded up to 16 steps per octave (at most
+  // 1/16 of the next power of two idle), kept while the count stays above
+  // 3/4 of it (fewer re-captures as instances leave)
   auto size_class = [](int n, int cur) {
+    if (n <= 0) return 0;
     int g = 64;
     while (g < n) g <<= 1;
-    return (cur >= n && 2 * n > cur) ? cur : (n ? g : 0);
+    const int step = std::max(8, g / 16);
+    const int c = (n + step - 1) / step * step;
+    return (cur >= n && 4 * n > 3 * cur) ? cur : c;
   };
   for (int v = 0; v < kVariants; ++v) b->sgrid[v] = size_class((int)b->h_sw[v].size(), b->sgrid[v]);
   b->vgrid = size_class((int)b->h_vwork.size(), b->vgrid);
@@ -751,6 +967,13 @@ static int capture(elastic_batch* b) {
   return ELASTIC_OK;
 }
 
+// folds an instance's FP32 increments of x (no-op without them)
+static void fold_increments(const BInst& I, hipStream_t s) {
+  if (!I.dx32) return;
+  const int g = std::max(1, std::min((I.n + kThreads - 1) / kThreads, 2048));
+  hipLaunchKernelGGL(k_bt_fold, dim3(g), dim3(kThreads), 0, s, I.n, I.x, I.dx32);
+}
+
 extern "C" {
 
 int elastic_batch_create(elastic_batch** out, int32_t device, int32_t max_slots,
@@ -777,6 +1000,8 @@ int elastic_batch_create(elastic_batch** out, int32_t device, int32_t max_slots,
       hipMalloc((void**)&b->d_slist, sizeof(int32_t) * max_slots) != hipSuccess ||
       hipMalloc((void**)&b->d_counts, sizeof(int32_t) * (kVariants + 2)) != hipSuccess ||
       hipMalloc((void**)&b->d_status, sizeof(BStatus) * max_slots) != hipSuccess ||
+      hipMalloc((void**)&b->d_dyn, sizeof(DynList) * kDynLists) != hipSuccess ||
+      hipMemset(b->d_dyn, 0, sizeof(DynList) * kDynLists) != hipSuccess ||
       hipHostMalloc((void**)&b->h_status, sizeof(BStatus) * max_slots) != hipSuccess) {
     elastic_batch_destroy(b);
     return ELASTIC_HIP_ERROR;
@@ -796,15 +1021,18 @@ int elastic_batch_destroy(elastic_batch* b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   // contexts of instances never retired go back to the CSRC order (with the
   // x their iteration reached)
-  for (auto& S : b->slots)
+  for (size_t k = 0; k < b->slots.size(); ++k) {
+    auto& S = b->slots[k];
     if (S.used && S.ctx) {
+      fold_increments(b->h_insts[k], S.ctx->c.stream);
       (void)leave_layout_order(&S.ctx->c);
       hipStreamSynchronize(S.ctx->c.stream);
     }
+  }
   if (b->gexec) hipGraphExecDestroy(b->gexec);
   if (b->graph) hipGraphDestroy(b->graph);
   for (void* p : {(void*)b->d_insts, (void*)b->d_vwork, (void*)b->d_slist, (void*)b->d_counts,
-                  (void*)b->d_status})
+                  (void*)b->d_status, (void*)b->d_dyn})
     if (p) hipFree(p);
   for (uint32_t* p : b->d_sw)
     if (p) hipFree(p);
@@ -826,6 +1054,8 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   if (c->layout_nb < 2) return berr(b, ELASTIC_BAD_ARG, "batched solves need a node-blocked layout");
   if (c->blk.nblk_max > 256 * (c->layout_nb == 3 ? batch_per<3>() : batch_per<2>()))
     return berr(b, ELASTIC_BAD_ARG, "tiles longer than the batch product holds (512 blocks in 2-D, 1024 in 3-D)");
+  if (tile_lds(c->layout_nb, c->blk.ncols_max, c->blk.nblk_max) + kDynStaticLds > 156 * 1024)
+    return berr(b, ELASTIC_BAD_ARG, "tiles with more local columns than the batch product's LDS holds");
   if (c->zero_diag_row > 0) return berr(b, ELASTIC_ZERO_DIAG, "zero diagonal coefficient");
   if (precision != ELASTIC_FP64 && precision != ELASTIC_MIXED) return ELASTIC_BAD_ARG;
   if (maxit <= 0) return ELASTIC_BAD_ARG;
@@ -878,6 +1108,7 @@ int elastic_batch_submit(elastic_batch* b, elastic_ctx* x, int32_t precision, do
   I.p32 = v32 ? c->p32 : nullptr;
   I.r32 = v32 ? c->p32 + vec32_stride(c->neq) : nullptr;
   I.m32 = v32 ? c->p32 + 2 * vec32_stride(c->neq) : nullptr;
+  I.dx32 = v32 ? c->p32 + 3 * vec32_stride(c->neq) : nullptr;
   I.m = c->A64.m;
   I.bm = bm;
   I.part = c->red64;
@@ -962,6 +1193,12 @@ int elastic_batch_poll(elastic_batch* b, int64_t* tags, elastic_stats* stats, in
     auto& S = b->slots[k];
     if (!S.used) continue;
     tags[nd] = S.tag;
+    // an instance stopped by maxit still holds FP32 increments of x (a
+    // converged one folded them at its last replacement)
+    if (st.status != 0) {
+      fold_increments(b->h_insts[k], S.ctx->c.stream);
+      BATCH_HIP(b, hipStreamSynchronize(S.ctx->c.stream));
+    }
     if (stats) {
       elastic_stats& o = stats[nd];
       std::memset(&o, 0, sizeof(o));

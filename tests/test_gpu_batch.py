@@ -270,3 +270,66 @@ def test_batch_node_block_jacobi(golden, name):
     assert sx["status"] == 0 and rel(xx, g["x"]) <= 1e-8
     for o in others + [c]:
         o.close()
+
+
+def test_mixed_precisions_and_node_sizes_in_one_batch(golden):
+    """FP64 and mixed-precision instances, 2-D and 3-D, in one batch (the
+    FP64 list, the FP32 lists and the device-built lists of the mixed
+    instances' FP64 products all active): every instance gives the same x,
+    bit for bit, and the same iteration count as alone, and meets its bar
+    against the reference's x."""
+    from elastic_amd.serving import Batch
+
+    names = ["hex8_5j", "tria3_8", "tet4_3j", "quad4_8"]
+    gs = [golden(n) for n in names]
+    ctxs = [make_ctx(g) for g in gs]
+    precs = [MIXED, MIXED, FP64, FP64]
+    ctxs[0].set_option("accum32", 1)
+    alone = [run_batch([c], p, 1e-10)[0] for c, p in zip(ctxs, precs)]
+    out = {}
+    with Batch(0, max_slots=4, iters_per_step=8) as b:
+        for i, (c, p) in enumerate(zip(ctxs, precs)):
+            assert b.submit(c, p, 1e-10, 20000, i)
+        steps = 0
+        while len(out) < len(ctxs):
+            b.step()
+            steps += 1
+            for tag, st in b.poll():
+                out[tag] = (ctxs[tag].download_x(), st)
+            assert steps < 100000
+    for i, g in enumerate(gs):
+        print("mixed batch", names[i], precs[i], out[i][1]["iters"], flush=True)
+        assert out[i][1]["iters"] == alone[i][1]["iters"]
+        assert np.array_equal(out[i][0], alone[i][0])
+        assert rel(out[i][0], g["x"]) <= (1e-8 if precs[i] == MIXED else 1e-10)
+    for c in ctxs:
+        c.close()
+
+
+def test_maxit_keeps_fp32_increments_of_x(golden, monkeypatch):
+    """An FP32-vector instance (FP32 sums, Jacobi) stopped by maxit before any
+    true-residual replacement: its x is the sum of the FP32 increments it
+    kept, which retirement folds into x.  The returned x must reduce the
+    residual of the reference's system like the iteration did (x = 0 would
+    leave ||b - A x|| = ||b||)."""
+    from oracle import binding as O
+    from elastic_amd.serving import Batch
+
+    monkeypatch.setenv("ELASTIC_RELIABLE_DELTA", "1e-30")  # no replacement before maxit
+    g = golden("hex8_5j")
+    c = make_ctx(g)
+    c.set_option("accum32", 1)
+    with Batch(0, max_slots=2, iters_per_step=8) as b:
+        assert b.submit(c, MIXED, 1e-14, 24, 0)
+        done = []
+        while not done:
+            b.step()
+            done = b.poll()
+    st = done[0][1]
+    assert st["status"] != 0 and st["iters"] == 24
+    x = c.download_x()
+    y = O.matvec(g["ia"], g["ja"], g["ad"], g["al"], x)
+    rr = np.linalg.norm(g["b"] - y) / np.linalg.norm(g["b"])
+    print("maxit fold: rel residual", rr, flush=True)
+    assert rr < 0.5
+    c.close()
