@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--plastic-compare", type=int, default=32,
                     help="J2 hex8 60^3 load paths whose tangent systems are solved "
                          "sequentially and batched (0: skip)")
+    ap.add_argument("--plastic-last", type=int, default=0,
+                    help="stop the J2 end-to-end run after this increment (0: all 10; "
+                         "fewer for a profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve path is not sharded across GPUs; "
@@ -217,7 +220,7 @@ def main():
     plastic = None
     if args.plastic_compare > 0:
         phase("serving done")
-        plastic = plastic_compare(args.plastic_compare)
+        plastic = plastic_compare(args.plastic_compare, last=args.plastic_last or None)
 
     value = neq * args.steps / t_total
     out = {
@@ -364,7 +367,7 @@ def serving_bench(nreq: int, ncompare: int, fixed: bool = False):
     return out
 
 
-def plastic_compare(npaths: int, n: int = 60, increments: int = 10):
+def plastic_compare(npaths: int, n: int = 60, increments: int = 10, last=None):
     """The J2 workload (hex8 60^3 between grips, E = 1, nu = 0.3, H = 0.05,
     yield stresses U[5e-4, 2e-3], load in 10 increments), run end to end: the
     paths' batched mixed-precision Newton iterations (GPU re-assembly of K_t
@@ -406,7 +409,7 @@ def plastic_compare(npaths: int, n: int = 60, increments: int = 10):
                                                "of": 8 * paths[0].mesh.numel}
     with Batch(0, max_slots=slots, iters_per_step=16) as b:
         a = time.perf_counter()
-        run_paths(paths, increments=increments, precision=MIXED, batch=b, first=2,
+        run_paths(paths, increments=increments, precision=MIXED, batch=b, first=2, last=last,
                   counts=counts2, progress=sys.stderr)
         t_rest = time.perf_counter() - a
     wall = t_inc1 + t_rest
@@ -414,8 +417,9 @@ def plastic_compare(npaths: int, n: int = 60, increments: int = 10):
     plastic_end = [int(p.commit()[1]) for p in paths]
     out["end_to_end"] = {
         "workload": "J2 hex8 %d^3 pulled 5e-3 between grips, %d load paths, sy U[5e-4, 2e-3], "
-                    "H 0.05, %d increments, Newton to 1e-8, linear solves to 1e-11"
-                    % (n, npaths, increments),
+                    "H 0.05, %d increments%s, Newton to 1e-8, linear solves to 1e-11"
+                    % (n, npaths, increments,
+                       "" if not last or last >= increments else " (stopped after %d)" % last),
         "method": "batched mixed-precision PCG (FP32 operator, FP64 residual replacement), "
                   "GPU re-assembly of K_t and R every Newton iteration",
         "wall_s": wall, "linear_solves": solves,
