@@ -108,3 +108,20 @@ def test_gmres_maxit_is_reported(ctx, golden):
     load(ctx, g)
     x, st = ctx.solve(g["b"], solver=GMRES, tol=1e-12, maxit=1, ngram=5, allow_maxit=True)
     assert st["status"] == MAXIT_CODE and st["iters"] == 5 and st["outer"] == 1
+
+
+def test_mixed_gmres_converging_in_the_last_allowed_cycle(ctx, golden):
+    """Mixed GMRES tests its FP64 residual at the top of the next cycle; when
+    the cycle limit (maxit) ends the loop right after the cycle that
+    converged, the final FP64 test still sees it: with maxit equal to the
+    cycles a free run needs, the solve reports success and the same x."""
+    g = golden("hex8_5j")
+    load(ctx, g)
+    x0, s0 = ctx.solve(g["b"], solver=GMRES, precision=MIXED, tol=1e-10, ngram=10, maxit=1000)
+    assert s0["status"] == 0 and s0["outer"] >= 2
+    x1, s1 = ctx.solve(g["b"], solver=GMRES, precision=MIXED, tol=1e-10, ngram=10,
+                       maxit=s0["outer"])
+    print("gmres cycles", s0["outer"], s1["outer"], s1["status"], flush=True)
+    assert s1["status"] == 0 and s1["outer"] == s0["outer"]
+    assert np.array_equal(x0, x1)
+    assert rel(x1, g["x"]) <= 1e-8
