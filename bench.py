@@ -270,6 +270,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs32 / HBM_PEAK_GBS, "bytes_per_launch": m32,
                      "traffic": measured_traffic(neq, nad, m32),
+                     "traffic_source": ("profiles/spmv_traffic_hex8_100.json: rocprofv3 FETCH_SIZE "
+                                        "and WRITE_SIZE of this kernel on a layout with these "
+                                        "model bytes, scaled by the copy probe's readings"),
                      "kernel": ("k_spmv_blocks<float,%s,%d,PER,%d lanes> with tile shares of p.q"
                                 % ("float" if acc32 else "double", nb,
                                    lanes32))
