@@ -355,7 +355,7 @@ __device__ inline double bt_update_chunk(const BInst& I, int c, double* sh, Alph
       xv[r] = (double)X[i0 + r];
       pv[r] = (double)P[i0 + r];
       rv[r] = (double)R[i0 + r];
-      mv[r] = (double)M[i0 + r];
+      mv[r] = I.bm ? 0.0 : (double)M[i0 + r];  // node-block Jacobi applies bm below
     }
     for (int k = k0; k < k1; ++k) {
       const EP* e = reinterpret_cast<const EP*>(I.epart) + (int64_t)NB * I.mg.idx[k];

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
       xv[c] = (double)x[i0 + c];
       pv[c] = (double)p[i0 + c];
       rv[c] = (double)r[i0 + c];
-      mv[c] = (double)m[i0 + c];
+      mv[c] = jacobi ? (double)m[i0 + c] : 0.0;  // (the block preconditioner's r.z is k_rz's)
     }
   };
   if (J < nn) load_node(J);
