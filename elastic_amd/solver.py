@@ -86,9 +86,10 @@ class Context:
     def layout_info(self):
         info = np.zeros(4, np.int64)
         self._chk(self._L.elastic_layout_info(self._h, info))
-        # ncols_max holds the node block size of the active layout (1 = scalar tiles)
+        # nb: node block size of the active layout (1 = scalar tiles, 0 = the
+        # gather product); nnz_max: blocks (or nonzeros) of the largest tile
         return dict(ntiles=int(info[0]), nnz_max=int(info[1]), ext_partials=int(info[2]),
-                    ncols_max=int(info[3]), nb=int(info[3]))
+                    nb=int(info[3]))
 
     def set_option(self, name: str, value: int):
         """Tuning knob of the context (see elastic_set_option)."""

@@ -183,7 +183,7 @@ def test_layouts_agree(golden, name, layout, monkeypatch):
     g = golden(name)
     with Context(0) as c:
         load_system(c, g)
-        nb = c.layout_info()["ncols_max"]
+        nb = c.layout_info()["nb"]
         assert nb == (1 if layout == "scalar" else (3 if name[:3] in ("hex", "tet") else 2))
         assert rel(c.spmv(g["spmv_v"]), g["spmv_y"]) <= 1e-13
         assert rel(c.spmv(g["spmv_v"], precision=MIXED), g["spmv_y"]) <= 2e-6
@@ -207,7 +207,7 @@ def test_partially_restrained_nodes_use_scalar_tiles():
     with Context(0) as c:
         c.set_structure(s.ia, s.ja)
         c.set_values(s.ad, s.al[: s.nad])
-        assert c.layout_info()["ncols_max"] == 1
+        assert c.layout_info()["nb"] == 1
         x, st = c.solve(s.b, tol=1e-10)
     xo, so = O.pcg(s.ia, s.ja, s.ad, s.al[: s.nad], s.b, tol=1e-10)
     assert abs(st["iters"] - so["iters"]) <= 2 and rel(x, xo) <= 1e-10
