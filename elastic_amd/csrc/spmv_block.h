@@ -274,9 +274,26 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
         for (int r = 0; r < NB; ++r) s[r] += pp[(2 * c + 1) * NB + r];
       }
     }
-    for (int q = sp[c]; q < sp[c + 1]; ++q)
+    {
+      // the column's transposed products in sorted order; four entries' LDS
+      // reads are issued together, then added in order
+      int q = sp[c];
+      const int qe = sp[c + 1];
+      for (; q + 4 <= qe; q += 4) {
+        T v[4][NB];
 #pragma unroll
-      for (int r = 0; r < NB; ++r) s[r] += tr[q * NB + r];
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < NB; ++r) v[j][r] = tr[(q + j) * NB + r];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < NB; ++r) s[r] += v[j][r];
+      }
+      for (; q < qe; ++q)
+#pragma unroll
+        for (int r = 0; r < NB; ++r) s[r] += tr[q * NB + r];
+    }
     if (c < nr) {
       if (sizeof(T) == 4 && y32) {  // FP32 sums, y asked in FP32 (exact)
         float* yf = reinterpret_cast<float*>(y);
