@@ -281,8 +281,25 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "batched_3d": plastic,
+        "speedups_vs_plain_fp64": speedup_summary(float(np.mean(lat64)) / float(np.mean(lat)),
+                                                  serving, plastic),
     }
     print(json.dumps(out), flush=True)
+
+
+def speedup_summary(single: float, serving, plastic):
+    """The ratios against plain FP64 GPU PCG that the mixed and batched forms
+    reach, gathered from the sections above (None where a section was
+    skipped)."""
+    out = {"hex8_100_single_system_mixed": single}
+    so = (serving or {}).get("solve_only")
+    if so:
+        out["plane_strain_48_batched_policy"] = so["batched_policy"]["speedup_vs_sequential_fp64"]
+        out["plane_strain_48_batched_mixed"] = so["batched_mixed"]["speedup_vs_sequential_fp64"]
+    if plastic:
+        out["j2_hex8_60_32_systems_batched_mixed_fp32_sums"] = (
+            plastic["batched_mixed_fp32_sums"]["speedup_vs_sequential_fp64"])
+    return out
 
 
 def serving_bench(nreq: int, ncompare: int, fixed: bool = False):
