@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--plastic-compare", type=int, default=32,
                     help="J2 hex8 60^3 load paths whose tangent systems are solved "
                          "sequentially and batched (0: skip)")
+    ap.add_argument("--no-tet", action="store_true",
+                    help="skip the tet4 80^3 section (the ill-conditioned workload)")
     ap.add_argument("--plastic-last", type=int, default=0,
                     help="stop the J2 end-to-end run after this increment (0: all 10; "
                          "fewer for a profile)")
@@ -212,10 +214,14 @@ def main():
                       "oracle": cpu["kind"] + " pcg_omp FP64 solve to %g" % TOL}
 
     phase("cpu baseline done")
+    ctx.close()
+    del sysm, ctx
+    tet = None
+    if not args.no_tet:
+        tet = tet_section()
+        phase("tet4 80^3 done")
     serving = None
     if args.serving > 0 or args.serving_compare > 0:
-        ctx.close()
-        del sysm, ctx
         serving = serving_bench(args.serving, args.serving_compare, args.serving_fixed)
     plastic = None
     if args.plastic_compare > 0:
@@ -281,13 +287,14 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "batched_3d": plastic,
+        "tet4_80": tet,
         "speedups_vs_plain_fp64": speedup_summary(float(np.mean(lat64)) / float(np.mean(lat)),
-                                                  serving, plastic),
+                                                  serving, plastic, tet),
     }
     print(json.dumps(out), flush=True)
 
 
-def speedup_summary(single: float, serving, plastic):
+def speedup_summary(single: float, serving, plastic, tet=None):
     """The ratios against plain FP64 GPU PCG that the mixed and batched forms
     reach, gathered from the sections above (None where a section was
     skipped)."""
@@ -296,9 +303,49 @@ def speedup_summary(single: float, serving, plastic):
     if so:
         out["plane_strain_48_batched_policy"] = so["batched_policy"]["speedup_vs_sequential_fp64"]
         out["plane_strain_48_batched_mixed"] = so["batched_mixed"]["speedup_vs_sequential_fp64"]
+    if tet:
+        out["tet4_80_best_mixed_vs_fp64_jacobi"] = tet["best_mixed_vs_fp64_jacobi"]
     if plastic:
         out["j2_hex8_60_32_systems_batched_mixed_fp32_sums"] = (
             plastic["batched_mixed_fp32_sums"]["speedup_vs_sequential_fp64"])
+    return out
+
+
+def tet_section():
+    """The tet4 workload (80^3 cells split into 6 tetrahedra, jitter 0.2h,
+    E = 10^U[-1,1] per element, nu = 0.45; 1,574,640 equations): PCG to 1e-10
+    in FP64 and with the FP32 operator inside the FP64 reliable-update loop,
+    with Jacobi and with the 3x3 node-block Jacobi (block_precond), each
+    solve timed once between device synchronisations; errors against an FP64
+    solve to 1e-12."""
+    import torch
+
+    from elastic_amd import mesh as M
+    from elastic_amd.solver import BDIAG3, DIAG, FP64, MIXED
+    from elastic_amd.system import System
+
+    s = System(M.config_mesh(3))
+    c = s.ctx
+    c.upload_rhs(s.b)
+    c.solve_resident(precision=FP64, tol=1e-12, maxit=200000)
+    xref = c.download_x()
+    nref = float(np.linalg.norm(xref))
+    out = {"workload": "tet4 80^3 x 6 jittered 0.2h, E 10^U[-1,1], nu 0.45 (%d DOF), PCG to %g"
+                       % (s.neq, TOL)}
+    for pname, pc in (("jacobi", DIAG), ("block_jacobi_3x3", BDIAG3)):
+        for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+            c.solve_resident(precond=pc, precision=prec, tol=TOL, maxit=200000)  # warm-up
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            st = c.solve_resident(precond=pc, precision=prec, tol=TOL, maxit=200000)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - a
+            err = float(np.linalg.norm(c.download_x() - xref)) / nref
+            out["%s_%s" % (name, pname)] = {"ms": 1e3 * t, "iters": st["iters"],
+                                            "dof_per_s": s.neq / t, "rel_err_vs_tight": err}
+    best = min(out["mixed_jacobi"]["ms"], out["mixed_block_jacobi_3x3"]["ms"])
+    out["best_mixed_vs_fp64_jacobi"] = out["fp64_jacobi"]["ms"] / best
+    c.close()
     return out
 
 

@@ -159,7 +159,10 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update(int n, const double* __
 // is then the vector the product staged, so x += alpha p moves along exactly
 // that vector).  XT: type of the stored x; float when the iteration adds its
 // steps to FP32 increments dx (x = x64 + dx), which every true-residual
-// replacement folds into the FP64 x (k_rel_resid) and clears.
+// replacement folds into the FP64 x (k_rel_resid) and clears.  bm (3-dof
+// layouts, FP32 vectors): the 3x3 node-block inverse of block_precond
+// (c11 c22 c33 c21 c31 c32 per node, Precond.f:419-540) gives z = M^-1 r of
+// the node here, so r.z and z.q need no pass of their own.
 template <int NB, typename EP, typename QT, typename VT, typename XT = double>
 __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs mg,
                                                                 const double* __restrict__ part_pr,
@@ -169,7 +172,8 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
                                                                 VT* r, double* q, int keep_q,
                                                                 const VT* __restrict__ m,
                                                                 int jacobi, double* part_rz,
-                                                                double* part_zq, SolveScalars* sc) {
+                                                                double* part_zq, SolveScalars* sc,
+                                                                const double* __restrict__ bm) {
   __shared__ double sh[8];
   if (block_flag(&sc->done, sh)) return;
   // the loads of each lane's first node are issued before alpha's
@@ -210,9 +214,24 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
       xo[c] = xv[c] + alpha * pv[c];
       const VT rs = (VT)(rv[c] - alpha * qv[c]);
       ro[c] = (double)rs;
-      const double zi = ro[c] * mv[c];
-      acc += ro[c] * zi;
-      azq += zi * qv[c];
+    }
+    if (NB == 3 && bm) {  // z = B^-1 r of the node (the block inverse's products)
+      const double* a = bm + 6 * (int64_t)J;
+      const double z[3] = {a[0] * ro[0] + a[3] * ro[1] + a[4] * ro[2],
+                           a[3] * ro[0] + a[1] * ro[1] + a[5] * ro[2],
+                           a[4] * ro[0] + a[5] * ro[1] + a[2] * ro[2]};
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        acc += ro[c] * z[c];
+        azq += z[c] * qv[c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        const double zi = ro[c] * mv[c];
+        acc += ro[c] * zi;
+        azq += zi * qv[c];
+      }
     }
     double qo[NB];
 #pragma unroll
@@ -225,7 +244,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_update_nodes(int nn, MergeArgs
       r[i0 + c] = (VT)ro[c];
     }
   }
-  if (jacobi) {
+  if (jacobi || bm) {
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) part_rz[blockIdx.x] = s;
     if (part_zq) {
@@ -281,7 +300,7 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
   // FP32 vectors: the first kPre grid-stride groups of r, m and p are loaded
   // before beta's block-wide sums, so that their latency overlaps
   constexpr int kPre = 4;
-  const int n4 = r32 ? (n >> 2) : 0;
+  const int n4 = (r32 && !bm) ? (n >> 2) : 0;  // (the node-block path loads per node)
   const int stride = gridDim.x * kThreads;
   int k = blockIdx.x * kThreads + threadIdx.x;
   float4 rf[kPre], mf[kPre], pf[kPre];
@@ -295,7 +314,26 @@ __global__ __launch_bounds__(kThreads) void k_pcg_pupdate(int n, const double* _
   const double di = grid_sum(part_rz, sh);
   const double beta = part_zq ? -grid_sum(part_zq, sh) / sc->pq : di / *dold;
   double apr = 0.0;
-  if (r32) {
+  if (r32 && bm) {
+    // FP32 vectors with the 3x3 node-block inverse (3-dof layouts, n = 3 nn):
+    // one lane per node, p = fl32(B^-1 r + beta p) in FP64 arithmetic, and
+    // p.r of the stored values
+    const int nn = n / 3;
+    for (int J = blockIdx.x * kThreads + threadIdx.x; J < nn; J += gridDim.x * kThreads) {
+      const int64_t i0 = 3 * (int64_t)J;
+      const double r0 = (double)r32[i0], r1 = (double)r32[i0 + 1], r2 = (double)r32[i0 + 2];
+      const double* a = bm + 6 * (int64_t)J;
+      const double z[3] = {a[0] * r0 + a[3] * r1 + a[4] * r2, a[3] * r0 + a[1] * r1 + a[5] * r2,
+                           a[4] * r0 + a[5] * r1 + a[2] * r2};
+      const double rv[3] = {r0, r1, r2};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float pi = (float)(z[c] + beta * (double)p32[i0 + c]);
+        p32[i0 + c] = pi;
+        apr += (double)pi * rv[c];
+      }
+    }
+  } else if (r32) {
     // the iteration's vectors are FP32 (Jacobi): p = fl32(m r + beta p), in
     // FP64 arithmetic, and p.r of the stored values; four rows per lane
     while (k < n4) {
@@ -403,11 +441,36 @@ __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
                                                          const float* __restrict__ p32,
                                                          float* r32, double* part_pr,
                                                          double* x, float* dx32,
-                                                         const SolveScalars* sc) {
+                                                         const SolveScalars* sc,
+                                                         const double* __restrict__ bm) {
   __shared__ double sh[8];
   if (!block_flag(&sc->upd, sh) || block_flag(&sc->done, sh)) return;
   double acc = 0.0, apr = 0.0;
-  GRID_LOOP(i, n) {
+  if (r32 && bm) {  // FP32 vectors, 3x3 node-block inverse: r.(B^-1 r) per node (n = 3 nn)
+    for (int J = blockIdx.x * kThreads + threadIdx.x; J < n / 3; J += gridDim.x * kThreads) {
+      double rv[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int i = 3 * J + c;
+        if (dx32) {
+          x[i] = x[i] + (double)dx32[i];
+          dx32[i] = 0.0f;
+        }
+        double s = t[i];
+        s = merge_row(mg, i, s);
+        rv[c] = b[i] - s;
+        const float rs = (float)rv[c];
+        r32[i] = rs;
+        if (part_pr) apr += (double)p32[i] * (double)rs;
+      }
+      const double* a = bm + 6 * (int64_t)J;
+      acc += rv[0] * (a[0] * rv[0] + a[3] * rv[1] + a[4] * rv[2]) +
+             rv[1] * (a[3] * rv[0] + a[1] * rv[1] + a[5] * rv[2]) +
+             rv[2] * (a[4] * rv[0] + a[5] * rv[1] + a[2] * rv[2]);
+    }
+  }
+  for (int i = (r32 && bm) ? n : blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
     if (dx32) {
       x[i] = x[i] + (double)dx32[i];
       dx32[i] = 0.0f;
@@ -425,7 +488,7 @@ __global__ __launch_bounds__(kThreads) void k_rel_resid(int n, MergeArgs mg,
       if (part_pr) apr += p[i] * ri;
     }
   }
-  if (jacobi) {
+  if (jacobi || (r32 && bm)) {
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
   }
@@ -508,9 +571,12 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   // Jacobi scale in FP32 (x, the true residuals and all sums stay FP64).  If
   // the progress guard hands over to the FP64 operator, p and r are copied to
   // their FP64 vectors and the iteration continues there.
-  bool v32 = p32 != nullptr && jac && reliable && epart_f32(c, true);
+  // With the 3x3 node-block inverse (3-dof layouts) the FP32 iteration applies
+  // it per node inside the update, the p update and the replacement pass.
+  const bool bm3 = !jac && c->layout_nb == 3 && tdots;
+  bool v32 = p32 != nullptr && (jac || bm3) && reliable && epart_f32(c, true);
   float* r32 = v32 ? p32 + vec32_stride(n) : nullptr;
-  float* m32 = v32 ? p32 + 2 * vec32_stride(n) : nullptr;
+  float* m32 = (v32 && jac) ? p32 + 2 * vec32_stride(n) : nullptr;
   // On large systems the FP32 iteration also keeps its steps of x as FP32
   // increments dx32 (x = V.x + dx32): the update then moves 4 bytes of x per
   // row instead of 8 each way.  The increments since the last replacement
@@ -518,7 +584,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
   // and every replacement forms its product on V.x + dx32 and folds dx32
   // into V.x, so x, the true residuals and the stop test stay FP64.
   float* dx32 = (v32 && n >= kPredictRows) ? p32 + 3 * vec32_stride(n) : nullptr;
-  if (v32) launch_cast<double, float>(n, V.m, m32, s);
+  if (v32 && jac) launch_cast<double, float>(n, V.m, m32, s);
   hipLaunchKernelGGL(k_pcg_init, G, TB, 0, s, n, V.b, V.m, V.bm, V.x, V.r, V.p, part_a, pr_next,
                      p32, r32, dx32);
   hipLaunchKernelGGL(k_pcg_init2, dim3(1), TB, 0, s, part_a, tol, delta, reliable ? 1 : 0, dslots,
@@ -529,8 +595,8 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
     hipLaunchKernelGGL(k_rel_resid, G, TB, 0, s, n, merge_of(c, V.epart),
                        V.t, V.b, V.m, V.r, jac, part_b, (const double*)V.p,
                        (const float*)(v32 ? p32 : nullptr), v32 ? r32 : nullptr, pr_next, V.x,
-                       dx32, sc);
-    if (!jac)
+                       dx32, sc, (const double*)(v32 && !jac ? V.bm : nullptr));
+    if (!jac && !v32)
       hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)nullptr,
                          (double*)nullptr, &sc->upd, sc);
     hipLaunchKernelGGL(k_rel_finish, dim3(1), TB, 0, s, part_b, dnext, sc);
@@ -575,7 +641,9 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
       const bool timed = stride > 0 && it % stride == 0;
       if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j], s));
       if (tdots) {
-        const int keep_q = (!jac && reliable) ? 1 : 0;
+        // q is read again only by k_rz's (M^-1 r).q (FP64 vectors with the
+        // block preconditioner)
+        const int keep_q = (!jac && reliable && !v32) ? 1 : 0;
         const bool e32 = epart_f32(c, op32);
         // q in FP32 when the product sums in FP32 and nothing reads q afterwards
         const bool q32 = e32 && !keep_q;
@@ -583,26 +651,27 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
         if (timed) ELASTIC_HIP_CHECK(c, hipEventRecord(c->tev[2 * j + 1], s));
         // q is read again only by the block preconditioner's (M^-1 r).q
         const MergeArgs mg = merge_of(c, V.epart, e32);
-#define ELASTIC_UPD(NB, EP, QT, VT, P, R, M)                                                   \
+        const double* bmv = (v32 && !jac) ? V.bm : nullptr;  // node-block inverse, FP32 vectors
+#define ELASTIC_UPD(NB, EP, QT, VT, P, R, M, BM)                                               \
   hipLaunchKernelGGL((k_pcg_update_nodes<NB, EP, QT, VT>), G, TB, 0, s, n / NB, mg,            \
                      (const double*)part_pr, dcur, V.x, (const VT*)(P), (VT*)(R), V.q, keep_q,  \
-                     (const VT*)(M), jac, part_b, part_zq, sc)
+                     (const VT*)(M), jac, part_b, part_zq, sc, (const double*)(BM))
 #define ELASTIC_UPD_DX(NB)                                                                     \
   hipLaunchKernelGGL((k_pcg_update_nodes<NB, float, float, float, float>), G, TB, 0, s, n / NB, \
                      mg, (const double*)part_pr, dcur, dx32, (const float*)p32, r32, V.q,      \
-                     keep_q, (const float*)m32, jac, part_b, part_zq, sc)
+                     keep_q, (const float*)m32, jac, part_b, part_zq, sc, bmv)
         if (c->layout_nb == 3) {
           if (q32 && v32 && dx32) ELASTIC_UPD_DX(3);
-          else if (q32 && v32) ELASTIC_UPD(3, float, float, float, p32, r32, m32);
-          else if (q32) ELASTIC_UPD(3, float, float, double, V.p, V.r, V.m);
-          else if (e32) ELASTIC_UPD(3, float, double, double, V.p, V.r, V.m);
-          else ELASTIC_UPD(3, double, double, double, V.p, V.r, V.m);
+          else if (q32 && v32) ELASTIC_UPD(3, float, float, float, p32, r32, m32, bmv);
+          else if (q32) ELASTIC_UPD(3, float, float, double, V.p, V.r, V.m, nullptr);
+          else if (e32) ELASTIC_UPD(3, float, double, double, V.p, V.r, V.m, nullptr);
+          else ELASTIC_UPD(3, double, double, double, V.p, V.r, V.m, nullptr);
         } else {
           if (q32 && v32 && dx32) ELASTIC_UPD_DX(2);
-          else if (q32 && v32) ELASTIC_UPD(2, float, float, float, p32, r32, m32);
-          else if (q32) ELASTIC_UPD(2, float, float, double, V.p, V.r, V.m);
-          else if (e32) ELASTIC_UPD(2, float, double, double, V.p, V.r, V.m);
-          else ELASTIC_UPD(2, double, double, double, V.p, V.r, V.m);
+          else if (q32 && v32) ELASTIC_UPD(2, float, float, float, p32, r32, m32, nullptr);
+          else if (q32) ELASTIC_UPD(2, float, float, double, V.p, V.r, V.m, nullptr);
+          else if (e32) ELASTIC_UPD(2, float, double, double, V.p, V.r, V.m, nullptr);
+          else ELASTIC_UPD(2, double, double, double, V.p, V.r, V.m, nullptr);
         }
 #undef ELASTIC_UPD
 #undef ELASTIC_UPD_DX
@@ -615,7 +684,7 @@ int pcg_run(Context* c, const PcgVecs& V, bool fp32_operator, double tol, int ma
                            (const double*)part_pr, dcur, V.x, V.p, V.r, (const double*)V.q, V.m,
                            jac, part_b, part_zq, sc);
       }
-      if (!jac)
+      if (!jac && !(v32 && tdots))
         hipLaunchKernelGGL(k_rz, G, TB, 0, s, n, V.r, V.m, V.bm, part_b, (const double*)V.q,
                            part_zq, (const int32_t*)nullptr, sc);
       hipLaunchKernelGGL(k_pcg_pupdate, G, TB, 0, s, n, part_b, (const double*)part_zq, dcur, dnxt,

@@ -328,3 +328,52 @@ def test_rows_not_a_multiple_of_four(accum32):
     xm, stm = c.solve(s.b, precision=MIXED, tol=1e-10)
     assert stm["norm_m_r"] < stm["conv"] and rel(xm, x64) <= 1e-8
     c.close()
+
+
+@pytest.mark.parametrize("name", ["hex8_5j", "tet4_3j"])
+@pytest.mark.parametrize("order", [0, 1])
+def test_block_jacobi_fp32_vectors(golden, name, order):
+    """The mixed PCG with FP32 sums keeps its vectors in FP32 with the 3x3
+    node-block inverse too (applied per node in the update, the p update and
+    the replacement pass, Precond.f:419-540): x within 1e-8 of the
+    reference's, in both node orders, and the iteration count within 15% of
+    the mixed block-Jacobi PCG that keeps FP64 vectors (FP64 sums).  The FP32
+    values themselves cost iterations against FP64 PCG on these small meshes
+    (tet4_3j: 168 against 121), in either form."""
+    from elastic_amd.solver import BDIAG3, Context
+
+    g = golden(name)
+    with Context(0) as c:
+        c.set_option("node_order", order)
+        load_system(c, g)
+        c.set_option("accum32", 0)
+        x64, s64 = c.solve(g["b"], precond=BDIAG3, precision=32, tol=1e-12)
+        c.set_option("accum32", 1)
+        x, st = c.solve(g["b"], precond=BDIAG3, precision=32, tol=1e-12)
+        print("bdiag fp32 vectors", name, order, st["iters"], s64["iters"], flush=True)
+        assert st["status"] == 0 and rel(x, g["x"]) <= 1e-8
+        assert st["iters"] <= 1.15 * s64["iters"] + 3
+
+
+def test_block_jacobi_fp32_vectors_large():
+    """hex8 40^3 (201,720 equations: FP32 increments of x, gated replacement
+    passes, compact node order): the mixed block-Jacobi PCG with FP32 vectors
+    lands within 1e-8 of the FP64 block-Jacobi solution at 1e-12, and alone
+    and in a second run bit-identical."""
+    from elastic_amd import mesh as M
+    from elastic_amd.solver import BDIAG3, FP64, MIXED
+    from elastic_amd.system import System
+
+    s = System(M.hex8_cube(40, jitter=0.15, seed=M.BASE_SEED + 2))
+    c = s.ctx
+    c.upload_rhs(s.b)
+    c.solve_resident(precond=BDIAG3, precision=FP64, tol=1e-12, maxit=100000)
+    xref = c.download_x()
+    st = c.solve_resident(precond=BDIAG3, precision=MIXED, tol=1e-10, maxit=100000)
+    x = c.download_x()
+    st2 = c.solve_resident(precond=BDIAG3, precision=MIXED, tol=1e-10, maxit=100000)
+    x2 = c.download_x()
+    print("hex8_40 bdiag fp32 vectors", st["iters"], st["outer"], rel(x, xref), flush=True)
+    assert st["status"] == 0 and rel(x, xref) <= 1e-8
+    assert np.array_equal(x, x2) and st["iters"] == st2["iters"]
+    c.close()
