@@ -245,7 +245,8 @@ static int build_layout(elastic_ctx* x) {
       rc = rc ? rc : dalloc(c, &c->xd, (size_t)neq);
       rc = rc ? rc : dalloc(c, &c->bd, (size_t)neq);
     }
-    const size_t nvals = (size_t)L.nblocks * nb * nb, ndiag = (size_t)L.nn * (nb * (nb + 1) / 2);
+    // one padding block: a tile without blocks still loads in bounds
+    const size_t nvals = ((size_t)L.nblocks + 1) * nb * nb, ndiag = (size_t)L.nn * (nb * (nb + 1) / 2);
     rc = rc ? rc : dalloc(c, &b.v64, nvals);
     rc = rc ? rc : dalloc(c, &b.v32, nvals);
     rc = rc ? rc : dalloc(c, &b.d64, ndiag);

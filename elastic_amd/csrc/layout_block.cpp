@@ -262,7 +262,7 @@ void build_block_tiles(int neq, int nb, const int32_t* ia1, const int32_t* ja1, 
   L.ext0.assign(nt, 0);
   L.seg0.assign(nt, 0);
   L.rp0.assign(nt, 0);
-  L.word.assign((size_t)b0, 0u);
+  L.word.assign((size_t)b0 + 1, 0u);  // one padding word: tiles load in bounds
   L.src_row.assign((size_t)b0, 0);
   L.src_k.assign((size_t)b0, 0);
   std::vector<std::vector<int32_t>> ext(nt);

@@ -67,18 +67,33 @@ struct Planes<float, NB> {
   }
 };
 
-// Registers of one tile: the words and values of the lane's PER blocks and,
-// for lanes below the tile's row count, the diagonal block of that row node.
+// Staged x entries a lane gathers into registers (entries tid + NT k, k <
+// kXR); a tile's entries beyond kXR * NT are gathered by a loop of their own.
+constexpr int kXR = 3;
+
+// Registers of one tile: the words and values of the lane's PER blocks (valid
+// for blocks below the tile's block count) and, for lanes below the tile's
+// row count, the diagonal block of that row node; gx (loaded with GX only):
+// the node of each x entry the lane stages, where that entry is an external
+// column.
 template <typename TV, int NB, int PER>
 struct TileRegs {
   uint32_t wr[PER];
   TV vr[PER][NB * NB];
   TV dg[NB * (NB + 1) / 2];
+  int32_t gx[kXR];
 };
 
 // Issues the loads of tile t (values nontemporal: read once per product).
 // NT: lanes of the workgroup (256 or 512); lane tid owns blocks tid + NT i.
-template <typename TV, int NB, int PER, int NT = 256>
+// GX: the nodes of the x entries the lane stages are loaded first: they
+// return while the value loads are still in flight, so compute_tile can issue
+// the x gathers behind the values without waiting for them.  Every load is
+// then issued by every lane, from a clamped index where the lane has nothing
+// to load (compute_tile does not use such a result), so the number of loads
+// in flight is the same on every path and each use waits only for the loads
+// it needs.
+template <typename TV, int NB, int PER, int NT = 256, bool GX = false>
 __device__ __forceinline__ void load_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
                                           const TV* __restrict__ diag, TileRegs<TV, NB, PER>& R) {
   constexpr int NB2 = NB * NB;
@@ -86,22 +101,47 @@ __device__ __forceinline__ void load_tile(const BlkArgs& a, int t, const TV* __r
   const int nbk = a.nblk[t], nr = a.nrows[t];
   const int64_t base = a.blk0[t];
   const TV* v = vals + base * NB2;
+  if constexpr (GX) {
+    const int nx = a.ncols[t] * NB;
+    const int32_t* ext = a.ext + a.ext0[t];
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int q = tid + NT * i;
-    if (q < nbk) {
-      R.wr[i] = __builtin_nontemporal_load(a.word + base + q);
-      Planes<TV, NB>::load(v, nbk, q, R.vr[i]);
-    } else {
-      R.wr[i] = 0xFFFFFFFFu;
-#pragma unroll
-      for (int p = 0; p < NB2; ++p) R.vr[i][p] = TV(0);
+    for (int k = 0; k < kXR; ++k) {
+      // lanes past the end take the last entry again (in bounds, not stored)
+      const int node = min(tid + NT * k, nx - 1) / NB;
+      const int e = node - nr;  // external column, or negative for an own row
+      // own rows (node I0 + node, formed where it is used) load a word that
+      // is not used, so that nothing waits for these loads here
+      R.gx[k] = *(e >= 0 ? ext + e : a.row0 + t);
     }
-  }
-  if (tid < nr) {
-    const int g = a.row0[t] + tid;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      // blocks q >= nbk: not used; a tile without blocks reads the padding
+      // block that follows the last tile's
+      const int qc = min(tid + NT * i, max(nbk, 1) - 1);
+      R.wr[i] = __builtin_nontemporal_load(a.word + base + qc);
+      Planes<TV, NB>::load(v, max(nbk, 1), qc, R.vr[i]);
+    }
+    const int g = a.row0[t] + min(tid, nr - 1);  // used for tid < nr only
 #pragma unroll
     for (int k = 0; k < NB * (NB + 1) / 2; ++k) R.dg[k] = diag[(int64_t)k * a.nn + g];
+  } else {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int q = tid + NT * i;
+      if (q < nbk) {
+        R.wr[i] = __builtin_nontemporal_load(a.word + base + q);
+        Planes<TV, NB>::load(v, nbk, q, R.vr[i]);
+      } else {
+        R.wr[i] = 0xFFFFFFFFu;
+#pragma unroll
+        for (int p = 0; p < NB2; ++p) R.vr[i][p] = TV(0);
+      }
+    }
+    if (tid < nr) {
+      const int g = a.row0[t] + tid;
+#pragma unroll
+      for (int k = 0; k < NB * (NB + 1) / 2; ++k) R.dg[k] = diag[(int64_t)k * a.nn + g];
+    }
   }
 }
 
@@ -138,7 +178,7 @@ __device__ __forceinline__ void tile_dot_store(double* part, int t, double v) {
 // are added in tile order by a kernel of their own (k_sum_shares), so x.(A x)
 // needs no further pass over y and no floating-point atomics.
 template <typename TV, typename T, int NB, int PER, bool DOT = false, typename Hook = NoHook,
-          int NT = 256, bool XD = false>
+          int NT = 256, bool XD = false, bool GX = false>
 __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
                                              const TileRegs<TV, NB, PER>& R,
                                              const double* __restrict__ x, double* __restrict__ y,
@@ -157,34 +197,69 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   const int32_t* ext = a.ext + ext0;
   const uint32_t* wr = R.wr;
   const auto& vr = R.vr;
-  // x of the tile's local columns: the first 8 * NT entries are loaded into
+  // x of the tile's local columns: the first entries are loaded into
   // registers, then `hook` runs (a caller may issue more loads there, which
-  // stay in flight behind these), then they go to LDS.  XD (FP64 staging
-  // only): x is held as x + dx, an FP64 base plus FP32 increments, and the
-  // product is formed on their FP64 sum (a separate instance, so the other
-  // products keep their registers)
-  constexpr int XR = 8;
+  // stay in flight behind these), then they go to LDS; x32: an FP32 copy of x
+  // (FP32 staging takes the same values from it).  XD (FP64 staging only): x
+  // is held as x + dx, an FP64 base plus FP32 increments, and the product is
+  // formed on their FP64 sum (a separate instance, so the other products keep
+  // their registers)
   const int nx = ncols * NB;
-  double xv[XR];
+  int xdone;  // entries staged from registers
+  if constexpr (GX) {
+    // kXR * NT entries, from the nodes load_tile fetched; all of a lane's
+    // loads are issued before any is used
+    auto stage = [&](auto src) {
+      using S = typename std::remove_const<typename std::remove_pointer<decltype(src)>::type>::type;
+      S xv[kXR];
+      float dxv[kXR];
 #pragma unroll
-  for (int k = 0; k < XR; ++k) {
-    const int c = tid + NT * k;
-    if (c < nx) {
-      const int node = c / NB, comp = c - node * NB;
-      const int g = node < nr ? I0 + node : ext[node - nr];
-      // x32: an FP32 copy of x (FP32 staging takes the same values from it)
-      xv[k] = x32 ? (double)x32[(int64_t)NB * g + comp] : x[(int64_t)NB * g + comp];
-      if constexpr (XD) xv[k] += (double)dx[(int64_t)NB * g + comp];
+      for (int k = 0; k < kXR; ++k) {
+        const int c = min(tid + NT * k, nx - 1);
+        const int node = c / NB;
+        const int g = node < nr ? I0 + node : R.gx[k];
+        const int64_t i = (int64_t)NB * g + (c - node * NB);
+        xv[k] = src[i];
+        if constexpr (XD) dxv[k] = dx[i];
+      }
+      hook();
+      __syncthreads();  // the LDS of a previous tile is free
+#pragma unroll
+      for (int k = 0; k < kXR; ++k) {
+        const int c = tid + NT * k;
+        if (c < nx) {
+          double v = (double)xv[k];
+          if constexpr (XD) v += (double)dxv[k];
+          xs[c] = (T)v;
+        }
+      }
+    };
+    if (x32) stage(x32);
+    else stage(x);
+    xdone = kXR * NT;
+  } else {
+    constexpr int XR = 8;
+    double xv[XR];
+#pragma unroll
+    for (int k = 0; k < XR; ++k) {
+      const int c = tid + NT * k;
+      if (c < nx) {
+        const int node = c / NB, comp = c - node * NB;
+        const int g = node < nr ? I0 + node : ext[node - nr];
+        xv[k] = x32 ? (double)x32[(int64_t)NB * g + comp] : x[(int64_t)NB * g + comp];
+        if constexpr (XD) xv[k] += (double)dx[(int64_t)NB * g + comp];
+      }
     }
-  }
-  hook();
-  __syncthreads();  // the LDS of a previous tile is free
+    hook();
+    __syncthreads();  // the LDS of a previous tile is free
 #pragma unroll
-  for (int k = 0; k < XR; ++k) {
-    const int c = tid + NT * k;
-    if (c < nx) xs[c] = (T)xv[k];
+    for (int k = 0; k < XR; ++k) {
+      const int c = tid + NT * k;
+      if (c < nx) xs[c] = (T)xv[k];
+    }
+    xdone = XR * NT;
   }
-  for (int c = tid + NT * XR; c < nx; c += NT) {
+  for (int c = tid + xdone; c < nx; c += NT) {
     const int node = c / NB, comp = c - node * NB;
     const int g = node < nr ? I0 + node : ext[node - nr];
     double v = x32 ? (double)x32[(int64_t)NB * g + comp] : x[(int64_t)NB * g + comp];
@@ -321,7 +396,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
 
 // One tile: its loads, then its product.
 template <typename TV, typename T, int NB, int PER, bool DOT = false, int NT = 256,
-          bool XD = false>
+          bool XD = false, bool GX = false>
 __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
                                            const TV* __restrict__ diag,
                                            const double* __restrict__ x, double* __restrict__ y,
@@ -330,9 +405,9 @@ __device__ __forceinline__ void block_tile(const BlkArgs& a, int t, const TV* __
                                            const float* __restrict__ x32 = nullptr,
                                            const float* __restrict__ dx = nullptr) {
   TileRegs<TV, NB, PER> R;
-  load_tile<TV, NB, PER, NT>(a, t, vals, diag, R);
-  compute_tile<TV, T, NB, PER, DOT, NoHook, NT, XD>(a, t, R, x, y, epart, smem, tdot, y32,
-                                                    x32, NoHook(), dx);
+  load_tile<TV, NB, PER, NT, GX>(a, t, vals, diag, R);
+  compute_tile<TV, T, NB, PER, DOT, NoHook, NT, XD, GX>(a, t, R, x, y, epart, smem, tdot, y32,
+                                                        x32, NoHook(), dx);
 }
 
 }  // namespace elastic

@@ -6,21 +6,21 @@
 namespace elastic {
 
 // DOT: also the tiles' shares of x.(A x) into tdot; NT lanes per tile; XD:
-// the product of x + dx (FP32 increments dx)
+// the product of x + dx (FP32 increments dx).  FP32 values and sums: the
+// register budget is held to eight waves per SIMD, the occupancy the tile's
+// LDS allows.
 template <typename TV, typename T, int NB, int PER, bool DOT, int NT, bool XD = false>
-__global__ __launch_bounds__(NT) void k_spmv_blocks(BlkArgs a, const TV* __restrict__ vals,
-                                                      const TV* __restrict__ diag,
-                                                      const double* __restrict__ x,
-                                                      double* __restrict__ y,
-                                                      double* __restrict__ epart,
-                                                      const int32_t* __restrict__ skip,
-                                                      double* __restrict__ tdot, int y32,
-                                                      const float* __restrict__ x32,
-                                                      const float* __restrict__ dx) {
+__global__ __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu(sizeof(TV) == 4 && sizeof(T) == 4 ? 8 : 1)))
+void k_spmv_blocks(BlkArgs a, const TV* __restrict__ vals, const TV* __restrict__ diag,
+                   const double* __restrict__ x, double* __restrict__ y,
+                   double* __restrict__ epart, const int32_t* __restrict__ skip,
+                   double* __restrict__ tdot, int y32, const float* __restrict__ x32,
+                   const float* __restrict__ dx) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (skip != nullptr && *skip != 0) return;
-  block_tile<TV, T, NB, PER, DOT, NT, XD>(a, blockIdx.x, vals, diag, x, y, epart, smem, tdot,
-                                          y32, x32, dx);
+  block_tile<TV, T, NB, PER, DOT, NT, XD, true>(a, blockIdx.x, vals, diag, x, y, epart, smem,
+                                                tdot, y32, x32, dx);
 }
 
 // *out = the n shares added in a fixed order (one workgroup of 1024 lanes):
