@@ -7,9 +7,9 @@
 // it loads their nb*nb values (one coalesced plane at a time) and the block
 // word, then forms A_IJ x_J for its row node and A_IJ^T x_I for its column
 // node from the LDS-staged x.  Row pieces are summed across the wavefront by a
-// segmented scan on the row key, transposed products land in LDS at their
-// column-sorted position and each local column node is summed by one lane in a
-// fixed order: deterministic, no atomics.  Values are TV (double, or float for
+// segmented scan on the row key (DPP lane moves), transposed products land in
+// LDS at their column-sorted position and each local column node is summed by
+// one lane in a fixed order: deterministic, no atomics.  Values are TV (double, or float for
 // the mixed-precision operator); staged x and sums are T.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -38,11 +38,29 @@ inline BlkArgs blk_args(const DevBlocks& b) {
 // lane loads both with one 8-byte load (the last odd plane stays single).
 template <typename TV, int NB>
 struct Planes;
+// A uniform pointer to global memory held in scalar registers: loads through
+// it take the lane's 32-bit offset in one vector register, whichever plane
+// they read.
+template <typename P>
+using global_ptr = const __attribute__((address_space(1))) P*;
+template <typename P>
+__device__ __forceinline__ global_ptr<P> scalar_ptr(const P* p) {
+  const uint64_t u = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (global_ptr<P>)(((uint64_t)hi << 32) | lo);
+}
 template <int NB>
 struct Planes<double, NB> {
   __device__ static inline void load(const double* v, int nbk, int q, double* out) {
 #pragma unroll
     for (int p = 0; p < NB * NB; ++p) out[p] = __builtin_nontemporal_load(v + (int64_t)p * nbk + q);
+  }
+  // the same loads, v uniform and q >= 0: plane bases in scalar registers
+  __device__ static inline void load_s(const double* v, int nbk, unsigned q, double* out) {
+#pragma unroll
+    for (int p = 0; p < NB * NB; ++p)
+      out[p] = __builtin_nontemporal_load(scalar_ptr(v + (int64_t)p * nbk) + q);
   }
   __device__ __host__ static inline int64_t index(int nbk, int q, int p) { return (int64_t)p * nbk + q; }
 };
@@ -60,12 +78,68 @@ struct Planes<float, NB> {
     }
     if (NB * NB % 2) out[NB * NB - 1] = __builtin_nontemporal_load(v + (int64_t)2 * NP * nbk + q);
   }
+  __device__ static inline void load_s(const float* v, int nbk, unsigned q, float* out) {
+    constexpr int NP = NB * NB / 2;
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2* v2 = reinterpret_cast<const f2*>(v);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const f2 f = __builtin_nontemporal_load(scalar_ptr(v2 + (int64_t)k * nbk) + q);
+      out[2 * k] = f.x;
+      out[2 * k + 1] = f.y;
+    }
+    if (NB * NB % 2)
+      out[NB * NB - 1] = __builtin_nontemporal_load(scalar_ptr(v + (int64_t)2 * NP * nbk) + q);
+  }
   __device__ __host__ static inline int64_t index(int nbk, int q, int p) {
     constexpr int NP = NB * NB / 2;
     if (p < 2 * NP) return (int64_t)2 * ((int64_t)(p / 2) * nbk + q) + (p & 1);
     return (int64_t)2 * NP * nbk + q;
   }
 };
+
+// Segmented inclusive scan of y over the wavefront, segments = runs of equal
+// keys, with the keys ascending along the lanes (a tile's blocks are sorted by
+// row node).  DPP moves on the VALU instead of LDS permutes: shifts by 1, 2,
+// 4 and 8 lanes inside each row of 16 lanes, then the last lane of rows 0 and
+// 2 added to rows 1 and 3, then lane 31 added to rows 2 and 3.  With ascending
+// keys a lane whose key equals the source lane's key has every lane between
+// them in its run, so each step adds a run's partial sum exactly once.
+template <int CTRL, int RMASK>
+__device__ __forceinline__ int dpp_move(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, RMASK, 0xF, false);
+}
+template <int CTRL, int RMASK>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(dpp_move<CTRL, RMASK>(0, __float_as_int(v)));
+}
+template <int CTRL, int RMASK>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_move<CTRL, RMASK>(0, (int)(b & 0xFFFFFFFFll));
+  const int hi = dpp_move<CTRL, RMASK>(0, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// one step: lanes without a source (outside the row, or rows the mask leaves
+// out) read key -1, which matches no key
+template <int CTRL, int RMASK, typename T, int NB>
+__device__ __forceinline__ void seg_step(int key, T (&y)[NB]) {
+  const int ok = dpp_move<CTRL, RMASK>(-1, key);
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    const T o = dpp_move<CTRL, RMASK>(y[r]);
+    if (ok == key) y[r] += o;
+  }
+}
+template <typename T, int NB>
+__device__ __forceinline__ void seg_scan(int key, T (&y)[NB]) {
+  seg_step<0x111, 0xF>(key, y);  // row_shr:1
+  seg_step<0x112, 0xF>(key, y);  // row_shr:2
+  seg_step<0x114, 0xF>(key, y);  // row_shr:4
+  seg_step<0x118, 0xF>(key, y);  // row_shr:8
+  seg_step<0x142, 0xA>(key, y);  // row_bcast:15 into rows 1 and 3
+  seg_step<0x143, 0xC>(key, y);  // row_bcast:31 into rows 2 and 3
+}
 
 // Staged x entries a lane gathers into registers (entries tid + NT k, k <
 // kXR); a tile's entries beyond kXR * NT are gathered by a loop of their own.
@@ -117,13 +191,13 @@ __device__ __forceinline__ void load_tile(const BlkArgs& a, int t, const TV* __r
     for (int i = 0; i < PER; ++i) {
       // blocks q >= nbk: not used; a tile without blocks reads the padding
       // block that follows the last tile's
-      const int qc = min(tid + NT * i, max(nbk, 1) - 1);
+      const unsigned qc = min((unsigned)(tid + NT * i), (unsigned)(max(nbk, 1) - 1));
       R.wr[i] = __builtin_nontemporal_load(a.word + base + qc);
-      Planes<TV, NB>::load(v, max(nbk, 1), qc, R.vr[i]);
+      Planes<TV, NB>::load_s(v, max(nbk, 1), qc, R.vr[i]);
     }
-    const int g = a.row0[t] + min(tid, nr - 1);  // used for tid < nr only
+    const unsigned g = a.row0[t] + min(tid, nr - 1);  // used for tid < nr only
 #pragma unroll
-    for (int k = 0; k < NB * (NB + 1) / 2; ++k) R.dg[k] = diag[(int64_t)k * a.nn + g];
+    for (int k = 0; k < NB * (NB + 1) / 2; ++k) R.dg[k] = scalar_ptr(diag + (int64_t)k * a.nn)[g];
   } else {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -268,29 +342,31 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   }
   __syncthreads();
   const uint16_t* rp = a.rowptr + a.rp0[t];
+  // the products of all the lane's blocks first (their values are then free),
+  // then the row scans, which are independent of each other
+  T yl[PER][NB];
+  int key[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const int q = tid + NT * i;
-    const bool valid = q < nbk;
+    const bool valid = tid + NT * i < nbk;
     const uint32_t w = wr[i];
-    const int key = valid ? (int)(w >> 25) : 255;
+    key[i] = valid ? (int)(w >> 25) : 255;
     const int lc = (w >> 10) & 0x7FFF, tp = w & 0x3FF;
-    T yl[NB];
 #pragma unroll
-    for (int r = 0; r < NB; ++r) yl[r] = T(0);
+    for (int r = 0; r < NB; ++r) yl[i][r] = T(0);
     if (valid) {
       T xc[NB], xr[NB];
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
         xc[r] = xs[lc * NB + r];
-        xr[r] = xs[key * NB + r];
+        xr[r] = xs[key[i] * NB + r];
       }
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
         T s = T(0);
 #pragma unroll
         for (int cc = 0; cc < NB; ++cc) s += (T)vr[i][r * NB + cc] * xc[cc];
-        yl[r] = s;
+        yl[i][r] = s;
       }
 #pragma unroll
       for (int cc = 0; cc < NB; ++cc) {
@@ -300,22 +376,20 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
         tr[tp * NB + cc] = s;
       }
     }
+  }
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int ok = __shfl_up(key, d, 64);
+  for (int i = 0; i < PER; ++i) seg_scan(key[i], yl[i]);
 #pragma unroll
-      for (int r = 0; r < NB; ++r) {
-        const T o = __shfl_up(yl[r], d, 64);
-        if (lane >= d && ok == key) yl[r] += o;
-      }
-    }
+  for (int i = 0; i < PER; ++i) {
+    const int q = tid + NT * i;
+    const bool valid = q < nbk;
     // a node row has at most 64 blocks (checked when the layout is built), so
     // it ends in its own chunk or in the next one: two pieces at most
-    const bool row_end = valid && (q + 1 == (int)rp[key + 1]);
+    const bool row_end = valid && (q + 1 == (int)rp[key[i] + 1]);
     if (valid && (lane == 63 || row_end)) {
-      const int slot = 2 * key + (row_end ? 1 : 0);
+      const int slot = 2 * key[i] + (row_end ? 1 : 0);
 #pragma unroll
-      for (int r = 0; r < NB; ++r) pp[slot * NB + r] = yl[r];
+      for (int r = 0; r < NB; ++r) pp[slot * NB + r] = yl[i][r];
     }
   }
   __syncthreads();
