@@ -795,7 +795,7 @@ static int berr(elastic_batch* b, int code, const std::string& m) {
 
 // LDS of one tile's product with FP64 sums (FP32 sums take half)
 static size_t tile_lds(int nb, int ncols_max, int nblk_max) {
-  return sizeof(double) * nb * ((size_t)ncols_max + 2 * kBlkRowsMax + (size_t)nblk_max);
+  return tile_smem(sizeof(double), nb, ncols_max, nblk_max);
 }
 // the dynamic LDS of variant v's launch: the batch's largest tile with the
 // variant's sum type

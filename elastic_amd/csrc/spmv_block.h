@@ -219,8 +219,18 @@ __device__ __forceinline__ void load_tile(const BlkArgs& a, int t, const TV* __r
   }
 }
 
+// Shared memory of one tile's product with sums of type T (bytes): staged x,
+// row pieces and transposed products (NB values of T per local column, per
+// two pieces of each row, per block), then the tile's row and segment
+// pointers (int32, one T-sized slot each, so the FP32 size is half the FP64
+// size).
+inline size_t tile_smem(size_t t_size, int nb, int ncols_max, int nblk_max) {
+  return t_size * ((size_t)nb * ((size_t)ncols_max + 2 * kBlkRowsMax + (size_t)nblk_max) +
+                   (kBlkRowsMax + 1) + ((size_t)ncols_max + 1));
+}
+
 // The product of tile t from its registers (a workgroup of NT lanes); smem
-// holds sizeof(T) * NB * (ncols_max + 2 * kBlkRowsMax + nblk_max) bytes.  T is
+// holds tile_smem(sizeof(T), NB, ncols_max, nblk_max) bytes.  T is
 // the type of the staged x, the products and the sums (double, or float for
 // the FP32 operator with FP32 accumulation); x, y and the external partials
 // are FP64 in memory either way.  Starts with a barrier, so tiles can follow
@@ -265,6 +275,9 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   T* xs = reinterpret_cast<T*>(smem);       // ncols_max * NB
   T* pp = xs + a.ncols_max * NB;            // 2 * kBlkRowsMax * NB (row pieces)
   T* tr = pp + 2 * kBlkRowsMax * NB;        // nblk_max * NB (transposed products)
+  // the tile's row pointers (nr + 1) and column segment pointers (ncols + 1)
+  int32_t* rps = reinterpret_cast<int32_t*>(tr + a.nblk_max * NB);
+  int32_t* sps = rps + kBlkRowsMax + 1;
   const int tid = threadIdx.x, lane = tid & 63;
   const int I0 = a.row0[t], nr = a.nrows[t], nbk = a.nblk[t], ncols = a.ncols[t];
   const int ext0 = a.ext0[t];
@@ -340,8 +353,13 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
     if constexpr (XD) v += (double)dx[(int64_t)NB * g + comp];
     xs[c] = (T)v;
   }
-  __syncthreads();
+  // the tile's row and segment pointers, kept in LDS for the scans and the
+  // column sums (one wait here instead of one at each use)
   const uint16_t* rp = a.rowptr + a.rp0[t];
+  const uint16_t* sp = a.segptr + a.seg0[t];
+  if (tid <= nr) rps[tid] = rp[tid];  // nr <= kBlkRowsMax < NT
+  for (int c = tid; c <= ncols; c += NT) sps[c] = sp[c];
+  __syncthreads();
   // the products of all the lane's blocks first (their values are then free),
   // then the row scans, which are independent of each other
   T yl[PER][NB];
@@ -385,7 +403,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
     const bool valid = q < nbk;
     // a node row has at most 64 blocks (checked when the layout is built), so
     // it ends in its own chunk or in the next one: two pieces at most
-    const bool row_end = valid && (q + 1 == (int)rp[key[i] + 1]);
+    const bool row_end = valid && (q + 1 == rps[key[i] + 1]);
     if (valid && (lane == 63 || row_end)) {
       const int slot = 2 * key[i] + (row_end ? 1 : 0);
 #pragma unroll
@@ -393,7 +411,6 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
     }
   }
   __syncthreads();
-  const uint16_t* sp = a.segptr + a.seg0[t];
   double dacc = 0.0;  // this lane's share of x.(Ax) (DOT)
   for (int c = tid; c < ncols; c += NT) {
     T s[NB];
@@ -414,7 +431,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
         s[0] = d[0] * xi[0] + d[2] * xi[1];
         s[1] = d[2] * xi[0] + d[1] * xi[1];
       }
-      const int b0 = rp[c], b1 = rp[c + 1];
+      const int b0 = rps[c], b1 = rps[c + 1];
       if (b1 > b0) {
         if ((b0 | 63) < b1 - 1)  // the row crosses a chunk boundary
 #pragma unroll
@@ -426,8 +443,8 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
     {
       // the column's transposed products in sorted order; four entries' LDS
       // reads are issued together, then added in order
-      int q = sp[c];
-      const int qe = sp[c + 1];
+      int q = sps[c];
+      const int qe = sps[c + 1];
       for (; q + 4 <= qe; q += 4) {
         T v[4][NB];
 #pragma unroll

@@ -123,7 +123,7 @@ static void spmv_blocks_launch(const DevBlocks& b, const TV* vals, const TV* dia
                                const double* x, double* y, double* epart, hipStream_t s,
                                const int32_t* skip, const DotOut* dot, int y32,
                                const float* x32, int lanes, const float* dx = nullptr) {
-  const size_t lds = sizeof(TA) * NB * ((size_t)b.ncols_max + 2 * kBlkRowsMax + (size_t)b.nblk_max);
+  const size_t lds = tile_smem(sizeof(TA), NB, b.ncols_max, b.nblk_max);
   const int nt = lanes == 512 && b.nblk_max > 256 ? 512 : 256;
   const int per = (b.nblk_max + nt - 1) / nt;
 #define ELASTIC_BLK_LAUNCH(K, P, D, T, G, ...)                                                  \
