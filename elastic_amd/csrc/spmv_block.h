@@ -239,14 +239,21 @@ struct NoHook {
   __device__ void operator()() const {}
 };
 
-// Block sum of the lanes' shares of x.(A x) into part[t] (fixed order).
+// Block sum of the lanes' shares of x.(A x) into part[t] (fixed order): each
+// wavefront sums by DPP moves (the steps of seg_scan without keys: lanes
+// without a source add zero), so lane 63 holds the wavefront's sum.
 template <int NT = 256>
 __device__ __forceinline__ void tile_dot_store(double* part, int t, double v) {
   __shared__ double red[NT / 64];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v += dpp_move<0x111, 0xF>(v);
+  v += dpp_move<0x112, 0xF>(v);
+  v += dpp_move<0x114, 0xF>(v);
+  v += dpp_move<0x118, 0xF>(v);
+  v += dpp_move<0x142, 0xA>(v);
+  v += dpp_move<0x143, 0xC>(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();  // red of a previous tile in this workgroup is free
-  if (lane == 0) red[wid] = v;
+  if (lane == 63) red[wid] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = (red[0] + red[1]) + (red[2] + red[3]);
@@ -282,6 +289,8 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   const int I0 = a.row0[t], nr = a.nrows[t], nbk = a.nblk[t], ncols = a.ncols[t];
   const int ext0 = a.ext0[t];
   const int32_t* ext = a.ext + ext0;
+  const uint16_t* rp = a.rowptr + a.rp0[t];
+  const uint16_t* sp = a.segptr + a.seg0[t];
   const uint32_t* wr = R.wr;
   const auto& vr = R.vr;
   // x of the tile's local columns: the first entries are loaded into
@@ -355,10 +364,13 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
   }
   // the tile's row and segment pointers, kept in LDS for the scans and the
   // column sums (one wait here instead of one at each use)
-  const uint16_t* rp = a.rowptr + a.rp0[t];
-  const uint16_t* sp = a.segptr + a.seg0[t];
-  if (tid <= nr) rps[tid] = rp[tid];  // nr <= kBlkRowsMax < NT
-  for (int c = tid; c <= ncols; c += NT) sps[c] = sp[c];
+  {
+    // both loads issued before either is used (clamped indices: in bounds)
+    const int rv = rp[min(tid, nr)], sv = sp[min(tid, ncols)];
+    if (tid <= nr) rps[tid] = rv;  // nr <= kBlkRowsMax < NT
+    if (tid <= ncols) sps[tid] = sv;
+    for (int c = tid + NT; c <= ncols; c += NT) sps[c] = sp[c];
+  }
   __syncthreads();
   // the products of all the lane's blocks first (their values are then free),
   // then the row scans, which are independent of each other
