@@ -127,6 +127,34 @@ def test_high_valence_fan(nring):
     s.ctx.close()
 
 
+@pytest.mark.parametrize("nring", [40, 60])
+def test_fan_row_in_node_blocked_tiles(nring):
+    """A hub row of 36 (nring 40) or 56 (nring 60) lower node blocks still
+    fits a node-blocked tile row (at most 64), so the blocked product serves
+    it: its row scan sums a run that spans several 16-lane DPP rows and
+    whatever 64-block chunk boundary falls inside it.  The FP64 product of the
+    GPU's own K against the reference's matvec_csrcsym1 (Matvec.f:528-621)
+    within 1e-13; the solve as the oracle's PCG (iterations within 2, x within
+    1e-10; mixed x within 1e-8)."""
+    m = M.tria3_fan(nring)
+    print("fan", nring, flush=True)
+    s = System(m)
+    so = O.System(m)
+    assert np.array_equal(so.ia, s.ia) and np.array_equal(so.ja, s.ja)
+    assert s.ctx.layout_info()["nb"] == 2
+    ad, al = s.values()
+    v = 1.0 + (np.arange(1, s.neq + 1) % 7) * 0.125
+    assert rel(s.ctx.spmv(v), O.matvec(s.ia, s.ja, ad, al, v)) <= 1e-13
+    xo, sto = O.pcg(so.ia, so.ja, so.ad, so.al, so.b, tol=1e-10, maxit=50000)
+    assert sto["status"] == 0
+    x, st = s.solve(tol=1e-10, maxit=50000)
+    assert abs(st["iters"] - sto["iters"]) <= 2
+    assert rel(x, xo) <= 1e-10
+    xm, sm = s.solve(tol=1e-10, precision=MIXED, maxit=50000)
+    assert rel(xm, xo) <= 1e-8
+    s.ctx.close()
+
+
 def _flipped(m):
     """The tria3 square with every other quad split along its other diagonal
     (same nodes and edge count, so the same neq and nad; another pattern)."""
