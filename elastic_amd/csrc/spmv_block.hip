@@ -24,23 +24,24 @@ void k_spmv_blocks(BlkArgs a, const TV* __restrict__ vals, const TV* __restrict_
 }
 
 // *out = the n shares added in a fixed order (one workgroup of 1024 lanes):
-// lane l sums shares l, l + 1024, ... (eight loads in flight at a time, added
-// in index order); then a fixed tree over the lanes.
+// lane l sums shares l, l + 1024, ... (sixteen loads in flight at a time, so
+// up to 16,384 shares take one round trip; added in index order); then a
+// fixed tree over the lanes.
 __global__ __launch_bounds__(1024) void k_sum_shares(const double* __restrict__ part, int n,
                                                       double* out,
                                                       const int32_t* __restrict__ skip) {
   __shared__ double red[16];
   if (skip != nullptr && *skip != 0) return;
   double v = 0.0;
-  for (int k0 = threadIdx.x; k0 < n; k0 += 8 * 1024) {
-    double w[8];
+  for (int k0 = threadIdx.x; k0 < n; k0 += 16 * 1024) {
+    double w[16];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < 16; ++j) {
       const int k = k0 + 1024 * j;
       w[j] = k < n ? part[k] : 0.0;
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v += w[j];
+    for (int j = 0; j < 16; ++j) v += w[j];
   }
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
