@@ -166,7 +166,9 @@ struct TileRegs {
 // then issued by every lane, from a clamped index where the lane has nothing
 // to load (compute_tile does not use such a result), so the number of loads
 // in flight is the same on every path and each use waits only for the loads
-// it needs.
+// it needs.  GX also tells compute_tile that the workgroup serves this tile
+// only (the single-system kernel), so it skips the barriers that guard an
+// earlier tile's LDS.
 template <typename TV, int NB, int PER, int NT = 256, bool GX = false>
 __device__ __forceinline__ void load_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
                                           const TV* __restrict__ diag, TileRegs<TV, NB, PER>& R) {
@@ -241,8 +243,9 @@ struct NoHook {
 
 // Block sum of the lanes' shares of x.(A x) into part[t] (fixed order): each
 // wavefront sums by DPP moves (the steps of seg_scan without keys: lanes
-// without a source add zero), so lane 63 holds the wavefront's sum.
-template <int NT = 256>
+// without a source add zero), so lane 63 holds the wavefront's sum.  FRESH:
+// the workgroup serves this tile only, so red holds no earlier tile's sums.
+template <int NT = 256, bool FRESH = false>
 __device__ __forceinline__ void tile_dot_store(double* part, int t, double v) {
   __shared__ double red[NT / 64];
   v += dpp_move<0x111, 0xF>(v);
@@ -252,7 +255,7 @@ __device__ __forceinline__ void tile_dot_store(double* part, int t, double v) {
   v += dpp_move<0x142, 0xA>(v);
   v += dpp_move<0x143, 0xC>(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();  // red of a previous tile in this workgroup is free
+  if constexpr (!FRESH) __syncthreads();  // red of a previous tile is free
   if (lane == 63) red[wid] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -318,8 +321,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
         xv[k] = src[i];
         if constexpr (XD) dxv[k] = dx[i];
       }
-      hook();
-      __syncthreads();  // the LDS of a previous tile is free
+      hook();  // (GX: the workgroup's LDS holds no earlier tile)
 #pragma unroll
       for (int k = 0; k < kXR; ++k) {
         const int c = tid + NT * k;
@@ -494,7 +496,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
       for (int r = 0; r < NB; ++r) dacc += (double)s[r] * (double)xs[c * NB + r];
     }
   }
-  if constexpr (DOT) tile_dot_store<NT>(tdot, t, dacc);
+  if constexpr (DOT) tile_dot_store<NT, GX>(tdot, t, dacc);
 }
 
 // One tile: its loads, then its product.
