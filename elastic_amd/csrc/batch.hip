@@ -135,10 +135,11 @@ __global__ __launch_bounds__(256) void k_bt_spmv(const BInst* __restrict__ insts
   const bool run = f64 ? !sc->done : (!sc->done && !sc->op64);
   if (!run) return;
   // the FP32-sum variant stages x from the FP32 copy of p and, for instances
-  // with FP32 vectors, stores its own rows of q in FP32
-  compute_tile<TV, TA, NB, PER, true>(I.a, t, R, I.p, I.q, I.epart, smem, I.tdot,
-                                      (sizeof(TA) == 4 && I.p32) ? 1 : 0,
-                                      sizeof(TA) == 4 ? I.p32 : nullptr);
+  // with FP32 vectors, stores its own rows of q in FP32; one tile per
+  // workgroup (FRESH)
+  compute_tile<TV, TA, NB, PER, true, NoHook, 256, false, false, true>(
+      I.a, t, R, I.p, I.q, I.epart, smem, I.tdot, (sizeof(TA) == 4 && I.p32) ? 1 : 0,
+      sizeof(TA) == 4 ? I.p32 : nullptr);
 }
 
 // Device-built lists of mixed instances with FP64 work, so that a captured

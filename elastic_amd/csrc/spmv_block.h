@@ -166,9 +166,8 @@ struct TileRegs {
 // then issued by every lane, from a clamped index where the lane has nothing
 // to load (compute_tile does not use such a result), so the number of loads
 // in flight is the same on every path and each use waits only for the loads
-// it needs.  GX also tells compute_tile that the workgroup serves this tile
-// only (the single-system kernel), so it skips the barriers that guard an
-// earlier tile's LDS.
+// it needs.  (The single-system kernel uses GX; its workgroups serve one
+// tile each.)
 template <typename TV, int NB, int PER, int NT = 256, bool GX = false>
 __device__ __forceinline__ void load_tile(const BlkArgs& a, int t, const TV* __restrict__ vals,
                                           const TV* __restrict__ diag, TileRegs<TV, NB, PER>& R) {
@@ -236,7 +235,8 @@ inline size_t tile_smem(size_t t_size, int nb, int ncols_max, int nblk_max) {
 // the type of the staged x, the products and the sums (double, or float for
 // the FP32 operator with FP32 accumulation); x, y and the external partials
 // are FP64 in memory either way.  Starts with a barrier, so tiles can follow
-// each other in one workgroup.
+// each other in one workgroup; FRESH (a workgroup that serves this tile only)
+// leaves out the barriers that guard an earlier tile's LDS.
 struct NoHook {
   __device__ void operator()() const {}
 };
@@ -272,7 +272,7 @@ __device__ __forceinline__ void tile_dot_store(double* part, int t, double v) {
 // are added in tile order by a kernel of their own (k_sum_shares), so x.(A x)
 // needs no further pass over y and no floating-point atomics.
 template <typename TV, typename T, int NB, int PER, bool DOT = false, typename Hook = NoHook,
-          int NT = 256, bool XD = false, bool GX = false>
+          int NT = 256, bool XD = false, bool GX = false, bool FRESH = GX>
 __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
                                              const TileRegs<TV, NB, PER>& R,
                                              const double* __restrict__ x, double* __restrict__ y,
@@ -321,7 +321,8 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
         xv[k] = src[i];
         if constexpr (XD) dxv[k] = dx[i];
       }
-      hook();  // (GX: the workgroup's LDS holds no earlier tile)
+      hook();
+      if constexpr (!FRESH) __syncthreads();  // the LDS of a previous tile is free
 #pragma unroll
       for (int k = 0; k < kXR; ++k) {
         const int c = tid + NT * k;
@@ -349,7 +350,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
       }
     }
     hook();
-    __syncthreads();  // the LDS of a previous tile is free
+    if constexpr (!FRESH) __syncthreads();  // the LDS of a previous tile is free
 #pragma unroll
     for (int k = 0; k < XR; ++k) {
       const int c = tid + NT * k;
@@ -496,7 +497,7 @@ __device__ __forceinline__ void compute_tile(const BlkArgs& a, int t,
       for (int r = 0; r < NB; ++r) dacc += (double)s[r] * (double)xs[c * NB + r];
     }
   }
-  if constexpr (DOT) tile_dot_store<NT, GX>(tdot, t, dacc);
+  if constexpr (DOT) tile_dot_store<NT, FRESH>(tdot, t, dacc);
 }
 
 // One tile: its loads, then its product.
