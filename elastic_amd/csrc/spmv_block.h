@@ -9,8 +9,9 @@
 // node from the LDS-staged x.  Row pieces are summed across the wavefront by a
 // segmented scan on the row key (DPP lane moves), transposed products land in
 // LDS at their column-sorted position and each local column node is summed by
-// one lane in a fixed order: deterministic, no atomics.  Values are TV (double, or float for
-// the mixed-precision operator); staged x and sums are T.
+// one lane in a fixed order: deterministic, no atomics.  Values are TV
+// (double, or float for the mixed-precision operator); staged x and sums are
+// T.
 #pragma once
 #include <hip/hip_runtime.h>
 
